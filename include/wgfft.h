@@ -1,0 +1,160 @@
+/*
+ * wgfft.h -- C-ABI of the MI355X-native batched complex64 Stockham FFT.
+ *
+ * Drop-in boundary for wgls-rs-fft's GpuFft path.  Data is interleaved f32
+ * (re, im) -- the layout of num_complex::Complex<f32>, so a Rust
+ * Vec<Complex<f32>> (or a flattened batch of them) is passed as float*.
+ * Transform b of a batch starts at complex offset b * n.
+ *
+ * Conventions
+ *   forward  X_k = sum_n x_n exp(-2 pi i k n / N)           (unnormalised)
+ *   inverse  x_n = (1/N) sum_k X_k exp(+2 pi i k n / N)      (1/N scaled)
+ *   Return value: WGFFT_OK (0) or a negative WGFFT_ERR_* code; the message
+ *   of the last failure on the calling thread is wgfft_last_error().
+ *   Nothing aborts or throws across this boundary.
+ *   Host entry points are synchronous.  Device entry points take device
+ *   pointers (16-byte aligned) and a hipStream_t passed as void* (NULL = the
+ *   handle's own stream) and are stream-ordered: they return once the work
+ *   is enqueued.
+ *   A handle may be used by one thread at a time (calls are serialised by an
+ *   internal mutex); distinct handles are independent.
+ */
+#ifndef WGFFT_H
+#define WGFFT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wgfft_handle wgfft_t;
+typedef struct wgfft_ragged_plan wgfft_ragged_t;
+
+enum {
+    WGFFT_OK = 0,
+    WGFFT_ERR_INVALID_LENGTH = -1,  /* "Transform length must be a non-zero power of two" */
+    WGFFT_ERR_RAGGED_BATCH = -2,    /* "All input vectors in a batch must have the same length" */
+    WGFFT_ERR_NO_DEVICE = -3,       /* no HIP device / device unusable */
+    WGFFT_ERR_OUT_OF_MEMORY = -4,   /* device or host allocation failed or exceeded the cap */
+    WGFFT_ERR_HIP = -5,             /* a HIP runtime call failed */
+    WGFFT_ERR_INVALID_ARGUMENT = -6 /* null pointer, misaligned buffer, bad direction, ... */
+};
+
+#define WGFFT_FORWARD (-1)
+#define WGFFT_INVERSE (+1)
+
+/* ---- device and handle ------------------------------------------------ */
+
+/* 1 if a HIP device is usable, else 0.
+ * Replaces GpuFft::is_gpu_available (src/lib.rs:381-389). */
+int wgfft_device_available(void);
+
+/* Create a handle on HIP device `device` (its own stream, plan cache and
+ * scratch).  Fails with WGFFT_ERR_NO_DEVICE when there is none.
+ * Replaces GpuFft::new (src/lib.rs:285-330). */
+int wgfft_create(wgfft_t** out, int device);
+
+/* Release a handle and everything it owns (NULL is a no-op).
+ * Replaces the drop of GpuFft and its SizeCache map (src/lib.rs:75-82). */
+void wgfft_destroy(wgfft_t* h);
+
+/* Message of the last failure on this thread ("" if none).  Replaces the
+ * Box<dyn Error> strings of src/lib.rs:508,530. */
+const char* wgfft_last_error(void);
+
+/* Executor name.  Replaces FftExecutor::name (src/lib.rs:85-87). */
+const char* wgfft_name(void);
+
+/* ---- validation (pure host logic, no device needed) ------------------- */
+
+/* Validate a batch of `count` signal lengths the way GpuFft does before any
+ * device work: empty batch -> WGFFT_OK with *n_out = 0; every length equal
+ * to lens[0] else WGFFT_ERR_RAGGED_BATCH; each a non-zero power of two else
+ * WGFFT_ERR_INVALID_LENGTH (checked per signal in order, so the first
+ * offending signal decides the code).
+ * Replaces transform_batch_internal's checks (src/lib.rs:522-533) and
+ * validate_input_size (src/lib.rs:506-511). */
+int wgfft_validate_lengths(const uint64_t* lens, uint64_t count, uint64_t* n_out);
+
+/* ---- batched transforms ------------------------------------------------ */
+
+/* Batched FFT of host buffers: in/out hold batch*n complex64 (may alias).
+ * dir = WGFFT_FORWARD or WGFFT_INVERSE (inverse is 1/N scaled).  batch == 0
+ * is a no-op.  Replaces GpuFft::fft / GpuFft::ifft (src/lib.rs:433-438,
+ * 483-488) and transform_batch_internal (src/lib.rs:517-568). */
+int wgfft_exec_host(wgfft_t* h, const float* in, float* out, uint64_t n, uint64_t batch, int dir);
+
+/* Same on device buffers, enqueued on `stream`.  in == out is allowed.
+ * Replaces execute_compute_pass (src/lib.rs:582-628) with data resident. */
+int wgfft_exec_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, uint64_t batch, int dir,
+                      void* stream);
+
+/* Windowed forward Y_b = FFT(w * x_b), w real (n floats) shared by the batch,
+ * multiplied at load (no extra pass).  Replaces apply_hann_window + fft
+ * (tests/test_utils.rs:39-50, examples/windowed_fft.rs:14-23). */
+int wgfft_exec_windowed_host(wgfft_t* h, const float* x, const float* w, float* y, uint64_t n, uint64_t batch);
+int wgfft_exec_windowed_device(wgfft_t* h, const float* d_x, const float* d_w, float* d_y, uint64_t n,
+                               uint64_t batch, void* stream);
+
+/* ---- differentiation of Y = FFT(w * x) (no counterpart in the reference) */
+
+/* Vector-Jacobian product for a real loss, gradients in the
+ * (dL/dRe + i dL/dIm) convention:  gu = F^H gY,  gx = w * gu,
+ * gw_n = sum_b Re(conj(gu_bn) x_bn).  d_gw (n floats) is overwritten. */
+int wgfft_vjp_windowed_device(wgfft_t* h, const float* d_x, const float* d_w, const float* d_gy, float* d_gx,
+                              float* d_gw, uint64_t n, uint64_t batch, void* stream);
+
+/* Fused forward + adjoint for L = sum_b sum_k m_k |Y_bk|^2, Y = FFT(w * x):
+ * one pass over x writes gx (and Y when d_y != NULL), gw (n floats) and the
+ * scalar loss (d_loss, one float).  The mask m is real (n floats). */
+int wgfft_fwd_adjoint_masked_device(wgfft_t* h, const float* d_x, const float* d_w, const float* d_mask,
+                                    float* d_y, float* d_gx, float* d_gw, float* d_loss, uint64_t n,
+                                    uint64_t batch, void* stream);
+
+/* ---- mixed-length batches (no counterpart in the reference: it needs one
+ *      GpuFft call per length, src/lib.rs:527-531) ---------------------- */
+
+/* Plan `count` transforms of lengths 2^log2n[i] (6 <= log2n[i] <= 16) stored
+ * contiguously at complex offsets offsets[i] (host arrays; offsets even).
+ * The plan uploads a device work queue; it stays valid until destroyed. */
+int wgfft_ragged_create(wgfft_t* h, const uint64_t* offsets, const uint32_t* log2n, uint64_t count,
+                        wgfft_ragged_t** out);
+void wgfft_ragged_destroy(wgfft_ragged_t* p);
+
+/* One persistent launch over the whole queue (mode 0), the same launch
+ * captured once into a hipGraph and replayed (mode 1), or one launch per
+ * distinct length (mode 2, the comparison baseline). */
+int wgfft_ragged_exec_device(wgfft_ragged_t* p, const float* d_in, float* d_out, int dir, int mode, void* stream);
+
+/* ---- measurement and tuning ------------------------------------------- */
+
+/* Device-only timing of wgfft_exec_device on resident buffers: `warmup`
+ * synchronised calls, then `iters` back-to-back calls bracketed by HIP
+ * events on the handle's stream; *sec_per_iter = elapsed / iters.
+ * Replaces GpuFftTrait::benchmark_gpu_only (src/lib.rs:48-68,109-143). */
+int wgfft_bench_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, uint64_t batch, int dir,
+                       int warmup, int iters, double* sec_per_iter);
+
+/* Time every compiled kernel variant for length n on a scratch batch of
+ * `batch` transforms and make the fastest the handle's choice for n.
+ * Writes the chosen variant's description into desc (if non-NULL). */
+int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc, uint64_t desc_len);
+
+/* Description of the variant the handle currently uses for length n. */
+int wgfft_plan_describe(wgfft_t* h, uint64_t n, char* desc, uint64_t desc_len);
+
+/* Fill nfloats floats with i.i.d. U[-1,1) = (u32 >> 8) * 2^-23 - 1 drawn
+ * from PCG32 seeded through SplitMix64(seed) (input synthesis, host). */
+void wgfft_synth_uniform(float* out, uint64_t nfloats, uint64_t seed);
+
+/* Cap on the bytes a handle may hold in device scratch (0 = no cap); an
+ * allocation beyond it fails with WGFFT_ERR_OUT_OF_MEMORY.  Host-side knob
+ * for exercising the out-of-memory path. */
+void wgfft_set_alloc_cap(uint64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WGFFT_H */

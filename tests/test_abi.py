@@ -1,0 +1,98 @@
+"""C-ABI library checks that need no GPU: it loads, exports every symbol
+include/wgfft.h declares, and its host-side logic (validation, error codes
+and messages, input synthesis) behaves like the reference's GpuFft
+(src/lib.rs:506-533)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+import wgfft
+from wgfft import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "wgfft.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(wgfft_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    declared = header_functions()
+    assert len(declared) >= 18
+    for name in declared:
+        assert hasattr(L, name), name
+    assert sorted(_lib.EXPORTS) == declared
+
+
+def test_no_device_here_is_a_clean_error():
+    if wgfft.is_gpu_available():
+        pytest.skip("a device is present")
+    with pytest.raises(wgfft.WgfftError) as e:
+        wgfft.GpuFft.new()
+    assert e.value.code == wgfft.WGFFT_ERR_NO_DEVICE
+    L = _lib.load()
+    h = ctypes.c_void_p(123)
+    assert L.wgfft_create(ctypes.byref(h), 0) == wgfft.WGFFT_ERR_NO_DEVICE
+    assert h.value is None
+
+
+def test_null_handle_paths():
+    L = _lib.load()
+    assert L.wgfft_exec_host(None, None, None, 8, 1, -1) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
+    assert L.wgfft_exec_device(None, None, None, 8, 1, -1, None) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
+    assert "handle" in _lib.last_error()
+    L.wgfft_destroy(None)  # no-op
+
+
+def test_validation_contract():
+    assert wgfft.validate_lengths([]) == 0
+    assert wgfft.validate_lengths([1024] * 3) == 1024
+    assert wgfft.validate_lengths([1]) == 1
+    with pytest.raises(wgfft.WgfftError) as e:
+        wgfft.validate_lengths([1024, 512])
+    assert e.value.code == wgfft.WGFFT_ERR_RAGGED_BATCH
+    assert str(e.value) == "All input vectors in a batch must have the same length"
+    for bad in ([0], [3], [1000, 1000], [6]):
+        with pytest.raises(wgfft.WgfftError) as e:
+            wgfft.validate_lengths(bad)
+        assert e.value.code == wgfft.WGFFT_ERR_INVALID_LENGTH
+        assert str(e.value) == "Transform length must be a non-zero power of two"
+    # per-signal order of src/lib.rs:528-533: the first signal decides
+    with pytest.raises(wgfft.WgfftError) as e:
+        wgfft.validate_lengths([3, 4])
+    assert e.value.code == wgfft.WGFFT_ERR_INVALID_LENGTH
+    with pytest.raises(wgfft.WgfftError) as e:
+        wgfft.validate_lengths([4, 3])
+    assert e.value.code == wgfft.WGFFT_ERR_RAGGED_BATCH
+
+
+def test_synth_uniform_matches_oracle_stream():
+    for seed in (0, 1, 0x5EED0001):
+        assert np.array_equal(wgfft.synth_uniform((5, 33), seed), oracle.synth_uniform((5, 33), seed))
+
+
+def test_name_and_alloc_cap_knob():
+    L = _lib.load()
+    assert "Stockham" in L.wgfft_name().decode()
+    wgfft.set_alloc_cap(1 << 20)
+    wgfft.set_alloc_cap(0)
+
+
+def test_python_mirror_validates_before_device_work():
+    # the mirror validates through the C library, so these errors need no GPU
+    from wgfft.gpu_fft import _flatten
+
+    flat, n = _flatten([])
+    assert flat.shape == (0, 0) and n == 0
+    with pytest.raises(wgfft.WgfftError):
+        _flatten([np.zeros(4, np.complex64), np.zeros(8, np.complex64)])
+    with pytest.raises(wgfft.WgfftError):
+        _flatten([np.zeros(0, np.complex64)])

@@ -1,0 +1,271 @@
+"""Parity of the HIP path (through the C-ABI) with the oracle.
+
+Bars (written next to each check):
+  * FP64 oracle: relative L2 <= 5e-6 for N <= 2^20, <= 2e-5 above;
+  * reference restatement (tests/gpu_fft_test.rs:6): max |diff| < 1e-3 where
+    max|Y| * 1.2e-7 << 1e-3;
+  * round trips: 1e-3 (tests/ifft_test.rs:9), 1e-4 (src/lib.rs:225).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import wgfft
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = "tests/golden/fft_golden.npz"
+
+
+def rel_bar(n):
+    return 5e-6 if n <= (1 << 20) else 2e-5
+
+
+def check_fwd_inv(gpu, x):
+    batch, n = x.shape
+    y = gpu.fft_array(x)
+    want = oracle.fft64(x, -1)
+    e_f = oracle.rel_l2(y, want)
+    assert e_f <= rel_bar(n), (n, batch, e_f)
+    yi = gpu.fft_array(x, inverse=True)
+    e_i = oracle.rel_l2(yi, oracle.fft64(x, +1) / n)
+    assert e_i <= rel_bar(n), (n, batch, e_i)
+    return y
+
+
+# ----------------------------------------------------------- golden vectors
+@pytest.mark.parametrize("n", [1, 2, 4, 8, 16, 32, 64, 256, 1024, 4096])
+def test_golden_vectors(gpu, n):
+    g = np.load(GOLDEN, allow_pickle=False)
+    x = g[f"x_{n}"]
+    assert oracle.rel_l2(gpu.fft_array(x), g[f"fwd_{n}"]) <= 5e-6
+    if f"inv_{n}" in g:
+        assert oracle.rel_l2(gpu.fft_array(x, inverse=True), g[f"inv_{n}"]) <= 5e-6
+
+
+# --------------------------------------------------- every length, ragged batches
+@pytest.mark.parametrize("log2n", list(range(0, 17)))
+def test_all_lengths_vs_fp64_and_reference(gpu, log2n):
+    n = 1 << log2n
+    batch = max(1, min(37, (1 << 19) // n))  # odd batch exercises the group tail
+    x = oracle.synth_uniform((batch, n), 0x5EED0100 + log2n)
+    y = check_fwd_inv(gpu, x)
+    ref = oracle.ref_fft_f32(x)
+    assert np.abs(y - ref).max() < 1e-3  # reference restatement, README bar
+    single = gpu.fft_array(x[-1:])
+    assert np.array_equal(single[0], y[-1])  # batch output == single output
+
+
+@pytest.mark.parametrize("log2n,batch", [(17, 3), (18, 2), (20, 2), (22, 2)])
+def test_multipass_lengths(gpu, log2n, batch):
+    n = 1 << log2n
+    x = oracle.synth_uniform((batch, n), 0x5EED0200 + log2n)
+    check_fwd_inv(gpu, x)
+
+
+@pytest.mark.parametrize("n,batch", [(4096, 1), (4096, 2), (4096, 3), (1024, 9), (256, 33), (64, 129),
+                                     (32, 257), (8192, 3), (16384, 2)])
+def test_group_tails(gpu, n, batch):
+    x = oracle.synth_uniform((batch, n), 0x5EED0300 + batch)
+    check_fwd_inv(gpu, x)
+
+
+def test_known_answers(gpu):
+    for n in (16, 256, 4096, 1 << 15):
+        k = np.arange(n)
+        imp = np.zeros((1, n), np.complex64)
+        imp[0, 0] = 1
+        assert np.abs(gpu.fft_array(imp) - 1).max() < 1e-6
+        sh = np.zeros((1, n), np.complex64)
+        sh[0, 7 % n] = 1
+        assert np.abs(gpu.fft_array(sh)[0] - np.exp(-2j * np.pi * k * (7 % n) / n)).max() < 2e-6
+        e = np.exp(2j * np.pi * 5 * k / n).astype(np.complex64)[None]
+        want = np.zeros(n)
+        want[5 % n] = n
+        assert np.abs(gpu.fft_array(e)[0] - want).max() / n < 2e-6
+
+
+# ------------------------------------- the reference's tests, on the GPU path
+def make_input(n):
+    i = np.arange(n, dtype=np.float32)
+    return (np.sin(i * np.float32(0.1)) + 1j * np.cos(i * np.float32(0.07))).astype(np.complex64)
+
+
+def generate_test_signal(n):
+    t = np.arange(n, dtype=np.float32) / np.float32(n)
+    two_pi = np.float32(2.0 * np.pi)
+    s = np.float32(0.7) * np.sin(two_pi * np.float32(10.0) * t) + np.float32(0.3) * np.sin(
+        two_pi * np.float32(50.0) * t)
+    return s.astype(np.float32).astype(np.complex64)
+
+
+def hann(n):
+    i = np.arange(n, dtype=np.float32)
+    return (np.float32(0.5) * (np.float32(1.0) - np.cos(np.float32(2.0 * np.pi) * i / np.float32(n - 1)))).astype(
+        np.float32)
+
+
+def test_gpu_fft_matches_rustfft(gpu):
+    # tests/gpu_fft_test.rs:17-42
+    x = make_input(1024)
+    out = gpu.fft([x])
+    assert len(out) == 1 and out[0].shape == (1024,)
+    assert np.abs(out[0] - oracle.fft64(x)[0]).max() < 1e-3
+
+
+def test_ifft_roundtrip_and_properties(gpu):
+    # tests/ifft_test.rs:22-107
+    x = make_input(1024)
+    spec = gpu.fft([x])
+    rec = gpu.ifft(spec)
+    assert np.abs(rec[0] - x).max() < 1e-3
+    assert np.abs(rec[0] - oracle.ifft64(oracle.fft64(x))[0]).max() < 1e-3
+    spec2 = gpu.fft(rec)
+    assert np.abs(spec2[0] - spec[0]).max() < 1e-3
+
+
+def test_unit_roundtrip_consistency(gpu):
+    # src/lib.rs:207-227: ramp 0.1 i, N = 1024, error < 1e-4
+    x = (np.arange(1024, dtype=np.float32) * np.float32(0.1)).astype(np.complex64)
+    rec = gpu.ifft(gpu.fft([x]))[0]
+    assert np.abs(rec - x).max() < 1e-4
+
+
+def test_windowed_reference_tests(gpu):
+    # tests/windowed_fft_test.rs:10-133 at N = 2048, windowing on the CPU as
+    # the reference does, and through the fused windowed entry
+    n = 2048
+    sig = generate_test_signal(n)
+    win = (sig * hann(n)).astype(np.complex64)
+    spec = gpu.fft([win])[0]
+    assert np.abs(gpu.ifft([spec])[0] - win).max() < 1e-3
+    ratio = np.sum(np.abs(spec) ** 2) / np.sum(np.abs(gpu.fft([sig])[0]) ** 2)
+    assert 0.1 < ratio < 10.0
+    p = np.abs(spec) ** 2
+    peaks = [i for i in range(1, n // 2) if p[i] > p[i - 1] and p[i] > p[i + 1] and p[i] > 10.0]
+    assert any(8 <= f < 12 for f in peaks) and any(45 <= f < 55 for f in peaks)
+    fused = gpu.fft_windowed([sig], hann(n))[0]
+    assert np.abs(fused - spec).max() < 1e-3
+    assert oracle.rel_l2(fused, oracle.windowed_fwd64(sig[None], hann(n))[0]) < 5e-6
+
+
+def test_impulse_and_batch_semantics(gpu):
+    # examples/test_parallel.rs:90-128, examples/test_batch_processing.rs:130-211
+    n = 256
+    imp = np.zeros(n, np.complex64)
+    imp[0] = 1
+    assert np.abs(np.abs(gpu.fft([imp])[0]) - 1).max() < 1e-3
+    sigs = [oracle.synth_uniform((n,), 40 + i) for i in range(8)]
+    batch = gpu.fft(sigs)
+    assert len(batch) == 8
+    for i, s in enumerate(sigs):
+        assert np.abs(batch[i] - gpu.fft([s])[0]).max() < 1e-4
+    const = [np.full(n, 1.0 + 0j, np.complex64)] * 2
+    rt = gpu.ifft(gpu.fft(const))
+    assert np.abs(rt[0] - const[0]).max() < 1e-4
+
+
+def test_error_contract_on_device(gpu):
+    assert gpu.fft([]) == [] and gpu.ifft([]) == []
+    with pytest.raises(wgfft.WgfftError) as e:
+        gpu.fft([np.zeros(8, np.complex64), np.zeros(16, np.complex64)])
+    assert str(e.value) == "All input vectors in a batch must have the same length"
+    with pytest.raises(wgfft.WgfftError) as e:
+        gpu.fft([np.zeros(12, np.complex64)])
+    assert str(e.value) == "Transform length must be a non-zero power of two"
+    assert gpu.name()
+    assert wgfft.is_gpu_available()
+
+
+def test_alloc_cap_out_of_memory(gpu):
+    fresh = wgfft.GpuFft.new()
+    try:
+        wgfft.set_alloc_cap(1 << 16)
+        with pytest.raises(wgfft.WgfftError) as e:
+            fresh.fft_array(np.zeros((64, 1024), np.complex64))
+        assert e.value.code == wgfft.WGFFT_ERR_OUT_OF_MEMORY
+    finally:
+        wgfft.set_alloc_cap(0)
+        fresh.close()
+    x = oracle.synth_uniform((2, 1024), 3)
+    check_fwd_inv(gpu, x)
+
+
+# ------------------------------------------------ full-size configurations
+@pytest.mark.parametrize("n,batch", [(4096, 4096), (256, 65536), (16384, 1024)])
+def test_headline_shapes_vs_fp64(gpu, n, batch):
+    x = wgfft.synth_uniform((batch, n), 0x5EED0002)
+    y = gpu.fft_array(x)
+    err = oracle.rel_l2(y, oracle.fft64(x))
+    print(f"N={n} batch={batch} fwd rel-L2 vs FP64 = {err:.3e}")
+    assert err <= 5e-6
+    rt = gpu.fft_array(y, inverse=True)
+    assert oracle.rel_l2(rt, x) <= 5e-6
+
+
+def test_c1_forward_then_inverse(gpu):
+    x = wgfft.synth_uniform((8, 1024), 0x5EED0001)
+    y = gpu.fft_array(x)
+    assert oracle.rel_l2(y, oracle.fft64(x)) <= 5e-6
+    r = gpu.fft_array(y, inverse=True)
+    assert oracle.rel_l2(r, oracle.ifft64(y.astype(np.complex128))) <= 5e-6
+    assert np.abs(r - x).max() < 1e-4
+
+
+def test_c3_large_n_2pow22(gpu):
+    n, batch = 1 << 22, 16
+    x = oracle.synth_sinusoids(n, batch, 0x5EED0003)
+    y = gpu.fft_array(x)
+    err = oracle.rel_l2(y, oracle.fft64(x))
+    print(f"N=2^22 batch=16 fwd rel-L2 vs FP64 = {err:.3e}")
+    assert err <= 2e-5
+    r = gpu.fft_array(y, inverse=True)
+    assert oracle.rel_l2(r, x) <= 2e-5
+
+
+def test_c3_large_n_2pow26(gpu):
+    n = 1 << 26
+    x = wgfft.synth_uniform((1, n), 0x5EED0013)
+    y = gpu.fft_array(x)
+    bins = np.array([0, 1, 12345, n // 2, n - 1, 33554431 - 777], dtype=np.uint64)
+    want = oracle.dft_bins64(x[0], bins)
+    scale = np.sqrt(np.sum(np.abs(x.astype(np.complex128)) ** 2))  # |Y_k| ~ ||x||
+    assert np.abs(y[0, bins.astype(np.int64)] - want).max() / scale < 2e-5
+    # Parseval and round trip over the whole transform
+    py = np.sum(np.abs(y.astype(np.complex128)) ** 2)
+    assert abs(py / (n * scale**2) - 1) < 1e-5
+    r = gpu.fft_array(y, inverse=True)
+    assert oracle.rel_l2(r, x) <= 2e-5
+
+
+# ------------------------------------------------------- device entry points
+def test_device_api_inplace_and_stream(gpu, torch_cuda):
+    torch = torch_cuda
+    x = wgfft.synth_uniform((64, 4096), 9)
+    t = torch.from_numpy(x).cuda()
+    out = torch.empty_like(t)
+    gpu.exec_device(t, out)
+    torch.cuda.synchronize()
+    y = out.cpu().numpy()
+    assert oracle.rel_l2(y, oracle.fft64(x)) <= 5e-6
+    gpu.exec_device(t, t)  # in place
+    torch.cuda.synchronize()
+    assert np.array_equal(t.cpu().numpy(), y)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        z = torch.empty_like(out)
+        gpu.exec_device(out, z, inverse=True)
+    s.synchronize()
+    assert oracle.rel_l2(z.cpu().numpy(), x) <= 5e-6
+
+
+def test_windowed_device_c5_shape(gpu, torch_cuda):
+    torch = torch_cuda
+    n, batch = 8192, 1024
+    x = wgfft.synth_uniform((batch, n), 0x5EED0004)
+    w, _ = oracle.synth_window_mask(n, 0x5EED0014)
+    tx, tw = torch.from_numpy(x).cuda(), torch.from_numpy(w).cuda()
+    ty = torch.empty_like(tx)
+    gpu.exec_windowed_device(tx, tw, ty)
+    torch.cuda.synchronize()
+    assert oracle.rel_l2(ty.cpu().numpy(), oracle.windowed_fwd64(x, w)) <= 5e-6

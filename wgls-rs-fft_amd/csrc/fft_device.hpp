@@ -1,0 +1,372 @@
+// fft_device.hpp -- gfx950 device building blocks for the batched complex64
+// Stockham FFT: register DFTs (radix 2/4/8/16), twiddle generation and the
+// LDS-resident multi-pass Stockham engine shared by every kernel.
+//
+// Algorithm (the same decimation as the reference, wgls-rs-fft
+// src/shaders.rs:18-62, generalised from radix 4 to radix R): pass s with
+// stride L (product of the earlier radices) takes butterfly b, k = b mod L,
+// j = b / L, reads x_m = X[b + m N/R] (m < R), multiplies by W_{RL}^{m k},
+// does a length-R DFT and writes Y[j R L + k + m L].  The first pass reads
+// global memory and the last pass writes it; the passes between exchange
+// data through LDS.  Each thread owns E elements; the first and last pass
+// give thread t the element pairs (2t + 2T q, 2t + 1 + 2T q), so global
+// traffic is 16-byte vectors and a wavefront touches 1 KiB contiguous.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wgfft {
+
+// LDS barrier that does not wait for outstanding global loads: only LDS
+// traffic (lgkmcnt) is drained before s_barrier, so loads prefetched for the
+// next transform stay in flight across the exchange.
+#define WGFFT_LDS_SYNC() asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory")
+
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// a * W_4 with W_4 = -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 mul_w4(float2 a) {
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+template <bool INV>
+__device__ __forceinline__ float2 conj_if(float2 w) {
+    return INV ? make_float2(w.x, -w.y) : w;
+}
+
+constexpr float kR2 = 0.70710678118654752440f;  // 1/sqrt(2)
+constexpr float kC8 = 0.92387953251128675613f;  // cos(pi/8)
+constexpr float kS8 = 0.38268343236508977173f;  // sin(pi/8)
+
+// a * W_8^1, W_8 = exp(-+ 2 pi i / 8)
+template <bool INV>
+__device__ __forceinline__ float2 mul_w8_1(float2 a) {
+    return INV ? make_float2((a.x - a.y) * kR2, (a.x + a.y) * kR2)
+               : make_float2((a.x + a.y) * kR2, (a.y - a.x) * kR2);
+}
+// a * W_8^3
+template <bool INV>
+__device__ __forceinline__ float2 mul_w8_3(float2 a) {
+    return INV ? make_float2(-(a.x + a.y) * kR2, (a.x - a.y) * kR2)
+               : make_float2((a.y - a.x) * kR2, -(a.x + a.y) * kR2);
+}
+// a * (c - i s) forward, a * (c + i s) inverse, for constants c, s
+template <bool INV>
+__device__ __forceinline__ float2 mul_cs(float2 a, float c, float s) {
+    return INV ? make_float2(a.x * c - a.y * s, a.y * c + a.x * s)
+               : make_float2(a.x * c + a.y * s, a.y * c - a.x * s);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft2(float2& a, float2& b) {
+    float2 t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2& x0, float2& x1, float2& x2, float2& x3) {
+    float2 s02 = cadd(x0, x2), d02 = csub(x0, x2);
+    float2 s13 = cadd(x1, x3), d13 = mul_w4<INV>(csub(x1, x3));
+    x0 = cadd(s02, s13);
+    x2 = csub(s02, s13);
+    x1 = cadd(d02, d13);
+    x3 = csub(d02, d13);
+}
+
+// In-place DFT of v[0..R), natural order in and out.
+template <int R, bool INV>
+struct Dft;
+
+template <bool INV>
+struct Dft<2, INV> {
+    __device__ __forceinline__ static void run(float2 (&v)[2]) { dft2<INV>(v[0], v[1]); }
+};
+
+template <bool INV>
+struct Dft<4, INV> {
+    __device__ __forceinline__ static void run(float2 (&v)[4]) { dft4<INV>(v[0], v[1], v[2], v[3]); }
+};
+
+template <bool INV>
+struct Dft<8, INV> {
+    // 8 = 2 x 4: A_r = DFT4(v[r], v[r+2], v[r+4], v[r+6]); y[q] = A0 + W8^q A1,
+    // y[q+4] = A0 - W8^q A1.
+    __device__ __forceinline__ static void run(float2 (&v)[8]) {
+        dft4<INV>(v[0], v[2], v[4], v[6]);
+        dft4<INV>(v[1], v[3], v[5], v[7]);
+        float2 a0 = v[0], a1 = v[2], a2 = v[4], a3 = v[6];
+        float2 b0 = v[1], b1 = mul_w8_1<INV>(v[3]), b2 = mul_w4<INV>(v[5]), b3 = mul_w8_3<INV>(v[7]);
+        v[0] = cadd(a0, b0); v[4] = csub(a0, b0);
+        v[1] = cadd(a1, b1); v[5] = csub(a1, b1);
+        v[2] = cadd(a2, b2); v[6] = csub(a2, b2);
+        v[3] = cadd(a3, b3); v[7] = csub(a3, b3);
+    }
+};
+
+template <bool INV>
+struct Dft<16, INV> {
+    // 16 = 4 x 4: m = 4 m1 + m2, q = q1 + 4 q2.  DFT4 over m1 for each m2,
+    // multiply by W16^{m2 q1}, DFT4 over m2 for each q1.
+    __device__ __forceinline__ static void run(float2 (&v)[16]) {
+        dft4<INV>(v[0], v[4], v[8], v[12]);
+        dft4<INV>(v[1], v[5], v[9], v[13]);
+        dft4<INV>(v[2], v[6], v[10], v[14]);
+        dft4<INV>(v[3], v[7], v[11], v[15]);
+        // A_{m2}[q1] now sits in v[4 q1 + m2]
+        v[5] = mul_cs<INV>(v[5], kC8, kS8);    // q1=1 m2=1: W^1
+        v[6] = mul_w8_1<INV>(v[6]);            // q1=1 m2=2: W^2
+        v[7] = mul_cs<INV>(v[7], kS8, kC8);    // q1=1 m2=3: W^3
+        v[9] = mul_w8_1<INV>(v[9]);            // q1=2 m2=1: W^2
+        v[10] = mul_w4<INV>(v[10]);            // q1=2 m2=2: W^4
+        v[11] = mul_w8_3<INV>(v[11]);          // q1=2 m2=3: W^6
+        v[13] = mul_cs<INV>(v[13], kS8, kC8);  // q1=3 m2=1: W^3
+        v[14] = mul_w8_3<INV>(v[14]);          // q1=3 m2=2: W^6
+        v[15] = mul_cs<INV>(v[15], -kC8, -kS8);  // q1=3 m2=3: W^9 = -(c - i s)^... (cos 9pi/8, -sin 9pi/8)
+        dft4<INV>(v[0], v[1], v[2], v[3]);
+        dft4<INV>(v[4], v[5], v[6], v[7]);
+        dft4<INV>(v[8], v[9], v[10], v[11]);
+        dft4<INV>(v[12], v[13], v[14], v[15]);
+        // y[q1 + 4 q2] sits in v[4 q1 + q2]: transpose the 4x4 index.
+        float2 t[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) t[i] = v[i];
+#pragma unroll
+        for (int q1 = 0; q1 < 4; ++q1)
+#pragma unroll
+            for (int q2 = 0; q2 < 4; ++q2) v[q1 + 4 * q2] = t[4 * q1 + q2];
+    }
+};
+
+// v[m] *= W_{RL}^{m k} = W_N^{m k s}, with tw[j] = W_N^j (forward sign) and
+// s = N / (R L).  The table supplies W^{ks}, W^{2ks}, W^{4ks}, W^{8ks}; the
+// other powers are products of at most two table entries.
+template <int R, bool INV>
+__device__ __forceinline__ void apply_twiddles(float2 (&v)[R], const float2* __restrict__ tw,
+                                               uint32_t base) {
+    float2 w1 = conj_if<INV>(tw[base]);
+    if constexpr (R == 2) {
+        v[1] = cmul(v[1], w1);
+    } else {
+        float2 w2 = conj_if<INV>(tw[2 * base]);
+        float2 w3 = cmul(w1, w2);
+        v[1] = cmul(v[1], w1);
+        v[2] = cmul(v[2], w2);
+        v[3] = cmul(v[3], w3);
+        if constexpr (R >= 8) {
+            float2 w4 = conj_if<INV>(tw[4 * base]);
+            float2 w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
+            v[4] = cmul(v[4], w4);
+            v[5] = cmul(v[5], w5);
+            v[6] = cmul(v[6], w6);
+            v[7] = cmul(v[7], w7);
+            if constexpr (R == 16) {
+                float2 w8 = conj_if<INV>(tw[8 * base]);
+                v[8] = cmul(v[8], w8);
+                v[9] = cmul(v[9], cmul(w8, w1));
+                v[10] = cmul(v[10], cmul(w8, w2));
+                v[11] = cmul(v[11], cmul(w8, w3));
+                v[12] = cmul(v[12], cmul(w8, w4));
+                v[13] = cmul(v[13], cmul(w8, w5));
+                v[14] = cmul(v[14], cmul(w8, w6));
+                v[15] = cmul(v[15], cmul(w8, w7));
+            }
+        }
+    }
+}
+
+// LDS index padding for one exchange: phys = i + ((i >> A) << B).
+template <int A, int B>
+__device__ __forceinline__ uint32_t pad_idx(uint32_t i) {
+    return i + ((i >> A) << B);
+}
+
+// Padded LDS footprint (in complex elements) of one length-N transform for a
+// given exchange padding.
+constexpr int padded_len(int n, int a, int b) { return n + ((n >> a) << b); }
+
+// Compile-time radix list.
+template <int... Rs>
+struct Radices {};
+
+template <int... Rs>
+struct RadixInfo;
+template <int R>
+struct RadixInfo<R> {
+    static constexpr int product = R;
+    static constexpr int first = R;
+    static constexpr int last = R;
+    static constexpr int count = 1;
+};
+template <int R, int... Rest>
+struct RadixInfo<R, Rest...> {
+    static constexpr int product = R * RadixInfo<Rest...>::product;
+    static constexpr int first = R;
+    static constexpr int last = RadixInfo<Rest...>::last;
+    static constexpr int count = 1 + RadixInfo<Rest...>::count;
+};
+
+// Exchange padding written after a pass of radix R at stride L.
+//  first pass (pairs of butterflies per thread, output b R + m): lanes are
+//  2R apart, so one pad element per 2R staggers them over all banks;
+//  later passes with L < 32 write runs of L contiguous elements spaced R L
+//  apart, so a pad of L per R L staggers the runs;
+//  L >= 32 writes are already contiguous.
+template <int R, int L, bool FIRST>
+struct ExchangePad {
+    static constexpr int A = FIRST ? ilog2(2 * R) : (L < 32 ? ilog2(R * L) : 30);
+    static constexpr int B = FIRST ? 0 : (L < 32 ? ilog2(L) : 0);
+};
+
+template <int N, int E, int... Rs>
+struct LdsLayout;
+// Max padded length over the exchanges of a plan (the LDS footprint).
+template <int N, int E>
+struct LdsLayoutAcc {
+    template <int L, bool FIRST, int R, int... Rest>
+    static constexpr int max_len() {
+        if constexpr (sizeof...(Rest) == 0) {
+            return N;
+        } else {
+            using P = ExchangePad<R, L, FIRST>;
+            constexpr int here = padded_len(N, P::A, P::B);
+            constexpr int rest = max_len<L * R, false, Rest...>();
+            return here > rest ? here : rest;
+        }
+    }
+};
+
+template <int N, int E, int... Rs>
+struct LdsLayout {
+    static constexpr int raw = LdsLayoutAcc<N, E>::template max_len<1, true, Rs...>();
+    // keep every transform's LDS slot 16-byte aligned
+    static constexpr int len = (raw + 1) & ~1;
+};
+
+// Stockham pass engine.  d[] holds the thread's E elements; on entry (first
+// pass) and on exit (last pass) in the pair layout d[2q + delta] =
+// X[2t + delta + 2T q].
+template <int N, int E, int T, bool INV>
+struct Engine {
+    static_assert(N % E == 0, "E must divide N");
+    static_assert(T * E == N, "T * E must equal N");
+
+    template <int R>
+    static constexpr int pair_slots() { return E / (2 * R); }
+
+    template <int L, int PA, int PB, bool FIRST, int R, int... Rest>
+    __device__ __forceinline__ static void passes(float2 (&d)[E], float2* __restrict__ lds,
+                                                  const float2* __restrict__ tw, uint32_t t) {
+        constexpr bool LAST = sizeof...(Rest) == 0;
+        constexpr bool IO = FIRST || LAST;
+        constexpr int NR = N / R;
+        static_assert(E % R == 0, "radix must divide E");
+        static_assert(!IO || (E % (2 * R) == 0), "first/last radix must satisfy E >= 2R");
+        static_assert(!(FIRST && LAST), "a plan needs at least two passes");
+
+        if constexpr (!FIRST) {
+            // gather this pass's inputs from the exchange written by the previous pass
+            if constexpr (IO) {
+#pragma unroll
+                for (int q = 0; q < E / 2; ++q) {
+                    uint32_t i = 2 * t + 2 * T * q;
+                    if constexpr (PB >= 1) {
+                        float4 v = *reinterpret_cast<const float4*>(&lds[pad_idx<PA, PB>(i)]);
+                        d[2 * q] = make_float2(v.x, v.y);
+                        d[2 * q + 1] = make_float2(v.z, v.w);
+                    } else {
+                        d[2 * q] = lds[pad_idx<PA, PB>(i)];
+                        d[2 * q + 1] = lds[pad_idx<PA, PB>(i + 1)];
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < E / R; ++i) {
+                    uint32_t b = t + T * i;
+#pragma unroll
+                    for (int m = 0; m < R; ++m) d[i * R + m] = lds[pad_idx<PA, PB>(b + m * NR)];
+                }
+            }
+        }
+
+        // butterflies
+        if constexpr (IO) {
+#pragma unroll
+            for (int ip = 0; ip < pair_slots<R>(); ++ip) {
+#pragma unroll
+                for (int dl = 0; dl < 2; ++dl) {
+                    float2 v[R];
+#pragma unroll
+                    for (int m = 0; m < R; ++m) v[m] = d[2 * (ip + m * pair_slots<R>()) + dl];
+                    if constexpr (L > 1) {
+                        uint32_t b = 2 * t + dl + 2 * T * ip;
+                        uint32_t k = b & (L - 1);
+                        apply_twiddles<R, INV>(v, tw, k * (N / (R * L)));
+                    }
+                    Dft<R, INV>::run(v);
+#pragma unroll
+                    for (int m = 0; m < R; ++m) d[2 * (ip + m * pair_slots<R>()) + dl] = v[m];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < E / R; ++i) {
+                float2 v[R];
+#pragma unroll
+                for (int m = 0; m < R; ++m) v[m] = d[i * R + m];
+                uint32_t b = t + T * i;
+                uint32_t k = b & (L - 1);
+                apply_twiddles<R, INV>(v, tw, k * (N / (R * L)));
+                Dft<R, INV>::run(v);
+#pragma unroll
+                for (int m = 0; m < R; ++m) d[i * R + m] = v[m];
+            }
+        }
+
+        if constexpr (!LAST) {
+            using P = ExchangePad<R, L, FIRST>;
+            if constexpr (!FIRST) WGFFT_LDS_SYNC();  // everyone has read the previous exchange
+            if constexpr (IO) {
+#pragma unroll
+                for (int ip = 0; ip < pair_slots<R>(); ++ip)
+#pragma unroll
+                    for (int dl = 0; dl < 2; ++dl) {
+                        uint32_t b = 2 * t + dl + 2 * T * ip;
+                        uint32_t k = b & (L - 1), j = b / L;
+#pragma unroll
+                        for (int m = 0; m < R; ++m)
+                            lds[pad_idx<P::A, P::B>(j * R * L + k + m * L)] =
+                                d[2 * (ip + m * pair_slots<R>()) + dl];
+                    }
+            } else {
+#pragma unroll
+                for (int i = 0; i < E / R; ++i) {
+                    uint32_t b = t + T * i;
+                    uint32_t k = b & (L - 1), j = b / L;
+#pragma unroll
+                    for (int m = 0; m < R; ++m) lds[pad_idx<P::A, P::B>(j * R * L + k + m * L)] = d[i * R + m];
+                }
+            }
+            WGFFT_LDS_SYNC();
+            passes<L * R, P::A, P::B, false, Rest...>(d, lds, tw, t);
+        }
+    }
+
+    template <int... Rs>
+    __device__ __forceinline__ static void run(float2 (&d)[E], float2* __restrict__ lds,
+                                               const float2* __restrict__ tw, uint32_t t,
+                                               Radices<Rs...>) {
+        static_assert(RadixInfo<Rs...>::product == N, "radices must multiply to N");
+        passes<1, 30, 0, true, Rs...>(d, lds, tw, t);
+    }
+};
+
+}  // namespace wgfft

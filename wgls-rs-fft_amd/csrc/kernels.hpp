@@ -1,0 +1,350 @@
+// kernels.hpp -- kernel templates and the variant registry.
+//
+//  k_batched      one LDS-resident Stockham FFT per transform, G transforms per
+//                 workgroup, grid-stride over the batch with the next group's
+//                 input prefetched into registers (N = 32 .. 16384).
+//  k_small        one thread per transform, register DFT (N = 1 .. 16).
+//  k_global_pass  one pass of a multi-pass Stockham for N > 16384: a
+//                 workgroup runs G radix-R butterflies (R up to 8192) as an
+//                 LDS sub-FFT, twiddles W_{RL}^{mk} fused at load.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "fft_device.hpp"
+
+namespace wgfft {
+
+enum Mode : int { kForward = 0, kInverse = 1, kWindowed = 2 };
+
+struct BatchedArgs {
+    const float2* in;
+    float2* out;
+    const float2* tw;   // W_N^j, j < N
+    const float* win;   // real window (kWindowed)
+    uint64_t batch;
+    float scale;        // applied at store (1/N for the inverse)
+    int grid;           // workgroups to launch (0: one per group)
+    hipStream_t stream;
+};
+
+struct BatchedVariant {
+    int log2n;
+    int E, G, threads;
+    int lds_bytes;
+    const char* plan;  // radix split, e.g. "16x16x16"
+    hipError_t (*launch)(Mode, const BatchedArgs&);
+    int (*blocks_per_cu)(Mode);
+};
+
+struct GlobalPassArgs {
+    const float2* in;
+    float2* out;
+    const float2* tw_r;   // W_R^j, j < R (sub-FFT table)
+    const float2* tw_hi;  // W_N^{(j >> h) << h}
+    const float2* tw_lo;  // W_N^{j & (2^h - 1)}
+    const float* win;     // window for the first pass of the windowed forward
+    uint32_t log2n;       // full transform length N = 2^log2n
+    uint32_t h;           // split of the 2-level twiddle table
+    uint64_t L;           // stride of this pass (product of earlier radices)
+    uint64_t batch;
+    float scale;
+    hipStream_t stream;
+};
+
+struct GlobalPassVariant {
+    int log2r;
+    int E, G, threads;
+    int lds_bytes;
+    const char* plan;
+    hipError_t (*launch)(Mode, const GlobalPassArgs&);
+};
+
+std::vector<BatchedVariant>& batched_registry();
+std::vector<GlobalPassVariant>& global_pass_registry();
+
+struct Registrar {
+    explicit Registrar(const BatchedVariant& v) { batched_registry().push_back(v); }
+    explicit Registrar(const GlobalPassVariant& v) { global_pass_registry().push_back(v); }
+};
+
+hipError_t launch_small(Mode mode, int log2n, const BatchedArgs& a);
+
+// ---------------------------------------------------------------------------
+// Global I/O in the pair layout: d[2q + delta] = X[2t + delta + 2T q].
+template <int N, int E, int T>
+__device__ __forceinline__ void load_pairs(float2 (&d)[E], const float2* __restrict__ src, uint32_t t) {
+    const float4* s4 = reinterpret_cast<const float4*>(src);
+#pragma unroll
+    for (int q = 0; q < E / 2; ++q) {
+        float4 v = s4[t + T * q];
+        d[2 * q] = make_float2(v.x, v.y);
+        d[2 * q + 1] = make_float2(v.z, v.w);
+    }
+}
+
+template <int N, int E, int T>
+__device__ __forceinline__ void window_pairs(float2 (&d)[E], const float* __restrict__ win, uint32_t t) {
+    const float2* w2 = reinterpret_cast<const float2*>(win);
+#pragma unroll
+    for (int q = 0; q < E / 2; ++q) {
+        float2 w = w2[t + T * q];
+        d[2 * q] = cscale(d[2 * q], w.x);
+        d[2 * q + 1] = cscale(d[2 * q + 1], w.y);
+    }
+}
+
+template <int N, int E, int T, bool SCALE>
+__device__ __forceinline__ void store_pairs(float2* __restrict__ dst, const float2 (&d)[E], uint32_t t,
+                                            float scale) {
+    float4* d4 = reinterpret_cast<float4*>(dst);
+#pragma unroll
+    for (int q = 0; q < E / 2; ++q) {
+        float2 a = d[2 * q], b = d[2 * q + 1];
+        if constexpr (SCALE) {
+            a = cscale(a, scale);
+            b = cscale(b, scale);
+        }
+        d4[t + T * q] = make_float4(a.x, a.y, b.x, b.y);
+    }
+}
+
+// ---------------------------------------------------------------------------
+template <int N, int E, int G, int MODE, int... Rs>
+__global__ void __launch_bounds__(G*(N / E))
+k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
+          const float* __restrict__ win, uint64_t batch, float scale) {
+    constexpr int T = N / E;
+    constexpr bool INV = MODE == kInverse;
+    constexpr int LEN = LdsLayout<N, E, Rs...>::len;
+    __shared__ __align__(16) float2 lds_all[G * LEN];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t g = tid / T, t = tid % T;
+    float2* lds = lds_all + g * LEN;
+    const uint64_t ngroups = (batch + G - 1) / G;
+
+    uint64_t grp = blockIdx.x;
+    float2 nxt[E];
+    if (grp < ngroups && grp * G + g < batch) load_pairs<N, E, T>(nxt, in + (grp * G + g) * N, t);
+
+    for (; grp < ngroups; grp += gridDim.x) {
+        const uint64_t tr = grp * G + g;
+        const bool active = tr < batch;
+        float2 d[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i) d[i] = nxt[i];
+        const uint64_t ngrp = grp + gridDim.x;
+        if (ngrp < ngroups && ngrp * G + g < batch) load_pairs<N, E, T>(nxt, in + (ngrp * G + g) * N, t);
+        if constexpr (MODE == kWindowed) window_pairs<N, E, T>(d, win, t);
+        Engine<N, E, T, INV>::run(d, lds, tw, t, Radices<Rs...>{});
+        if (active) store_pairs<N, E, T, INV>(out + tr * N, d, t, scale);
+        WGFFT_LDS_SYNC();  // the next group's first pass overwrites the exchange buffer
+    }
+}
+
+template <int N, int E, int G, int... Rs>
+struct BatchedKernel {
+    static constexpr int T = N / E;
+    static constexpr int THREADS = G * T;
+    static constexpr int LDS_BYTES = G * LdsLayout<N, E, Rs...>::len * 8;
+
+    template <int MODE>
+    static hipError_t go(const BatchedArgs& a) {
+        const uint64_t ngroups = (a.batch + G - 1) / G;
+        uint64_t grid = a.grid > 0 ? (uint64_t)a.grid : ngroups;
+        if (grid > ngroups) grid = ngroups;
+        if (grid == 0) return hipSuccess;
+        hipLaunchKernelGGL((k_batched<N, E, G, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0,
+                           a.stream, a.in, a.out, a.tw, a.win, a.batch, a.scale);
+        return hipGetLastError();
+    }
+    static hipError_t launch(Mode m, const BatchedArgs& a) {
+        switch (m) {
+            case kForward: return go<kForward>(a);
+            case kInverse: return go<kInverse>(a);
+            default: return go<kWindowed>(a);
+        }
+    }
+    template <int MODE>
+    static int occ() {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_batched<N, E, G, MODE, Rs...>,
+                                                         THREADS, 0) != hipSuccess)
+            return 1;
+        return blocks > 0 ? blocks : 1;
+    }
+    static int blocks_per_cu(Mode m) {
+        switch (m) {
+            case kForward: return occ<kForward>();
+            case kInverse: return occ<kInverse>();
+            default: return occ<kWindowed>();
+        }
+    }
+};
+
+#define WGFFT_REGISTER_BATCHED(N, E, G, PLAN, ...)                                                      \
+    static ::wgfft::Registrar _wgfft_reg_##N##_##E##_##G##_##PLAN(::wgfft::BatchedVariant{                                     \
+        ::wgfft::ilog2(N), E, G, ::wgfft::BatchedKernel<N, E, G, __VA_ARGS__>::THREADS,                  \
+        ::wgfft::BatchedKernel<N, E, G, __VA_ARGS__>::LDS_BYTES, #PLAN,                                 \
+        &::wgfft::BatchedKernel<N, E, G, __VA_ARGS__>::launch,                                          \
+        &::wgfft::BatchedKernel<N, E, G, __VA_ARGS__>::blocks_per_cu})
+
+// LDS slot (complex elements) per butterfly of a global pass: the larger of
+// the sub-FFT's exchange footprint and the staging layout (R + R/32), kept
+// 4 mod 32 so adjacent slots start on different banks.
+template <int R, int E, int... Rs>
+struct GlobalPassSlot {
+    static constexpr int a = LdsLayout<R, E, Rs...>::len;
+    static constexpr int b = R + (R >> 5);
+    static constexpr int m = a > b ? a : b;
+    static constexpr int len = m + ((4 - m % 32) + 32) % 32;
+};
+
+// ---------------------------------------------------------------------------
+// Two-level twiddle: W_N^j = W_N^{(j >> h) << h} * W_N^{j & (2^h - 1)}.
+template <bool INV>
+__device__ __forceinline__ float2 tw2(const float2* __restrict__ hi, const float2* __restrict__ lo, uint64_t j,
+                                      uint32_t h) {
+    float2 w = cmul(hi[j >> h], lo[j & ((1ull << h) - 1)]);
+    return conj_if<INV>(w);
+}
+
+// One multi-pass Stockham pass over a length-N transform with radix R done as
+// an LDS sub-FFT.  The workgroup handles butterflies b0 .. b0+G-1 of one
+// transform (G <= L, so both the strided loads and the strided stores are
+// runs of G contiguous elements); when L == 1 each butterfly's R outputs are
+// contiguous and are stored straight from registers.
+template <int R, int E, int G, int MODE, int... Rs>
+__global__ void __launch_bounds__(G*(R / E))
+k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw_r,
+              const float2* __restrict__ tw_hi, const float2* __restrict__ tw_lo,
+              const float* __restrict__ win, uint32_t log2n, uint32_t h, uint64_t L, uint64_t batch,
+              float scale) {
+    constexpr int T = R / E;
+    constexpr int THREADS = G * T;
+    constexpr bool INV = MODE == kInverse;
+    constexpr int SLOT = GlobalPassSlot<R, E, Rs...>::len;
+    __shared__ __align__(16) float2 lds_all[G * SLOT];
+
+    const uint64_t N = 1ull << log2n;
+    const uint64_t NR = N >> ilog2(R);  // butterflies per transform
+    const uint64_t groups_per_tr = NR / G;
+    const uint64_t total = groups_per_tr * batch;
+
+    // XCD-aware remap: block ids are dealt round-robin over the 8 XCDs; map
+    // consecutive logical groups onto one XCD so neighbouring column runs
+    // share that XCD's L2 lines.
+    const uint64_t pid = blockIdx.x;
+    const uint64_t nb = gridDim.x;
+    uint64_t lid = pid;
+    if ((nb & 7) == 0) lid = (pid & 7) * (nb >> 3) + (pid >> 3);
+    if (lid >= total) return;
+    const uint64_t tr = lid / groups_per_tr;
+    const uint64_t b0 = (lid % groups_per_tr) * G;
+    const float2* src = in + tr * N;
+    float2* dst = out + tr * N;
+
+    const uint32_t tid = threadIdx.x;
+    const uint64_t s = N / ((uint64_t)R * L);  // W_{RL} = W_N^s
+    const bool first = (L == 1);
+    const bool last = (L * R == N);
+
+    // staged load of the R x G tile (rows m, adjacent butterfly pairs per lane)
+    constexpr int PAIRS = R * (G / 2);
+    for (int it = tid; it < PAIRS; it += THREADS) {
+        const int gp = it % (G / 2);
+        const int m = it / (G / 2);
+        const uint64_t e = b0 + 2 * gp + (uint64_t)m * NR;
+        float4 v = *reinterpret_cast<const float4*>(src + e);
+        float2 a = make_float2(v.x, v.y), b = make_float2(v.z, v.w);
+        if constexpr (MODE == kWindowed) {
+            if (first) {
+                a = cscale(a, win[e]);
+                b = cscale(b, win[e + 1]);
+            }
+        }
+        if (!first) {
+            const uint64_t k0 = (b0 + 2 * gp) & (L - 1);
+            a = cmul(a, tw2<INV>(tw_hi, tw_lo, ((uint64_t)m * k0 * s) & (N - 1), h));
+            b = cmul(b, tw2<INV>(tw_hi, tw_lo, ((uint64_t)m * (k0 + 1) * s) & (N - 1), h));
+        }
+        const uint32_t pm = pad_idx<5, 0>(m);
+        lds_all[(2 * gp) * SLOT + pm] = a;
+        lds_all[(2 * gp + 1) * SLOT + pm] = b;
+    }
+    WGFFT_LDS_SYNC();
+
+    const uint32_t g = tid / T, t = tid % T;
+    float2* lds = lds_all + g * SLOT;
+    float2 d[E];
+#pragma unroll
+    for (int q = 0; q < E / 2; ++q) {
+        d[2 * q] = lds[pad_idx<5, 0>(2 * t + 2 * T * q)];
+        d[2 * q + 1] = lds[pad_idx<5, 0>(2 * t + 1 + 2 * T * q)];
+    }
+    WGFFT_LDS_SYNC();
+    Engine<R, E, T, INV>::run(d, lds, tw_r, t, Radices<Rs...>{});
+    const float sc = last ? scale : 1.0f;
+
+    if (first) {
+        // output b R + m: R contiguous elements per butterfly
+        store_pairs<R, E, T, true>(dst + (b0 + g) * R, d, t, sc);
+        return;
+    }
+    WGFFT_LDS_SYNC();
+#pragma unroll
+    for (int q = 0; q < E / 2; ++q) {
+        lds[pad_idx<5, 0>(2 * t + 2 * T * q)] = d[2 * q];
+        lds[pad_idx<5, 0>(2 * t + 1 + 2 * T * q)] = d[2 * q + 1];
+    }
+    WGFFT_LDS_SYNC();
+    const uint64_t bj = b0 / L, k0base = b0 & (L - 1);
+    for (int it = tid; it < PAIRS; it += THREADS) {
+        const int gp = it % (G / 2);
+        const int m = it / (G / 2);
+        const uint32_t pm = pad_idx<5, 0>(m);
+        float2 a = cscale(lds_all[(2 * gp) * SLOT + pm], sc);
+        float2 b = cscale(lds_all[(2 * gp + 1) * SLOT + pm], sc);
+        const uint64_t o = bj * R * L + k0base + 2 * gp + (uint64_t)m * L;
+        *reinterpret_cast<float4*>(dst + o) = make_float4(a.x, a.y, b.x, b.y);
+    }
+}
+
+template <int R, int E, int G, int... Rs>
+struct GlobalPassKernel {
+    static constexpr int T = R / E;
+    static constexpr int THREADS = G * T;
+    static constexpr int LDS_BYTES = G * GlobalPassSlot<R, E, Rs...>::len * 8;
+
+    template <int MODE>
+    static hipError_t go(const GlobalPassArgs& a) {
+        const uint64_t N = 1ull << a.log2n;
+        const uint64_t groups = (N / R) / G * a.batch;
+        if (groups == 0) return hipSuccess;
+        uint64_t grid = groups;
+        if (grid % 8) grid += 8 - grid % 8;  // keep the XCD remap a bijection; extra blocks exit
+        hipLaunchKernelGGL((k_global_pass<R, E, G, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0,
+                           a.stream, a.in, a.out, a.tw_r, a.tw_hi, a.tw_lo, a.win, a.log2n, a.h, a.L,
+                           a.batch, a.scale);
+        return hipGetLastError();
+    }
+    static hipError_t launch(Mode m, const GlobalPassArgs& a) {
+        switch (m) {
+            case kForward: return go<kForward>(a);
+            case kInverse: return go<kInverse>(a);
+            default: return go<kWindowed>(a);
+        }
+    }
+};
+
+#define WGFFT_REGISTER_GLOBAL_PASS(R, E, G, PLAN, ...)                                                  \
+    static ::wgfft::Registrar _wgfft_gp_##R##_##E##_##G##_##PLAN(::wgfft::GlobalPassVariant{           \
+        ::wgfft::ilog2(R), E, G, ::wgfft::GlobalPassKernel<R, E, G, __VA_ARGS__>::THREADS,               \
+        ::wgfft::GlobalPassKernel<R, E, G, __VA_ARGS__>::LDS_BYTES, #PLAN,                              \
+        &::wgfft::GlobalPassKernel<R, E, G, __VA_ARGS__>::launch})
+
+}  // namespace wgfft

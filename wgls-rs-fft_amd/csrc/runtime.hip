@@ -1,0 +1,504 @@
+// runtime.hip -- host runtime behind include/wgfft.h: handles, per-length
+// plans (kernel variant + twiddle tables), device scratch, validation and the
+// C-ABI entry points.  Mirrors wgls-rs-fft's GpuFft runtime (src/lib.rs):
+// the per-N SizeCache becomes a Plan, the ping-pong buffers become either one
+// LDS-resident launch (N <= 16384) or a chain of global passes, and the
+// conjugate-and-scale inverse of src/lib.rs:571-579,661-669 is done in-kernel
+// with conjugated twiddles and a 1/N scale at the final store.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/wgfft.h"
+#include "kernels.hpp"
+#include "runtime_internal.hpp"
+
+namespace wgfft {
+
+std::vector<BatchedVariant>& batched_registry() {
+    static std::vector<BatchedVariant> v;
+    return v;
+}
+std::vector<GlobalPassVariant>& global_pass_registry() {
+    static std::vector<GlobalPassVariant> v;
+    return v;
+}
+
+thread_local std::string g_last_error;
+static std::atomic<uint64_t> g_alloc_cap{0};
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(WGFFT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+const char* kMsgLength = "Transform length must be a non-zero power of two";
+const char* kMsgRagged = "All input vectors in a batch must have the same length";
+
+static bool is_pow2(uint64_t n) { return n != 0 && (n & (n - 1)) == 0; }
+static int log2u(uint64_t n) {
+    int l = 0;
+    while ((1ull << l) < n) ++l;
+    return l;
+}
+
+int Handle::ensure(DevBuf& b, uint64_t bytes) {
+    if (b.bytes >= bytes && b.p) return WGFFT_OK;
+    uint64_t cap = g_alloc_cap.load();
+    uint64_t after = held - b.bytes + bytes;
+    if (cap && after > cap)
+        return fail(WGFFT_ERR_OUT_OF_MEMORY, "device scratch of " + std::to_string(after) +
+                                                 " bytes exceeds the allocation cap of " + std::to_string(cap));
+    if (b.p) {
+        hipFree(b.p);
+        held -= b.bytes;
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(WGFFT_ERR_OUT_OF_MEMORY, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+    }
+    b.p = p;
+    b.bytes = bytes;
+    held += bytes;
+    return WGFFT_OK;
+}
+
+void Handle::release(DevBuf& b) {
+    if (b.p) {
+        hipFree(b.p);
+        held -= b.bytes;
+    }
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+// W_N^j = exp(-2 pi i j / N) for j < count, FP64 then rounded to f32.
+static std::vector<float2> twiddle_table(uint64_t N, uint64_t count, uint64_t step) {
+    std::vector<float2> t(count);
+    for (uint64_t j = 0; j < count; ++j) {
+        uint64_t e = (j * step) % N;
+        double a = 2.0 * M_PI * ((double)e / (double)N);
+        t[j] = make_float2((float)std::cos(a), (float)-std::sin(a));
+    }
+    return t;
+}
+
+int Handle::upload_table(const std::vector<float2>& host, float2** dev) {
+    void* p = nullptr;
+    uint64_t bytes = host.size() * sizeof(float2);
+    uint64_t cap = g_alloc_cap.load();
+    if (cap && held + bytes > cap) return fail(WGFFT_ERR_OUT_OF_MEMORY, "twiddle table exceeds the allocation cap");
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(WGFFT_ERR_OUT_OF_MEMORY, std::string("hipMalloc(twiddles) failed: ") + hipGetErrorString(e));
+    }
+    held += bytes;
+    e = hipMemcpy(p, host.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(p);
+        held -= bytes;
+        return hip_fail(e, "hipMemcpy(twiddles)");
+    }
+    tables.push_back({p, bytes});
+    *dev = (float2*)p;
+    return WGFFT_OK;
+}
+
+// Default variant per length: the first registered candidate for log2n (the
+// registration order lists the preferred split first); an entry made by
+// wgfft_autotune or read from the tune file overrides it.
+const BatchedVariant* default_variant(int log2n) {
+    for (const auto& v : batched_registry())
+        if (v.log2n == log2n) return &v;
+    return nullptr;
+}
+
+const BatchedVariant* find_variant(int log2n, const char* plan, int E, int G) {
+    for (const auto& v : batched_registry())
+        if (v.log2n == log2n && v.E == E && v.G == G && std::strcmp(v.plan, plan) == 0) return &v;
+    return nullptr;
+}
+
+static const GlobalPassVariant* pick_global_pass(int log2r, uint64_t L, uint64_t nr) {
+    const GlobalPassVariant* best = nullptr;
+    for (const auto& v : global_pass_registry()) {
+        if (v.log2r != log2r) continue;
+        if ((uint64_t)v.G > nr) continue;
+        if (L > 1 && (uint64_t)v.G > L) continue;
+        if (!best || v.G > best->G) best = &v;
+    }
+    return best;
+}
+
+int Handle::get_plan(uint64_t n, Plan** out) {
+    auto it = plans.find(n);
+    if (it != plans.end()) {
+        *out = &it->second;
+        return WGFFT_OK;
+    }
+    Plan p;
+    p.n = n;
+    p.log2n = log2u(n);
+    int rc;
+    if (p.log2n <= 4) {
+        p.kind = Plan::kSmall;
+    } else if (p.log2n <= 14) {
+        p.kind = Plan::kBatched;
+        p.var = tuned.count(p.log2n) ? tuned[p.log2n] : default_variant(p.log2n);
+        if (!p.var) return fail(WGFFT_ERR_INVALID_LENGTH, "no kernel variant for this length");
+        if ((rc = upload_table(twiddle_table(n, n, 1), &p.tw))) return rc;
+    } else {
+        if (p.log2n > 39) return fail(WGFFT_ERR_INVALID_LENGTH, "transform length above 2^39 is not supported");
+        p.kind = Plan::kMultiPass;
+        int passes = (p.log2n + 12) / 13;
+        int rem = p.log2n;
+        uint64_t L = 1;
+        for (int i = 0; i < passes; ++i) {
+            int r = (rem + (passes - i) - 1) / (passes - i);  // larger radices first
+            rem -= r;
+            const GlobalPassVariant* v = pick_global_pass(r, L, n >> r);
+            if (!v) return fail(WGFFT_ERR_INVALID_LENGTH, "no global-pass variant for radix 2^" + std::to_string(r));
+            p.pass_log2r.push_back(r);
+            p.pass_var.push_back(v);
+            if (!p.tw_r.count(r)) {
+                float2* t = nullptr;
+                if ((rc = upload_table(twiddle_table(1ull << r, 1ull << r, 1), &t))) return rc;
+                p.tw_r[r] = t;
+            }
+            L <<= r;
+        }
+        p.h = (uint32_t)((p.log2n + 1) / 2);
+        uint64_t lo_n = 1ull << p.h, hi_n = 1ull << (p.log2n - p.h);
+        if ((rc = upload_table(twiddle_table(n, lo_n, 1), &p.tw_lo))) return rc;
+        if ((rc = upload_table(twiddle_table(n, hi_n, lo_n), &p.tw_hi))) return rc;
+    }
+    auto res = plans.emplace(n, p);
+    *out = &res.first->second;
+    return WGFFT_OK;
+}
+
+int Handle::blocks_per_cu(const BatchedVariant* v, Mode m) {
+    auto key = std::make_pair(v, (int)m);
+    auto it = occ_cache.find(key);
+    if (it != occ_cache.end()) return it->second;
+    int b = v->blocks_per_cu(m);
+    occ_cache[key] = b;
+    return b;
+}
+
+// Enqueue one batched transform of `batch` length-n signals.
+int Handle::run(Plan* p, Mode mode, const float2* in, float2* out, const float* win, uint64_t batch,
+                float scale, hipStream_t s, const BatchedVariant* force) {
+    if (batch == 0) return WGFFT_OK;
+    hipError_t e = hipSuccess;
+    if (p->kind == Plan::kSmall) {
+        BatchedArgs a{in, out, nullptr, win, batch, scale, 0, s};
+        e = launch_small(mode, p->log2n, a);
+        if (e != hipSuccess) return hip_fail(e, "small-length kernel launch");
+        return WGFFT_OK;
+    }
+    if (p->kind == Plan::kBatched) {
+        const BatchedVariant* v = force ? force : p->var;
+        int grid = blocks_per_cu(v, mode) * num_cus;
+        BatchedArgs a{in, out, p->tw, win, batch, scale, grid, s};
+        e = v->launch(mode, a);
+        if (e != hipSuccess) return hip_fail(e, "batched kernel launch");
+        return WGFFT_OK;
+    }
+    // multi-pass: chunk the batch so the intermediate stays modest (and, at
+    // 2^22, inside the 256 MB last-level cache between the two passes)
+    const uint64_t tbytes = p->n * sizeof(float2);
+    uint64_t chunk = std::max<uint64_t>(1, mp_chunk_bytes / tbytes);
+    chunk = std::min(chunk, batch);
+    const int P = (int)p->pass_var.size();
+    const bool inplace = (const void*)in == (const void*)out;
+    int rc;
+    if ((rc = ensure(scratch, chunk * tbytes))) return rc;
+    if ((P % 2 == 1 || inplace) && P > 1) {
+        if ((rc = ensure(scratch2, chunk * tbytes))) return rc;
+    }
+    for (uint64_t b0 = 0; b0 < batch; b0 += chunk) {
+        uint64_t nb = std::min(chunk, batch - b0);
+        const float2* src = in + b0 * p->n;
+        float2* dst_final = out + b0 * p->n;
+        uint64_t L = 1;
+        for (int i = 0; i < P; ++i) {
+            float2* dst;
+            if (i == P - 1) {
+                dst = dst_final;
+            } else {
+                // alternate scratch buffers; never write the buffer being read
+                float2* a = (float2*)scratch.p;
+                float2* b = (float2*)(scratch2.p ? scratch2.p : scratch.p);
+                dst = (i % 2 == 0) ? a : b;
+                if ((const void*)dst == (const void*)src) dst = (dst == a) ? b : a;
+            }
+            const GlobalPassVariant* v = p->pass_var[i];
+            GlobalPassArgs a{src,     dst,     p->tw_r[p->pass_log2r[i]], p->tw_hi, p->tw_lo, win,
+                             (uint32_t)p->log2n, p->h, L, nb, (i == P - 1) ? scale : 1.0f, s};
+            Mode m = mode;
+            if (mode == kWindowed && i > 0) m = kForward;
+            e = v->launch(m, a);
+            if (e != hipSuccess) return hip_fail(e, "global pass launch");
+            src = dst;
+            L <<= p->pass_log2r[i];
+        }
+    }
+    return WGFFT_OK;
+}
+
+}  // namespace wgfft
+
+using namespace wgfft;
+
+static bool g_device_probed = false;
+static int g_device_count = 0;
+static std::mutex g_probe_mu;
+
+static int device_count() {
+    std::lock_guard<std::mutex> lk(g_probe_mu);
+    if (!g_device_probed) {
+        int c = 0;
+        if (hipGetDeviceCount(&c) != hipSuccess) {
+            (void)hipGetLastError();
+            c = 0;
+        }
+        g_device_count = c;
+        g_device_probed = true;
+    }
+    return g_device_count;
+}
+
+static hipStream_t pick_stream(wgfft_t* h, void* stream) {
+    return stream ? (hipStream_t)stream : h->stream;
+}
+
+static int check_dir(int dir) {
+    if (dir != WGFFT_FORWARD && dir != WGFFT_INVERSE)
+        return fail(WGFFT_ERR_INVALID_ARGUMENT, "direction must be WGFFT_FORWARD (-1) or WGFFT_INVERSE (+1)");
+    return WGFFT_OK;
+}
+
+static int check_len(uint64_t n) {
+    if (!is_pow2(n)) return fail(WGFFT_ERR_INVALID_LENGTH, kMsgLength);
+    return WGFFT_OK;
+}
+
+static int check_aligned(const void* p, const char* what) {
+    if (((uintptr_t)p & 15u) != 0)
+        return fail(WGFFT_ERR_INVALID_ARGUMENT, std::string(what) + " must be 16-byte aligned");
+    return WGFFT_OK;
+}
+
+static int check_size(uint64_t n, uint64_t batch) {
+    if (batch && n > (~0ull) / sizeof(float2) / batch)
+        return fail(WGFFT_ERR_INVALID_ARGUMENT, "n * batch overflows the address space");
+    return WGFFT_OK;
+}
+
+extern "C" {
+
+int wgfft_device_available(void) { return device_count() > 0 ? 1 : 0; }
+
+const char* wgfft_last_error(void) { return g_last_error.c_str(); }
+
+const char* wgfft_name(void) { return "MI355X Stockham (LDS-resident radix-16/8/4/2)"; }
+
+void wgfft_set_alloc_cap(uint64_t bytes) { g_alloc_cap.store(bytes); }
+
+int wgfft_validate_lengths(const uint64_t* lens, uint64_t count, uint64_t* n_out) {
+    if (n_out) *n_out = 0;
+    if (count == 0) return WGFFT_OK;
+    if (!lens) return fail(WGFFT_ERR_INVALID_ARGUMENT, "lens is NULL");
+    uint64_t n = lens[0];
+    for (uint64_t i = 0; i < count; ++i) {
+        if (lens[i] != n) return fail(WGFFT_ERR_RAGGED_BATCH, kMsgRagged);
+        if (!is_pow2(lens[i])) return fail(WGFFT_ERR_INVALID_LENGTH, kMsgLength);
+    }
+    if (n_out) *n_out = n;
+    return WGFFT_OK;
+}
+
+int wgfft_create(wgfft_t** out, int device) {
+    if (!out) return fail(WGFFT_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    int c = device_count();
+    if (c <= 0) return fail(WGFFT_ERR_NO_DEVICE, "no HIP device available");
+    if (device < 0 || device >= c) return fail(WGFFT_ERR_NO_DEVICE, "device index out of range");
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    wgfft_t* h = new (std::nothrow) wgfft_t();
+    if (!h) return fail(WGFFT_ERR_OUT_OF_MEMORY, "handle allocation failed");
+    h->device = device;
+    h->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
+    e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        return hip_fail(e, "hipStreamCreateWithFlags");
+    }
+    if (const char* env = std::getenv("WGFFT_MP_CHUNK_BYTES")) h->mp_chunk_bytes = std::strtoull(env, nullptr, 10);
+    load_tune_file(h);
+    *out = h;
+    return WGFFT_OK;
+}
+
+void wgfft_destroy(wgfft_t* h) {
+    if (!h) return;
+    {
+        std::lock_guard<std::mutex> lk(h->mu);
+        hipSetDevice(h->device);
+        hipStreamSynchronize(h->stream);
+        for (auto& t : h->tables) hipFree(t.p);
+        h->tables.clear();
+        for (DevBuf* b : {&h->io_in, &h->io_out, &h->io_w, &h->io_aux, &h->scratch, &h->scratch2, &h->red, &h->lossbuf})
+            h->release(*b);
+        hipStreamDestroy(h->stream);
+    }
+    delete h;
+}
+
+int wgfft_exec_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, uint64_t batch, int dir,
+                      void* stream) {
+    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
+    int rc;
+    if ((rc = check_len(n)) || (rc = check_dir(dir)) || (rc = check_size(n, batch))) return rc;
+    if (batch == 0) return WGFFT_OK;
+    if (!d_in || !d_out) return fail(WGFFT_ERR_INVALID_ARGUMENT, "device buffer is NULL");
+    if ((rc = check_aligned(d_in, "d_in")) || (rc = check_aligned(d_out, "d_out"))) return rc;
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    Plan* p;
+    if ((rc = h->get_plan(n, &p))) return rc;
+    bool inv = dir == WGFFT_INVERSE;
+    return h->run(p, inv ? kInverse : kForward, (const float2*)d_in, (float2*)d_out, nullptr, batch,
+                  inv ? 1.0f / (float)n : 1.0f, pick_stream(h, stream));
+}
+
+int wgfft_exec_host(wgfft_t* h, const float* in, float* out, uint64_t n, uint64_t batch, int dir) {
+    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
+    int rc;
+    if ((rc = check_len(n)) || (rc = check_dir(dir)) || (rc = check_size(n, batch))) return rc;
+    if (batch == 0) return WGFFT_OK;
+    if (!in || !out) return fail(WGFFT_ERR_INVALID_ARGUMENT, "host buffer is NULL");
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const uint64_t bytes = n * batch * sizeof(float2);
+    if ((rc = h->ensure(h->io_in, bytes)) || (rc = h->ensure(h->io_out, bytes))) return rc;
+    Plan* p;
+    if ((rc = h->get_plan(n, &p))) return rc;
+    if ((e = hipMemcpyAsync(h->io_in.p, in, bytes, hipMemcpyHostToDevice, h->stream)) != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync(H2D)");
+    bool inv = dir == WGFFT_INVERSE;
+    if ((rc = h->run(p, inv ? kInverse : kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch,
+                     inv ? 1.0f / (float)n : 1.0f, h->stream)))
+        return rc;
+    if ((e = hipMemcpyAsync(out, h->io_out.p, bytes, hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync(D2H)");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    return WGFFT_OK;
+}
+
+int wgfft_exec_windowed_device(wgfft_t* h, const float* d_x, const float* d_w, float* d_y, uint64_t n,
+                               uint64_t batch, void* stream) {
+    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
+    int rc;
+    if ((rc = check_len(n)) || (rc = check_size(n, batch))) return rc;
+    if (batch == 0) return WGFFT_OK;
+    if (!d_x || !d_w || !d_y) return fail(WGFFT_ERR_INVALID_ARGUMENT, "device buffer is NULL");
+    if ((rc = check_aligned(d_x, "d_x")) || (rc = check_aligned(d_y, "d_y")) || (rc = check_aligned(d_w, "d_w")))
+        return rc;
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    Plan* p;
+    if ((rc = h->get_plan(n, &p))) return rc;
+    return h->run(p, kWindowed, (const float2*)d_x, (float2*)d_y, d_w, batch, 1.0f, pick_stream(h, stream));
+}
+
+int wgfft_exec_windowed_host(wgfft_t* h, const float* x, const float* w, float* y, uint64_t n, uint64_t batch) {
+    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
+    int rc;
+    if ((rc = check_len(n)) || (rc = check_size(n, batch))) return rc;
+    if (batch == 0) return WGFFT_OK;
+    if (!x || !w || !y) return fail(WGFFT_ERR_INVALID_ARGUMENT, "host buffer is NULL");
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const uint64_t bytes = n * batch * sizeof(float2);
+    if ((rc = h->ensure(h->io_in, bytes)) || (rc = h->ensure(h->io_out, bytes)) ||
+        (rc = h->ensure(h->io_w, n * sizeof(float) + 16)))
+        return rc;
+    Plan* p;
+    if ((rc = h->get_plan(n, &p))) return rc;
+    if ((e = hipMemcpyAsync(h->io_in.p, x, bytes, hipMemcpyHostToDevice, h->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(h->io_w.p, w, n * sizeof(float), hipMemcpyHostToDevice, h->stream)) != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync(H2D)");
+    if ((rc = h->run(p, kWindowed, (const float2*)h->io_in.p, (float2*)h->io_out.p, (const float*)h->io_w.p, batch,
+                     1.0f, h->stream)))
+        return rc;
+    if ((e = hipMemcpyAsync(y, h->io_out.p, bytes, hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync(D2H)");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    return WGFFT_OK;
+}
+
+int wgfft_bench_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, uint64_t batch, int dir, int warmup,
+                       int iters, double* sec_per_iter) {
+    if (!h || !sec_per_iter) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle or output is NULL");
+    if (iters < 1) return fail(WGFFT_ERR_INVALID_ARGUMENT, "iters must be >= 1");
+    int rc;
+    hipError_t e;
+    for (int i = 0; i < warmup; ++i) {
+        if ((rc = wgfft_exec_device(h, d_in, d_out, n, batch, dir, nullptr))) return rc;
+        if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    }
+    hipEvent_t a, b;
+    if ((e = hipEventCreate(&a)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&b)) != hipSuccess) {
+        hipEventDestroy(a);
+        return hip_fail(e, "hipEventCreate");
+    }
+    rc = WGFFT_OK;
+    hipEventRecord(a, h->stream);
+    for (int i = 0; i < iters && rc == WGFFT_OK; ++i) rc = wgfft_exec_device(h, d_in, d_out, n, batch, dir, nullptr);
+    hipEventRecord(b, h->stream);
+    if (rc == WGFFT_OK) {
+        if ((e = hipEventSynchronize(b)) != hipSuccess) {
+            rc = hip_fail(e, "hipEventSynchronize");
+        } else {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, a, b);
+            *sec_per_iter = (double)ms * 1e-3 / iters;
+        }
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return rc;
+}
+
+void wgfft_synth_uniform(float* out, uint64_t nfloats, uint64_t seed) { synth_uniform(out, nfloats, seed); }
+
+}  // extern "C"

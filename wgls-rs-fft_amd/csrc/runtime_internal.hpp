@@ -1,0 +1,76 @@
+// runtime_internal.hpp -- types shared by the host-runtime translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace wgfft {
+
+extern thread_local std::string g_last_error;
+extern const char* kMsgLength;
+extern const char* kMsgRagged;
+
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+struct DevBuf {
+    void* p = nullptr;
+    uint64_t bytes = 0;
+};
+
+// Per-length plan; the counterpart of the reference's SizeCache
+// (src/lib.rs:231-249) without the fixed max_batch buffers.
+struct Plan {
+    enum Kind { kSmall, kBatched, kMultiPass };
+    uint64_t n = 0;
+    int log2n = 0;
+    Kind kind = kSmall;
+    const BatchedVariant* var = nullptr;  // kBatched
+    float2* tw = nullptr;                 // kBatched: W_N^j, j < N
+    std::vector<int> pass_log2r;          // kMultiPass
+    std::vector<const GlobalPassVariant*> pass_var;
+    std::map<int, float2*> tw_r;          // sub-FFT tables W_R^j per radix
+    float2* tw_hi = nullptr;
+    float2* tw_lo = nullptr;
+    uint32_t h = 0;
+};
+
+struct Handle;
+void load_tune_file(Handle* h);
+void synth_uniform(float* out, uint64_t nfloats, uint64_t seed);
+const BatchedVariant* default_variant(int log2n);
+const BatchedVariant* find_variant(int log2n, const char* plan, int E, int G);
+
+struct Handle {
+    int device = 0;
+    int num_cus = 1;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::map<uint64_t, Plan> plans;
+    std::map<int, const BatchedVariant*> tuned;  // log2n -> autotuned variant
+    std::map<std::pair<const BatchedVariant*, int>, int> occ_cache;
+    std::vector<DevBuf> tables;
+    DevBuf io_in, io_out, io_w, io_aux, scratch, scratch2, red, lossbuf;
+    uint64_t held = 0;
+    uint64_t mp_chunk_bytes = 64ull << 20;
+
+    int ensure(DevBuf& b, uint64_t bytes);
+    void release(DevBuf& b);
+    int upload_table(const std::vector<float2>& host, float2** dev);
+    int get_plan(uint64_t n, Plan** out);
+    int blocks_per_cu(const BatchedVariant* v, Mode m);
+    int run(Plan* p, Mode mode, const float2* in, float2* out, const float* win, uint64_t batch, float scale,
+            hipStream_t s, const BatchedVariant* force = nullptr);
+};
+
+}  // namespace wgfft
+
+struct wgfft_handle : wgfft::Handle {};

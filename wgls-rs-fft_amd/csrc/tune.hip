@@ -1,0 +1,161 @@
+// tune.hip -- kernel-variant autotuner, plan description, tune-file
+// persistence and host input synthesis.
+//
+// The autotuner times every compiled LDS variant for a length (radix split,
+// elements per thread, transforms per workgroup) on a resident scratch batch
+// and keeps the fastest for the handle.  With WGFFT_TUNE_FILE set, choices
+// are read at handle creation and appended when a tuning run finishes
+// (lines: "log2n plan E G").
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "../../include/wgfft.h"
+#include "runtime_internal.hpp"
+
+namespace wgfft {
+
+static uint64_t splitmix64_next(uint64_t* s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+struct Pcg32 {
+    uint64_t state = 0, inc = 1;
+    uint32_t next() {
+        uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+    void seed(uint64_t initstate, uint64_t initseq) {
+        state = 0u;
+        inc = (initseq << 1u) | 1u;
+        next();
+        state += initstate;
+        next();
+    }
+};
+
+void synth_uniform(float* out, uint64_t nfloats, uint64_t seed) {
+    uint64_t sm = seed;
+    uint64_t st = splitmix64_next(&sm);
+    uint64_t sq = splitmix64_next(&sm);
+    Pcg32 r;
+    r.seed(st, sq);
+    for (uint64_t i = 0; i < nfloats; ++i) out[i] = (float)((double)(r.next() >> 8) * (1.0 / 8388608.0) - 1.0);
+}
+
+void load_tune_file(Handle* h) {
+    const char* path = std::getenv("WGFFT_TUNE_FILE");
+    if (!path) return;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return;
+    int log2n, E, G;
+    char plan[64];
+    while (std::fscanf(f, "%d %63s %d %d", &log2n, plan, &E, &G) == 4) {
+        const BatchedVariant* v = find_variant(log2n, plan, E, G);
+        if (v) h->tuned[log2n] = v;
+    }
+    std::fclose(f);
+}
+
+static void describe(const BatchedVariant* v, char* desc, uint64_t len) {
+    if (!desc || !len) return;
+    std::snprintf(desc, len, "lds N=%d plan=%s E=%d G=%d threads=%d lds=%dB", 1 << v->log2n, v->plan, v->E, v->G,
+                  v->threads, v->lds_bytes);
+}
+
+}  // namespace wgfft
+
+using namespace wgfft;
+
+extern "C" int wgfft_plan_describe(wgfft_t* h, uint64_t n, char* desc, uint64_t desc_len) {
+    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (n == 0 || (n & (n - 1))) return fail(WGFFT_ERR_INVALID_LENGTH, kMsgLength);
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipSetDevice(h->device);
+    Plan* p;
+    int rc;
+    if ((rc = h->get_plan(n, &p))) return rc;
+    if (!desc || !desc_len) return WGFFT_OK;
+    if (p->kind == Plan::kSmall) {
+        std::snprintf(desc, desc_len, "register N=%llu one thread per transform", (unsigned long long)n);
+    } else if (p->kind == Plan::kBatched) {
+        describe(p->var, desc, desc_len);
+    } else {
+        std::string s = "multipass N=" + std::to_string(n) + " passes=";
+        for (size_t i = 0; i < p->pass_var.size(); ++i) {
+            const GlobalPassVariant* v = p->pass_var[i];
+            s += (i ? "," : "") + std::string("R") + std::to_string(1 << v->log2r) + "(" + v->plan + ",G" +
+                 std::to_string(v->G) + ")";
+        }
+        std::snprintf(desc, desc_len, "%s", s.c_str());
+    }
+    return WGFFT_OK;
+}
+
+extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc, uint64_t desc_len) {
+    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (n == 0 || (n & (n - 1))) return fail(WGFFT_ERR_INVALID_LENGTH, kMsgLength);
+    if (batch == 0) return fail(WGFFT_ERR_INVALID_ARGUMENT, "batch must be >= 1");
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    Plan* p;
+    int rc;
+    if ((rc = h->get_plan(n, &p))) return rc;
+    if (p->kind != Plan::kBatched) {
+        if (desc && desc_len) std::snprintf(desc, desc_len, "no alternatives for N=%llu", (unsigned long long)n);
+        return WGFFT_OK;
+    }
+    const uint64_t bytes = n * batch * sizeof(float2);
+    if ((rc = h->ensure(h->io_in, bytes)) || (rc = h->ensure(h->io_out, bytes))) return rc;
+    if ((e = hipMemsetAsync(h->io_in.p, 0, bytes, h->stream)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    const BatchedVariant* best = nullptr;
+    float best_ms = 1e30f;
+    for (const auto& v : batched_registry()) {
+        if (v.log2n != p->log2n) continue;
+        for (int i = 0; i < 2; ++i)
+            h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream, &v);
+        const int iters = 10;
+        hipEventRecord(a, h->stream);
+        for (int i = 0; i < iters; ++i)
+            h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream, &v);
+        hipEventRecord(b, h->stream);
+        if ((e = hipEventSynchronize(b)) != hipSuccess) {
+            hipEventDestroy(a);
+            hipEventDestroy(b);
+            return hip_fail(e, "autotune timing");
+        }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, a, b);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = &v;
+        }
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (best) {
+        h->tuned[p->log2n] = best;
+        p->var = best;
+        describe(best, desc, desc_len);
+        if (const char* path = std::getenv("WGFFT_TUNE_FILE")) {
+            if (FILE* f = std::fopen(path, "a")) {
+                std::fprintf(f, "%d %s %d %d\n", best->log2n, best->plan, best->E, best->G);
+                std::fclose(f);
+            }
+        }
+    }
+    return WGFFT_OK;
+}

@@ -1,0 +1,6 @@
+// variants_4096.hip -- LDS-resident variants for N = 4096 (the headline length).
+#include "kernels.hpp"
+
+WGFFT_REGISTER_BATCHED(4096, 32, 1, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_BATCHED(4096, 32, 2, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_BATCHED(4096, 16, 1, 8x8x8x8, 8, 8, 8, 8);

@@ -1,0 +1,171 @@
+// vjp.hip -- differentiation of Y = FFT(w * x) (no counterpart in the
+// reference, which has no adjoint).
+//
+//   gu = F^H gY            conjugate-twiddle Stockham pass, no 1/N scale
+//   gx = w * gu
+//   gw_n = sum_b Re(conj(gu_bn) x_bn)
+//
+// The VJP entry runs the adjoint transform with the same LDS kernels as the
+// inverse (unit scale) and finishes with one elementwise pass that forms gx
+// in place and per-chunk partial sums of gw, reduced deterministically.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/wgfft.h"
+#include "runtime_internal.hpp"
+
+namespace wgfft {
+
+constexpr int kRowChunks = 64;  // batch chunks of the gw partial sums
+
+// gx <- w * gu (in place), part[c][i] = sum over the c-th batch chunk.
+__global__ void __launch_bounds__(256)
+k_vjp_finish(const float2* __restrict__ x, const float* __restrict__ w, float2* __restrict__ gx,
+             float* __restrict__ part, uint64_t n, uint64_t batch, uint64_t rows_per_chunk) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const uint64_t c = blockIdx.y;
+    if (i >= n) return;
+    const float wi = w[i];
+    const uint64_t b0 = c * rows_per_chunk;
+    const uint64_t b1 = min(batch, b0 + rows_per_chunk);
+    float acc = 0.f;
+    for (uint64_t b = b0; b < b1; ++b) {
+        const uint64_t e = b * n + i;
+        float2 u = gx[e];
+        float2 xv = x[e];
+        acc += u.x * xv.x + u.y * xv.y;
+        gx[e] = make_float2(wi * u.x, wi * u.y);
+    }
+    part[c * n + i] = acc;
+}
+
+__global__ void __launch_bounds__(256)
+k_sum_rows(const float* __restrict__ part, float* __restrict__ out, uint64_t n, uint64_t rows) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    float acc = 0.f;
+    for (uint64_t r = 0; r < rows; ++r) acc += part[r * n + i];
+    out[i] = acc;
+}
+
+// gY = 2 m Y in place; per-block partial sums of m |Y|^2.
+__global__ void __launch_bounds__(256)
+k_mask_grad(float2* __restrict__ y, const float* __restrict__ mask, float* __restrict__ part, uint64_t n,
+            uint64_t total) {
+    __shared__ float red[256];
+    float acc = 0.f;
+    for (uint64_t e = (uint64_t)blockIdx.x * 256u + threadIdx.x; e < total; e += (uint64_t)gridDim.x * 256u) {
+        const float m = mask[e & (n - 1)];
+        float2 v = y[e];
+        acc += m * (v.x * v.x + v.y * v.y);
+        y[e] = make_float2(2.f * m * v.x, 2.f * m * v.y);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_sum_all(const float* __restrict__ part, float* __restrict__ out, int count) {
+    __shared__ float red[256];
+    float acc = 0.f;
+    for (int i = threadIdx.x; i < count; i += 256) acc += part[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// gx, gw from gu (already in d_gx).
+static int vjp_finish(Handle* h, const float* d_x, const float* d_w, float* d_gx, float* d_gw, uint64_t n,
+                      uint64_t batch, hipStream_t s) {
+    const uint64_t chunks = std::min<uint64_t>(batch, kRowChunks);
+    const uint64_t rows = (batch + chunks - 1) / chunks;
+    int rc;
+    if ((rc = h->ensure(h->red, chunks * n * sizeof(float)))) return rc;
+    dim3 grid((unsigned)((n + 255) / 256), (unsigned)chunks);
+    hipLaunchKernelGGL(k_vjp_finish, grid, dim3(256), 0, s, (const float2*)d_x, d_w, (float2*)d_gx,
+                       (float*)h->red.p, n, batch, rows);
+    hipLaunchKernelGGL(k_sum_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const float*)h->red.p, d_gw,
+                       n, chunks);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "vjp finish launch");
+    return WGFFT_OK;
+}
+
+}  // namespace wgfft
+
+using namespace wgfft;
+
+static int vjp_checks(wgfft_t* h, uint64_t n, uint64_t batch) {
+    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (n == 0 || (n & (n - 1))) return fail(WGFFT_ERR_INVALID_LENGTH, kMsgLength);
+    if (batch && n > (~0ull) / 8 / batch) return fail(WGFFT_ERR_INVALID_ARGUMENT, "n * batch overflows");
+    return WGFFT_OK;
+}
+
+extern "C" int wgfft_vjp_windowed_device(wgfft_t* h, const float* d_x, const float* d_w, const float* d_gy,
+                                         float* d_gx, float* d_gw, uint64_t n, uint64_t batch, void* stream) {
+    int rc;
+    if ((rc = vjp_checks(h, n, batch))) return rc;
+    if (!d_x || !d_w || !d_gy || !d_gx || !d_gw) return fail(WGFFT_ERR_INVALID_ARGUMENT, "device buffer is NULL");
+    if ((((uintptr_t)d_x | (uintptr_t)d_gy | (uintptr_t)d_gx | (uintptr_t)d_w) & 15u) != 0)
+        return fail(WGFFT_ERR_INVALID_ARGUMENT, "device buffers must be 16-byte aligned");
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    if (batch == 0) {
+        hipMemsetAsync(d_gw, 0, n * sizeof(float), s);
+        return WGFFT_OK;
+    }
+    Plan* p;
+    if ((rc = h->get_plan(n, &p))) return rc;
+    // gu = F^H gY: inverse-sign transform with unit scale
+    if ((rc = h->run(p, kInverse, (const float2*)d_gy, (float2*)d_gx, nullptr, batch, 1.0f, s))) return rc;
+    return vjp_finish(h, d_x, d_w, d_gx, d_gw, n, batch, s);
+}
+
+extern "C" int wgfft_fwd_adjoint_masked_device(wgfft_t* h, const float* d_x, const float* d_w, const float* d_mask,
+                                               float* d_y, float* d_gx, float* d_gw, float* d_loss, uint64_t n,
+                                               uint64_t batch, void* stream) {
+    int rc;
+    if ((rc = vjp_checks(h, n, batch))) return rc;
+    if (!d_x || !d_w || !d_mask || !d_gx || !d_gw || !d_loss)
+        return fail(WGFFT_ERR_INVALID_ARGUMENT, "device buffer is NULL");
+    if ((((uintptr_t)d_x | (uintptr_t)d_gx | (uintptr_t)d_w | (uintptr_t)d_y) & 15u) != 0)
+        return fail(WGFFT_ERR_INVALID_ARGUMENT, "device buffers must be 16-byte aligned");
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    if (batch == 0) {
+        hipMemsetAsync(d_gw, 0, n * sizeof(float), s);
+        hipMemsetAsync(d_loss, 0, sizeof(float), s);
+        return WGFFT_OK;
+    }
+    Plan* p;
+    if ((rc = h->get_plan(n, &p))) return rc;
+    const uint64_t total = n * batch;
+    // Y (into d_y or scratch), then gY = 2 m Y in the scratch copy
+    if ((rc = h->ensure(h->io_aux, total * sizeof(float2)))) return rc;
+    float2* gy = (float2*)h->io_aux.p;
+    if ((rc = h->run(p, kWindowed, (const float2*)d_x, gy, d_w, batch, 1.0f, s))) return rc;
+    if (d_y) {
+        e = hipMemcpyAsync(d_y, gy, total * sizeof(float2), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(Y)");
+    }
+    const int blocks = (int)std::min<uint64_t>((total + 255) / 256, 4096);
+    if ((rc = h->ensure(h->lossbuf, blocks * sizeof(float) + 256))) return rc;
+    hipLaunchKernelGGL(k_mask_grad, dim3(blocks), dim3(256), 0, s, gy, d_mask, (float*)h->lossbuf.p, n, total);
+    hipLaunchKernelGGL(k_sum_all, dim3(1), dim3(256), 0, s, (const float*)h->lossbuf.p, d_loss, blocks);
+    if ((rc = h->run(p, kInverse, gy, (float2*)d_gx, nullptr, batch, 1.0f, s))) return rc;
+    return vjp_finish(h, d_x, d_w, d_gx, d_gw, n, batch, s);
+}

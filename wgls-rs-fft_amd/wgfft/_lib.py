@@ -1,0 +1,92 @@
+"""ctypes binding of libwgfft.so (declared in include/wgfft.h).
+
+The library is loaded from ``wgfft/_lib/libwgfft.so``.  When it is missing
+the import raises: there is no CPU fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libwgfft.so")
+
+WGFFT_OK = 0
+WGFFT_ERR_INVALID_LENGTH = -1
+WGFFT_ERR_RAGGED_BATCH = -2
+WGFFT_ERR_NO_DEVICE = -3
+WGFFT_ERR_OUT_OF_MEMORY = -4
+WGFFT_ERR_HIP = -5
+WGFFT_ERR_INVALID_ARGUMENT = -6
+WGFFT_FORWARD = -1
+WGFFT_INVERSE = 1
+
+# Every symbol include/wgfft.h declares.
+EXPORTS = (
+    "wgfft_device_available", "wgfft_create", "wgfft_destroy", "wgfft_last_error", "wgfft_name",
+    "wgfft_validate_lengths", "wgfft_exec_host", "wgfft_exec_device", "wgfft_exec_windowed_host",
+    "wgfft_exec_windowed_device", "wgfft_vjp_windowed_device", "wgfft_fwd_adjoint_masked_device",
+    "wgfft_ragged_create", "wgfft_ragged_destroy", "wgfft_ragged_exec_device", "wgfft_bench_device",
+    "wgfft_autotune", "wgfft_plan_describe", "wgfft_synth_uniform", "wgfft_set_alloc_cap",
+)
+
+
+class WgfftError(RuntimeError):
+    """Error returned through the C-ABI (code + wgfft_last_error message)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+        self.message = message
+
+
+_u64 = ctypes.c_uint64
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python wgls-rs-fft_amd/build.py` "
+            "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    L.wgfft_device_available.restype = _i
+    L.wgfft_create.argtypes = [ctypes.POINTER(_vp), _i]
+    L.wgfft_destroy.argtypes = [_vp]
+    L.wgfft_destroy.restype = None
+    L.wgfft_last_error.restype = ctypes.c_char_p
+    L.wgfft_name.restype = ctypes.c_char_p
+    L.wgfft_validate_lengths.argtypes = [ctypes.POINTER(_u64), _u64, ctypes.POINTER(_u64)]
+    L.wgfft_exec_host.argtypes = [_vp, _vp, _vp, _u64, _u64, _i]
+    L.wgfft_exec_device.argtypes = [_vp, _vp, _vp, _u64, _u64, _i, _vp]
+    L.wgfft_exec_windowed_host.argtypes = [_vp, _vp, _vp, _vp, _u64, _u64]
+    L.wgfft_exec_windowed_device.argtypes = [_vp, _vp, _vp, _vp, _u64, _u64, _vp]
+    L.wgfft_vjp_windowed_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _u64, _u64, _vp]
+    L.wgfft_fwd_adjoint_masked_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _u64, _vp]
+    L.wgfft_ragged_create.argtypes = [_vp, _vp, _vp, _u64, ctypes.POINTER(_vp)]
+    L.wgfft_ragged_destroy.argtypes = [_vp]
+    L.wgfft_ragged_destroy.restype = None
+    L.wgfft_ragged_exec_device.argtypes = [_vp, _vp, _vp, _i, _i, _vp]
+    L.wgfft_bench_device.argtypes = [_vp, _vp, _vp, _u64, _u64, _i, _i, _i, ctypes.POINTER(ctypes.c_double)]
+    L.wgfft_autotune.argtypes = [_vp, _u64, _u64, ctypes.c_char_p, _u64]
+    L.wgfft_plan_describe.argtypes = [_vp, _u64, ctypes.c_char_p, _u64]
+    L.wgfft_synth_uniform.argtypes = [_vp, _u64, _u64]
+    L.wgfft_synth_uniform.restype = None
+    L.wgfft_set_alloc_cap.argtypes = [_u64]
+    L.wgfft_set_alloc_cap.restype = None
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return load().wgfft_last_error().decode()
+
+
+def check(rc: int) -> None:
+    if rc != WGFFT_OK:
+        raise WgfftError(rc, last_error())

@@ -1,0 +1,216 @@
+"""Host-side mirror of wgls-rs-fft's ``GpuFft`` / ``FftExecutor`` interface.
+
+Same method names, argument meaning and error behaviour as the reference
+(``src/lib.rs:32-45`` trait, ``src/lib.rs:285-568`` runtime):
+
+* ``GpuFft.new()`` creates the executor, raising when no GPU is usable
+  (``src/lib.rs:285-330``; ``Default`` panics the same way, ``:946-950``).
+* ``fft(inputs)`` / ``ifft(inputs)`` take a batch (list) of equal-length,
+  power-of-two complex signals and return a new list of complex64 arrays;
+  the inverse is 1/N scaled (``src/lib.rs:433-488``).
+* An empty batch returns ``[]``; a ragged batch raises "All input vectors in
+  a batch must have the same length"; a zero or non-power-of-two length
+  raises "Transform length must be a non-zero power of two"
+  (``src/lib.rs:506-533``).  Validation runs in the C library before any
+  device work, so it is exercised without a GPU.
+
+Every transform runs in the HIP kernels of libwgfft.so through the C-ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Iterable, List, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import WGFFT_FORWARD, WGFFT_INVERSE, WgfftError, check
+
+
+def validate_lengths(lengths: Sequence[int]) -> int:
+    """Return the common length n (0 for an empty batch) or raise WgfftError."""
+    L = _lib.load()
+    arr = (ctypes.c_uint64 * max(1, len(lengths)))(*[int(v) for v in lengths])
+    n = ctypes.c_uint64(0)
+    check(L.wgfft_validate_lengths(arr, len(lengths), ctypes.byref(n)))
+    return int(n.value)
+
+
+def is_gpu_available() -> bool:
+    """GpuFft::is_gpu_available (src/lib.rs:381-389)."""
+    return bool(_lib.load().wgfft_device_available())
+
+
+def _flatten(inputs: Sequence) -> tuple[np.ndarray, int]:
+    n = validate_lengths([len(x) for x in inputs])
+    if not inputs:
+        return np.empty((0, 0), dtype=np.complex64), 0
+    flat = np.empty((len(inputs), n), dtype=np.complex64)
+    for i, x in enumerate(inputs):
+        flat[i] = np.asarray(x, dtype=np.complex64)
+    return flat, n
+
+
+class FftExecutor:
+    """Plugin interface of the reference (src/lib.rs:32-45)."""
+
+    def name(self) -> str:  # pragma: no cover - interface
+        raise NotImplementedError
+
+    def fft(self, inputs):  # pragma: no cover - interface
+        raise NotImplementedError
+
+    def ifft(self, inputs):  # pragma: no cover - interface
+        raise NotImplementedError
+
+    def as_any(self):
+        return self
+
+
+class GpuFft(FftExecutor):
+    """Batched complex64 FFT on one MI355X through libwgfft.so."""
+
+    def __init__(self, device: int = 0):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        check(L.wgfft_create(ctypes.byref(h), int(device)))
+        self._h = h
+        self._L = L
+
+    @classmethod
+    def new(cls, device: int = 0) -> "GpuFft":
+        return cls(device)
+
+    is_gpu_available = staticmethod(is_gpu_available)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.wgfft_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- FftExecutor -------------------------------------------------
+    def name(self) -> str:
+        return self._L.wgfft_name().decode()
+
+    def fft(self, inputs: Sequence) -> List[np.ndarray]:
+        return self._transform(inputs, WGFFT_FORWARD)
+
+    def ifft(self, inputs: Sequence) -> List[np.ndarray]:
+        return self._transform(inputs, WGFFT_INVERSE)
+
+    def fft_windowed(self, inputs: Sequence, window) -> List[np.ndarray]:
+        """FFT(w * x) per signal, w real of length n (multiplied at load)."""
+        flat, n = _flatten(inputs)
+        if flat.shape[0] == 0:
+            return []
+        w = np.ascontiguousarray(window, dtype=np.float32)
+        if w.shape != (n,):
+            raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT, "window length must equal the signal length")
+        out = np.empty_like(flat)
+        check(self._L.wgfft_exec_windowed_host(self._h, flat.ctypes.data, w.ctypes.data, out.ctypes.data, n,
+                                               flat.shape[0]))
+        return list(out)
+
+    def _transform(self, inputs: Sequence, direction: int) -> List[np.ndarray]:
+        flat, n = _flatten(inputs)
+        if flat.shape[0] == 0:
+            return []
+        out = np.empty_like(flat)
+        check(self._L.wgfft_exec_host(self._h, flat.ctypes.data, out.ctypes.data, n, flat.shape[0], direction))
+        return list(out)
+
+    # ---- batched arrays (one contiguous buffer, TODO.md:7 of the reference)
+    def fft_array(self, x: np.ndarray, inverse: bool = False) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.complex64)
+        if x.ndim == 1:
+            x = x[None, :]
+        out = np.empty_like(x)
+        check(self._L.wgfft_exec_host(self._h, x.ctypes.data, out.ctypes.data, x.shape[1], x.shape[0],
+                                      WGFFT_INVERSE if inverse else WGFFT_FORWARD))
+        return out
+
+    # ---- device buffers (torch tensors on the GPU) ---------------------
+    @staticmethod
+    def _stream(stream) -> int:
+        if stream is None:
+            import torch
+
+            return torch.cuda.current_stream().cuda_stream
+        return int(stream)
+
+    def exec_device(self, x, out, inverse: bool = False, stream=None) -> None:
+        """Transform a (batch, n) complex64 CUDA tensor into `out`."""
+        _check_tensor(x)
+        _check_tensor(out)
+        batch, n = x.shape
+        check(self._L.wgfft_exec_device(self._h, x.data_ptr(), out.data_ptr(), n, batch,
+                                        WGFFT_INVERSE if inverse else WGFFT_FORWARD, self._stream(stream)))
+
+    def exec_windowed_device(self, x, w, out, stream=None) -> None:
+        _check_tensor(x)
+        _check_tensor(out)
+        batch, n = x.shape
+        check(self._L.wgfft_exec_windowed_device(self._h, x.data_ptr(), w.data_ptr(), out.data_ptr(), n, batch,
+                                                 self._stream(stream)))
+
+    def vjp_windowed_device(self, x, w, gy, gx, gw, stream=None) -> None:
+        batch, n = x.shape
+        check(self._L.wgfft_vjp_windowed_device(self._h, x.data_ptr(), w.data_ptr(), gy.data_ptr(),
+                                                gx.data_ptr(), gw.data_ptr(), n, batch, self._stream(stream)))
+
+    def fwd_adjoint_masked_device(self, x, w, mask, y, gx, gw, loss, stream=None) -> None:
+        batch, n = x.shape
+        check(self._L.wgfft_fwd_adjoint_masked_device(
+            self._h, x.data_ptr(), w.data_ptr(), mask.data_ptr(), 0 if y is None else y.data_ptr(),
+            gx.data_ptr(), gw.data_ptr(), loss.data_ptr(), n, batch, self._stream(stream)))
+
+    def bench_device(self, x, out, inverse: bool = False, warmup: int = 3, iters: int = 50) -> float:
+        batch, n = x.shape
+        sec = ctypes.c_double(0.0)
+        check(self._L.wgfft_bench_device(self._h, x.data_ptr(), out.data_ptr(), n, batch,
+                                         WGFFT_INVERSE if inverse else WGFFT_FORWARD, warmup, iters,
+                                         ctypes.byref(sec)))
+        return sec.value
+
+    def autotune(self, n: int, batch: int) -> str:
+        buf = ctypes.create_string_buffer(256)
+        check(self._L.wgfft_autotune(self._h, n, batch, buf, 256))
+        return buf.value.decode()
+
+    def describe(self, n: int) -> str:
+        buf = ctypes.create_string_buffer(256)
+        check(self._L.wgfft_plan_describe(self._h, n, buf, 256))
+        return buf.value.decode()
+
+
+def _check_tensor(t) -> None:
+    import torch
+
+    if not (t.is_cuda and t.dtype == torch.complex64 and t.is_contiguous() and t.dim() == 2):
+        raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT,
+                         "expected a contiguous (batch, n) complex64 tensor on the GPU")
+
+
+def synth_uniform(shape: Iterable[int], seed: int) -> np.ndarray:
+    """complex64 array, re/im i.i.d. U[-1,1) from the library's PCG32 stream."""
+    shape = tuple(int(s) for s in shape)
+    count = int(np.prod(shape))
+    out = np.empty(2 * count, dtype=np.float32)
+    _lib.load().wgfft_synth_uniform(out.ctypes.data, 2 * count, seed)
+    return out.view(np.complex64).reshape(shape)
+
+
+def set_alloc_cap(nbytes: int) -> None:
+    _lib.load().wgfft_set_alloc_cap(int(nbytes))

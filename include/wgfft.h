@@ -127,6 +127,9 @@ void wgfft_ragged_destroy(wgfft_ragged_t* p);
  * distinct length (mode 2, the comparison baseline). */
 int wgfft_ragged_exec_device(wgfft_ragged_t* p, const float* d_in, float* d_out, int dir, int mode, void* stream);
 
+/* Total complex samples covered by a mixed-length plan (0 for NULL). */
+uint64_t wgfft_ragged_samples(const wgfft_ragged_t* p);
+
 /* ---- measurement and tuning ------------------------------------------- */
 
 /* Device-only timing of wgfft_exec_device on resident buffers: `warmup`

@@ -204,54 +204,40 @@ struct GlobalPassSlot {
     static constexpr int len = m + ((4 - m % 32) + 32) % 32;
 };
 
-// ---------------------------------------------------------------------------
-// Two-level twiddle: W_N^j = W_N^{(j >> h) << h} * W_N^{j & (2^h - 1)}.
-template <bool INV>
-__device__ __forceinline__ float2 tw2(const float2* __restrict__ hi, const float2* __restrict__ lo, uint64_t j,
-                                      uint32_t h) {
-    float2 w = cmul(hi[j >> h], lo[j & ((1ull << h) - 1)]);
-    return conj_if<INV>(w);
-}
+// Twiddle sources for W_N^j (forward sign).
+struct TwTwoLevel {
+    const float2* hi;
+    const float2* lo;
+    uint32_t h;
+    __device__ __forceinline__ float2 operator()(uint64_t j) const {
+        return cmul(hi[j >> h], lo[j & ((1ull << h) - 1)]);
+    }
+};
+struct TwOneLevel {
+    const float2* t;
+    __device__ __forceinline__ float2 operator()(uint64_t j) const { return t[j]; }
+};
 
-// One multi-pass Stockham pass over a length-N transform with radix R done as
-// an LDS sub-FFT.  The workgroup handles butterflies b0 .. b0+G-1 of one
-// transform (G <= L, so both the strided loads and the strided stores are
-// runs of G contiguous elements); when L == 1 each butterfly's R outputs are
-// contiguous and are stored straight from registers.
-template <int R, int E, int G, int MODE, int... Rs>
-__global__ void __launch_bounds__(G*(R / E))
-k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw_r,
-              const float2* __restrict__ tw_hi, const float2* __restrict__ tw_lo,
-              const float* __restrict__ win, uint32_t log2n, uint32_t h, uint64_t L, uint64_t batch,
-              float scale) {
+// One group of one multi-pass Stockham pass over a length-N transform with
+// radix R done as an LDS sub-FFT: butterflies b0 .. b0+G-1 (G <= L, so both
+// the strided loads and the strided stores are runs of G contiguous
+// elements); when L == 1 each butterfly's R outputs are contiguous and are
+// stored straight from registers.  src/dst point at the transform.  The
+// caller provides G * SLOT complex elements of LDS and the LDS barrier
+// before the next group.
+template <int R, int E, int G, int MODE, class TW, int... Rs>
+__device__ __forceinline__ void global_pass_group(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                  const float2* __restrict__ tw_r, const TW& twN,
+                                                  const float* __restrict__ win, uint64_t N, uint64_t L,
+                                                  uint64_t b0, float sc, float2* __restrict__ lds_all) {
     constexpr int T = R / E;
     constexpr int THREADS = G * T;
     constexpr bool INV = MODE == kInverse;
     constexpr int SLOT = GlobalPassSlot<R, E, Rs...>::len;
-    __shared__ __align__(16) float2 lds_all[G * SLOT];
-
-    const uint64_t N = 1ull << log2n;
-    const uint64_t NR = N >> ilog2(R);  // butterflies per transform
-    const uint64_t groups_per_tr = NR / G;
-    const uint64_t total = groups_per_tr * batch;
-
-    // XCD-aware remap: block ids are dealt round-robin over the 8 XCDs; map
-    // consecutive logical groups onto one XCD so neighbouring column runs
-    // share that XCD's L2 lines.
-    const uint64_t pid = blockIdx.x;
-    const uint64_t nb = gridDim.x;
-    uint64_t lid = pid;
-    if ((nb & 7) == 0) lid = (pid & 7) * (nb >> 3) + (pid >> 3);
-    if (lid >= total) return;
-    const uint64_t tr = lid / groups_per_tr;
-    const uint64_t b0 = (lid % groups_per_tr) * G;
-    const float2* src = in + tr * N;
-    float2* dst = out + tr * N;
-
+    const uint64_t NR = N / R;
     const uint32_t tid = threadIdx.x;
     const uint64_t s = N / ((uint64_t)R * L);  // W_{RL} = W_N^s
     const bool first = (L == 1);
-    const bool last = (L * R == N);
 
     // staged load of the R x G tile (rows m, adjacent butterfly pairs per lane)
     constexpr int PAIRS = R * (G / 2);
@@ -269,8 +255,8 @@ k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const flo
         }
         if (!first) {
             const uint64_t k0 = (b0 + 2 * gp) & (L - 1);
-            a = cmul(a, tw2<INV>(tw_hi, tw_lo, ((uint64_t)m * k0 * s) & (N - 1), h));
-            b = cmul(b, tw2<INV>(tw_hi, tw_lo, ((uint64_t)m * (k0 + 1) * s) & (N - 1), h));
+            a = cmul(a, conj_if<INV>(twN(((uint64_t)m * k0 * s) & (N - 1))));
+            b = cmul(b, conj_if<INV>(twN(((uint64_t)m * (k0 + 1) * s) & (N - 1))));
         }
         const uint32_t pm = pad_idx<5, 0>(m);
         lds_all[(2 * gp) * SLOT + pm] = a;
@@ -288,7 +274,6 @@ k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const flo
     }
     WGFFT_LDS_SYNC();
     Engine<R, E, T, INV>::run(d, lds, tw_r, t, Radices<Rs...>{});
-    const float sc = last ? scale : 1.0f;
 
     if (first) {
         // output b R + m: R contiguous elements per butterfly
@@ -312,6 +297,34 @@ k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const flo
         const uint64_t o = bj * R * L + k0base + 2 * gp + (uint64_t)m * L;
         *reinterpret_cast<float4*>(dst + o) = make_float4(a.x, a.y, b.x, b.y);
     }
+}
+
+template <int R, int E, int G, int MODE, int... Rs>
+__global__ void __launch_bounds__(G*(R / E))
+k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw_r,
+              const float2* __restrict__ tw_hi, const float2* __restrict__ tw_lo,
+              const float* __restrict__ win, uint32_t log2n, uint32_t h, uint64_t L, uint64_t batch,
+              float scale) {
+    constexpr int SLOT = GlobalPassSlot<R, E, Rs...>::len;
+    __shared__ __align__(16) float2 lds_all[G * SLOT];
+
+    const uint64_t N = 1ull << log2n;
+    const uint64_t groups_per_tr = (N / R) / G;
+    const uint64_t total = groups_per_tr * batch;
+
+    // XCD-aware remap: block ids are dealt round-robin over the 8 XCDs; map
+    // consecutive logical groups onto one XCD so neighbouring column runs
+    // share that XCD's L2 lines.
+    const uint64_t pid = blockIdx.x;
+    const uint64_t nb = gridDim.x;
+    uint64_t lid = pid;
+    if ((nb & 7) == 0) lid = (pid & 7) * (nb >> 3) + (pid >> 3);
+    if (lid >= total) return;
+    const uint64_t tr = lid / groups_per_tr;
+    const uint64_t b0 = (lid % groups_per_tr) * G;
+    const bool last = (L * R == N);
+    global_pass_group<R, E, G, MODE, TwTwoLevel, Rs...>(in + tr * N, out + tr * N, tw_r, TwTwoLevel{tw_hi, tw_lo, h},
+                                                        win, N, L, b0, last ? scale : 1.0f, lds_all);
 }
 
 template <int R, int E, int G, int... Rs>

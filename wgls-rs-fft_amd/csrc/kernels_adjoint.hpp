@@ -1,0 +1,149 @@
+// kernels_adjoint.hpp -- fused differentiation kernels for Y = FFT(w * x).
+//
+//  k_fwd_adjoint  L = sum m |Y|^2: load x once, forward LDS FFT of w*x,
+//                 gY = 2 m Y in registers (and m|Y|^2 into the loss), the
+//                 conjugate-twiddle adjoint FFT of gY in the same workgroup,
+//                 gx = w * gu stored, Re(conj(gu) x) accumulated per element
+//                 in registers across the transforms this workgroup handles.
+//  k_vjp          given gY: adjoint FFT of gY, gx = w * gu, the same gw
+//                 accumulation.
+// Both write per-workgroup partial sums (gw: N floats, loss: 1 float) that
+// k_reduce_rows folds in a fixed order, so results are deterministic.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace wgfft {
+
+struct AdjointArgs {
+    const float2* x;
+    const float* w;
+    const float* mask;  // fwd_adjoint
+    const float2* gy;   // vjp
+    float2* y;          // fwd_adjoint, optional
+    float2* gx;
+    const float2* tw;
+    float* part_gw;     // [grid][N]
+    float* part_loss;   // [grid]
+    uint64_t batch;
+    int grid;
+    hipStream_t stream;
+};
+
+template <int N, int E, int T>
+__device__ __forceinline__ void load_real_pairs(float (&r)[E], const float* __restrict__ src, uint32_t t) {
+    const float2* s2 = reinterpret_cast<const float2*>(src);
+#pragma unroll
+    for (int q = 0; q < E / 2; ++q) {
+        float2 v = s2[t + T * q];
+        r[2 * q] = v.x;
+        r[2 * q + 1] = v.y;
+    }
+}
+
+template <int N, int E, bool FWD, int... Rs>
+__global__ void __launch_bounds__(N / E)
+k_adjoint(AdjointArgs a) {
+    constexpr int T = N / E;
+    constexpr int LEN = LdsLayout<N, E, Rs...>::len;
+    __shared__ __align__(16) float2 lds[LEN];
+    __shared__ float red[T];
+
+    const uint32_t t = threadIdx.x;
+    float acc[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) acc[i] = 0.f;
+    float lacc = 0.f;
+
+    // w and m are re-read per transform (L1/L2 hits) to keep registers for
+    // x, the transform data and the gw accumulators.
+    for (uint64_t tr = blockIdx.x; tr < a.batch; tr += gridDim.x) {
+        float2 xs[E], d[E];
+        load_pairs<N, E, T>(xs, a.x + tr * N, t);
+        if constexpr (FWD) {
+            {
+                float wv[E];
+                load_real_pairs<N, E, T>(wv, a.w, t);
+#pragma unroll
+                for (int i = 0; i < E; ++i) d[i] = cscale(xs[i], wv[i]);
+            }
+            Engine<N, E, T, false>::run(d, lds, a.tw, t, Radices<Rs...>{});
+            if (a.y) store_pairs<N, E, T, false>(a.y + tr * N, d, t, 1.f);
+            // gY = 2 m Y; loss += m |Y|^2
+            float mv[E];
+            load_real_pairs<N, E, T>(mv, a.mask, t);
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                lacc += mv[i] * (d[i].x * d[i].x + d[i].y * d[i].y);
+                d[i] = cscale(d[i], 2.f * mv[i]);
+            }
+            WGFFT_LDS_SYNC();  // forward's last exchange read before the adjoint writes
+        } else {
+            load_pairs<N, E, T>(d, a.gy + tr * N, t);
+        }
+        Engine<N, E, T, true>::run(d, lds, a.tw, t, Radices<Rs...>{});
+        // gu = d; gx = w gu; gw += Re(conj(gu) x)
+        float wv[E];
+        load_real_pairs<N, E, T>(wv, a.w, t);
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            acc[i] += d[i].x * xs[i].x + d[i].y * xs[i].y;
+            d[i] = cscale(d[i], wv[i]);
+        }
+        store_pairs<N, E, T, false>(a.gx + tr * N, d, t, 1.f);
+        WGFFT_LDS_SYNC();
+    }
+    float2* pg = reinterpret_cast<float2*>(a.part_gw + (uint64_t)blockIdx.x * N);
+#pragma unroll
+    for (int q = 0; q < E / 2; ++q) pg[t + T * q] = make_float2(acc[2 * q], acc[2 * q + 1]);
+    if constexpr (FWD) {
+        red[t] = lacc;
+        __syncthreads();
+        if (t == 0) {
+            float s = 0.f;
+            for (int i = 0; i < T; ++i) s += red[i];
+            a.part_loss[blockIdx.x] = s;
+        }
+    }
+}
+
+template <int N, int E, int... Rs>
+struct AdjointKernel {
+    static constexpr int THREADS = N / E;
+    template <bool FWD>
+    static hipError_t go(const AdjointArgs& a) {
+        hipLaunchKernelGGL((k_adjoint<N, E, FWD, Rs...>), dim3((unsigned)a.grid), dim3(THREADS), 0, a.stream, a);
+        return hipGetLastError();
+    }
+    static hipError_t launch(bool fwd, const AdjointArgs& a) { return fwd ? go<true>(a) : go<false>(a); }
+    template <bool FWD>
+    static int occ() {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_adjoint<N, E, FWD, Rs...>, THREADS,
+                                                         0) != hipSuccess)
+            return 1;
+        return blocks > 0 ? blocks : 1;
+    }
+    static int blocks_per_cu(bool fwd) { return fwd ? occ<true>() : occ<false>(); }
+};
+
+struct AdjointVariant {
+    int log2n;
+    int E;
+    const char* plan;
+    hipError_t (*launch)(bool, const AdjointArgs&);
+    int (*blocks_per_cu)(bool);
+};
+
+std::vector<AdjointVariant>& adjoint_registry();
+
+struct AdjointRegistrar {
+    explicit AdjointRegistrar(const AdjointVariant& v) { adjoint_registry().push_back(v); }
+};
+
+#define WGFFT_REGISTER_ADJOINT(N, E, PLAN, ...)                                                        \
+    static ::wgfft::AdjointRegistrar _wgfft_adj_##N##_##E##_##PLAN(::wgfft::AdjointVariant{           \
+        ::wgfft::ilog2(N), E, #PLAN, &::wgfft::AdjointKernel<N, E, __VA_ARGS__>::launch,               \
+        &::wgfft::AdjointKernel<N, E, __VA_ARGS__>::blocks_per_cu})
+
+}  // namespace wgfft

@@ -1,19 +1,378 @@
-// ragged.hip -- mixed-length batches (placeholder until the persistent
-// work-queue kernel lands).
+// ragged.hip -- mixed-length batches in one persistent launch (no
+// counterpart in the reference, which needs one GpuFft call per length,
+// src/lib.rs:527-531).
+//
+// The host sorts the transforms by length (largest first) and cuts them into
+// work units: for N <= 8192 a unit is G transforms of one length that a
+// 256-thread workgroup transforms together in LDS (T*G = 256); for
+// N = 2^14 .. 2^16 a unit is one transform that the workgroup does as a
+// two-pass Stockham (radix R1 then R2 LDS sub-FFTs) through its own slot of
+// global scratch.  Units never depend on each other, so the only shared
+// state is the queue head.  A persistent grid (occupancy x CUs) dequeues
+// units with one atomic per unit; the last workgroup to leave resets the
+// counters, so the launch can be captured once into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <numeric>
+#include <vector>
+
 #include "../../include/wgfft.h"
+#include "kernels.hpp"
 #include "runtime_internal.hpp"
+
+namespace wgfft {
+
+struct RaggedUnit {
+    uint32_t log2n;
+    uint32_t first;  // index into the sorted offset table
+    uint32_t count;  // transforms in the unit
+    uint32_t pad;
+};
+
+constexpr int kRaggedThreads = 256;
+constexpr int kMinLog2 = 6, kMaxLog2 = 16, kDirectMaxLog2 = 13;
+
+// transforms per direct unit for length 2^k with E = 32
+constexpr int direct_g(int log2n) { return kRaggedThreads / ((1 << log2n) / 32); }
+
+// LDS footprint (complex elements) of every unit kind
+template <int N, int E, int... Rs>
+constexpr int direct_lds() { return direct_g(ilog2(N)) * LdsLayout<N, E, Rs...>::len; }
+template <int R, int E, int... Rs>
+constexpr int pass_lds() { return (kRaggedThreads / (R / E)) * GlobalPassSlot<R, E, Rs...>::len; }
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int kRaggedLds = cmax(
+    cmax(cmax(direct_lds<64, 32, 16, 4>(), direct_lds<128, 32, 16, 8>()),
+         cmax(direct_lds<256, 32, 16, 16>(), direct_lds<512, 32, 16, 16, 2>())),
+    cmax(cmax(cmax(direct_lds<1024, 32, 16, 16, 4>(), direct_lds<2048, 32, 16, 16, 8>()),
+              cmax(direct_lds<4096, 32, 16, 16, 16>(), direct_lds<8192, 32, 16, 16, 16, 2>())),
+         cmax(pass_lds<128, 32, 16, 8>(), pass_lds<256, 32, 16, 16>())));
+
+template <int N, int E, bool INV, int... Rs>
+__device__ __forceinline__ void ragged_direct(const float2* __restrict__ in, float2* __restrict__ out,
+                                              const uint64_t* __restrict__ offs, uint32_t first, uint32_t count,
+                                              const float2* __restrict__ tw_all, float2* __restrict__ lds_raw) {
+    constexpr int T = N / E;
+    constexpr int LEN = LdsLayout<N, E, Rs...>::len;
+    static_assert(T * direct_g(ilog2(N)) == kRaggedThreads, "direct unit must fill the workgroup");
+    const uint32_t g = threadIdx.x / T, t = threadIdx.x % T;
+    const bool active = g < count;
+    const uint64_t off = active ? offs[first + g] : 0;
+    float2 d[E];
+    if (active) load_pairs<N, E, T>(d, in + off, t);
+    Engine<N, E, T, INV>::run(d, lds_raw + g * LEN, tw_all + N, t, Radices<Rs...>{});
+    if (active) store_pairs<N, E, T, INV>(out + off, d, t, 1.0f / (float)N);
+}
+
+// Two-pass transform of one length-N signal (N = R1 * R2) by one workgroup:
+// pass 1 (L = 1) src -> scratch, pass 2 (L = R1) scratch -> dst.
+template <int LOGN, int R1, int E1, class P1, int R2, int E2, class P2, bool INV>
+struct RaggedBig;
+
+template <int LOGN, int R1, int E1, int... Rs1, int R2, int E2, int... Rs2, bool INV>
+struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV> {
+    static constexpr uint64_t N = 1ull << LOGN;
+    static constexpr int G1 = kRaggedThreads / (R1 / E1);
+    static constexpr int G2 = kRaggedThreads / (R2 / E2);
+    static_assert(R1 * R2 == (1 << LOGN), "radices must multiply to N");
+    static_assert(G2 <= R1, "second pass needs G <= L");
+    __device__ __forceinline__ static void run(const float2* __restrict__ src, float2* __restrict__ dst,
+                                               float2* __restrict__ scratch, const float2* __restrict__ tw_all,
+                                               float2* __restrict__ lds) {
+        constexpr int M = INV ? kInverse : kForward;
+        const TwOneLevel twN{tw_all + N};
+        for (uint64_t b0 = 0; b0 < N / R1; b0 += G1) {
+            global_pass_group<R1, E1, G1, M, TwOneLevel, Rs1...>(src, scratch, tw_all + R1, twN, nullptr, N, 1, b0,
+                                                                 1.0f, lds);
+            WGFFT_LDS_SYNC();
+        }
+        __syncthreads();  // scratch stores of the whole workgroup visible to its loads
+        const float sc = INV ? 1.0f / (float)N : 1.0f;
+        for (uint64_t b0 = 0; b0 < N / R2; b0 += G2) {
+            global_pass_group<R2, E2, G2, M, TwOneLevel, Rs2...>(scratch, dst, tw_all + R2, twN, nullptr, N, R1, b0,
+                                                                 sc, lds);
+            WGFFT_LDS_SYNC();
+        }
+        __syncthreads();  // the next unit reuses this scratch slot
+    }
+};
+
+template <bool INV>
+__global__ void __launch_bounds__(kRaggedThreads)
+k_ragged(const float2* __restrict__ in, float2* __restrict__ out, const RaggedUnit* __restrict__ units,
+         uint32_t u_begin, uint32_t u_end, const uint64_t* __restrict__ offs, const float2* __restrict__ tw_all,
+         float2* __restrict__ scratch, uint32_t* __restrict__ counters) {
+    __shared__ __align__(16) float2 lds[kRaggedLds];
+    __shared__ uint32_t s_unit;
+    float2* my_scratch = scratch + (uint64_t)blockIdx.x * (1ull << kMaxLog2);
+
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_unit = u_begin + __hip_atomic_fetch_add(&counters[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const uint32_t u = s_unit;
+        __syncthreads();
+        if (u >= u_end) break;
+        const RaggedUnit un = units[u];
+        switch (un.log2n) {
+            case 6: ragged_direct<64, 32, INV, 16, 4>(in, out, offs, un.first, un.count, tw_all, lds); break;
+            case 7: ragged_direct<128, 32, INV, 16, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
+            case 8: ragged_direct<256, 32, INV, 16, 16>(in, out, offs, un.first, un.count, tw_all, lds); break;
+            case 9: ragged_direct<512, 32, INV, 16, 16, 2>(in, out, offs, un.first, un.count, tw_all, lds); break;
+            case 10: ragged_direct<1024, 32, INV, 16, 16, 4>(in, out, offs, un.first, un.count, tw_all, lds); break;
+            case 11: ragged_direct<2048, 32, INV, 16, 16, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
+            case 12: ragged_direct<4096, 32, INV, 16, 16, 16>(in, out, offs, un.first, un.count, tw_all, lds); break;
+            case 13:
+                ragged_direct<8192, 32, INV, 16, 16, 16, 2>(in, out, offs, un.first, un.count, tw_all, lds);
+                break;
+            case 14: {
+                const uint64_t o = offs[un.first];
+                RaggedBig<14, 128, 32, Radices<16, 8>, 128, 32, Radices<16, 8>, INV>::run(in + o, out + o, my_scratch,
+                                                                                         tw_all, lds);
+                break;
+            }
+            case 15: {
+                const uint64_t o = offs[un.first];
+                RaggedBig<15, 256, 32, Radices<16, 16>, 128, 32, Radices<16, 8>, INV>::run(in + o, out + o,
+                                                                                          my_scratch, tw_all, lds);
+                break;
+            }
+            default: {
+                const uint64_t o = offs[un.first];
+                RaggedBig<16, 256, 32, Radices<16, 16>, 256, 32, Radices<16, 16>, INV>::run(in + o, out + o,
+                                                                                           my_scratch, tw_all, lds);
+                break;
+            }
+        }
+        WGFFT_LDS_SYNC();
+    }
+    // the last workgroup out resets the queue for the next launch / replay
+    if (threadIdx.x == 0) {
+        const uint32_t prev = __hip_atomic_fetch_add(&counters[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == gridDim.x - 1) {
+            __hip_atomic_exchange(&counters[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_exchange(&counters[1], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+static int ragged_blocks_per_cu(bool inv) {
+    int b = 0;
+    const void* f = inv ? (const void*)k_ragged<true> : (const void*)k_ragged<false>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, f, kRaggedThreads, 0) != hipSuccess || b < 1) b = 1;
+    return b;
+}
+
+}  // namespace wgfft
 
 using namespace wgfft;
 
 struct wgfft_ragged_plan {
-    int unused;
+    wgfft_t* h = nullptr;
+    uint64_t count = 0;
+    uint64_t total_samples = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> size_ranges;  // unit ranges per length (largest first)
+    uint32_t nunits = 0;
+    void* d_units = nullptr;
+    void* d_offs = nullptr;
+    void* d_tw = nullptr;
+    void* d_counters = nullptr;
+    void* d_scratch = nullptr;
+    int grid = 0;
+    struct GraphKey {
+        const void* in;
+        void* out;
+        int dir;
+        void* stream;
+        bool operator<(const GraphKey& o) const {
+            return std::tie(in, out, dir, stream) < std::tie(o.in, o.out, o.dir, o.stream);
+        }
+    };
+    std::map<GraphKey, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
 };
 
-extern "C" int wgfft_ragged_create(wgfft_t*, const uint64_t*, const uint32_t*, uint64_t, wgfft_ragged_t** out) {
+static void ragged_free(wgfft_ragged_t* p) {
+    if (!p) return;
+    for (auto& kv : p->graphs) {
+        hipGraphExecDestroy(kv.second.second);
+        hipGraphDestroy(kv.second.first);
+    }
+    for (void* q : {p->d_units, p->d_offs, p->d_tw, p->d_counters, p->d_scratch})
+        if (q) hipFree(q);
+    delete p;
+}
+
+extern "C" int wgfft_ragged_create(wgfft_t* h, const uint64_t* offsets, const uint32_t* log2n, uint64_t count,
+                                   wgfft_ragged_t** out) {
     if (out) *out = nullptr;
-    return fail(WGFFT_ERR_INVALID_ARGUMENT, "mixed-length batches are not available in this build");
+    if (!h || !out) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle or out is NULL");
+    if (count && (!offsets || !log2n)) return fail(WGFFT_ERR_INVALID_ARGUMENT, "offset or length table is NULL");
+    if (count > 0xFFFFFFFFull) return fail(WGFFT_ERR_INVALID_ARGUMENT, "too many transforms");
+    // validate lengths, alignment and overlap
+    std::vector<uint32_t> by_off(count);
+    std::iota(by_off.begin(), by_off.end(), 0u);
+    for (uint64_t i = 0; i < count; ++i) {
+        if (log2n[i] < (uint32_t)kMinLog2 || log2n[i] > (uint32_t)kMaxLog2)
+            return fail(WGFFT_ERR_INVALID_LENGTH, "mixed-length transforms must have 2^6 <= N <= 2^16");
+        if (offsets[i] & 1ull) return fail(WGFFT_ERR_INVALID_ARGUMENT, "offsets must be even (16-byte aligned)");
+    }
+    std::sort(by_off.begin(), by_off.end(), [&](uint32_t a, uint32_t b) { return offsets[a] < offsets[b]; });
+    for (uint64_t i = 1; i < count; ++i) {
+        const uint32_t a = by_off[i - 1], b = by_off[i];
+        if (offsets[a] + (1ull << log2n[a]) > offsets[b]) return fail(WGFFT_ERR_INVALID_ARGUMENT, "transforms overlap");
+    }
+    // sort by length (largest first), stable in index order
+    std::vector<uint32_t> order(count);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return log2n[a] > log2n[b]; });
+    std::vector<uint64_t> offs(count);
+    std::vector<RaggedUnit> units;
+    std::vector<std::pair<uint32_t, uint32_t>> ranges;
+    uint64_t samples = 0;
+    for (uint64_t i = 0; i < count;) {
+        const uint32_t k = log2n[order[i]];
+        const uint32_t u0 = (uint32_t)units.size();
+        uint64_t j = i;
+        while (j < count && log2n[order[j]] == k) {
+            offs[j] = offsets[order[j]];
+            samples += 1ull << k;
+            ++j;
+        }
+        const uint32_t per = k <= (uint32_t)kDirectMaxLog2 ? (uint32_t)direct_g((int)k) : 1u;
+        for (uint64_t s = i; s < j; s += per) {
+            const uint32_t c = (uint32_t)std::min<uint64_t>(per, j - s);
+            units.push_back(RaggedUnit{k, (uint32_t)s, c, 0u});
+        }
+        ranges.push_back({u0, (uint32_t)units.size()});
+        i = j;
+    }
+
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    auto* p = new (std::nothrow) wgfft_ragged_plan();
+    if (!p) return fail(WGFFT_ERR_OUT_OF_MEMORY, "plan allocation failed");
+    p->h = h;
+    p->count = count;
+    p->total_samples = samples;
+    p->size_ranges = ranges;
+    p->nunits = (uint32_t)units.size();
+    p->grid = std::max(ragged_blocks_per_cu(false), 1) * h->num_cus;
+    // twiddles: tw_all[N + j] = W_N^j for N = 2 .. 2^16
+    const uint64_t tw_n = 2ull << kMaxLog2;
+    std::vector<float2> tw(tw_n);
+    for (int k = 0; k <= kMaxLog2; ++k) {
+        const uint64_t n = 1ull << k;
+        for (uint64_t j = 0; j < n; ++j) {
+            double a = 2.0 * M_PI * ((double)j / (double)n);
+            tw[n + j] = make_float2((float)std::cos(a), (float)-std::sin(a));
+        }
+    }
+    tw[0] = make_float2(1.f, 0.f);
+    const uint64_t scratch_bytes = (uint64_t)p->grid * (1ull << kMaxLog2) * sizeof(float2);
+    struct {
+        void** dst;
+        const void* src;
+        uint64_t bytes;
+    } uploads[] = {{&p->d_units, units.data(), units.size() * sizeof(RaggedUnit)},
+                   {&p->d_offs, offs.data(), offs.size() * sizeof(uint64_t)},
+                   {&p->d_tw, tw.data(), tw.size() * sizeof(float2)},
+                   {&p->d_counters, nullptr, 64},
+                   {&p->d_scratch, nullptr, scratch_bytes}};
+    for (auto& u : uploads) {
+        const uint64_t b = std::max<uint64_t>(u.bytes, 16);
+        if ((e = hipMalloc(u.dst, b)) != hipSuccess) {
+            (void)hipGetLastError();
+            ragged_free(p);
+            return fail(WGFFT_ERR_OUT_OF_MEMORY, std::string("hipMalloc(ragged plan) failed: ") + hipGetErrorString(e));
+        }
+        if (u.src && u.bytes) e = hipMemcpy(*u.dst, u.src, u.bytes, hipMemcpyHostToDevice);
+        else if (!u.src) e = hipMemset(*u.dst, 0, b);
+        if (e != hipSuccess) {
+            ragged_free(p);
+            return hip_fail(e, "ragged plan upload");
+        }
+    }
+    *out = p;
+    return WGFFT_OK;
 }
-extern "C" void wgfft_ragged_destroy(wgfft_ragged_t*) {}
-extern "C" int wgfft_ragged_exec_device(wgfft_ragged_t*, const float*, float*, int, int, void*) {
-    return fail(WGFFT_ERR_INVALID_ARGUMENT, "mixed-length batches are not available in this build");
+
+extern "C" void wgfft_ragged_destroy(wgfft_ragged_t* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(p->h->mu);
+    hipSetDevice(p->h->device);
+    hipDeviceSynchronize();
+    ragged_free(p);
 }
+
+static hipError_t ragged_launch(wgfft_ragged_t* p, const float* in, float* out, bool inv, uint32_t u0, uint32_t u1,
+                                int grid, hipStream_t s) {
+    if (inv)
+        hipLaunchKernelGGL(k_ragged<true>, dim3(grid), dim3(kRaggedThreads), 0, s, (const float2*)in, (float2*)out,
+                           (const RaggedUnit*)p->d_units, u0, u1, (const uint64_t*)p->d_offs,
+                           (const float2*)p->d_tw, (float2*)p->d_scratch, (uint32_t*)p->d_counters);
+    else
+        hipLaunchKernelGGL(k_ragged<false>, dim3(grid), dim3(kRaggedThreads), 0, s, (const float2*)in, (float2*)out,
+                           (const RaggedUnit*)p->d_units, u0, u1, (const uint64_t*)p->d_offs,
+                           (const float2*)p->d_tw, (float2*)p->d_scratch, (uint32_t*)p->d_counters);
+    return hipGetLastError();
+}
+
+extern "C" int wgfft_ragged_exec_device(wgfft_ragged_t* p, const float* d_in, float* d_out, int dir, int mode,
+                                        void* stream) {
+    if (!p) return fail(WGFFT_ERR_INVALID_ARGUMENT, "plan is NULL");
+    if (dir != WGFFT_FORWARD && dir != WGFFT_INVERSE)
+        return fail(WGFFT_ERR_INVALID_ARGUMENT, "direction must be WGFFT_FORWARD (-1) or WGFFT_INVERSE (+1)");
+    if (mode < 0 || mode > 2) return fail(WGFFT_ERR_INVALID_ARGUMENT, "mode must be 0, 1 or 2");
+    if (p->count == 0) return WGFFT_OK;
+    if (!d_in || !d_out) return fail(WGFFT_ERR_INVALID_ARGUMENT, "device buffer is NULL");
+    if ((((uintptr_t)d_in | (uintptr_t)d_out) & 15u) != 0)
+        return fail(WGFFT_ERR_INVALID_ARGUMENT, "device buffers must be 16-byte aligned");
+    wgfft_t* h = p->h;
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    const bool inv = dir == WGFFT_INVERSE;
+    if (mode == 0) {
+        if ((e = ragged_launch(p, d_in, d_out, inv, 0, p->nunits, p->grid, s)) != hipSuccess)
+            return hip_fail(e, "ragged kernel launch");
+        return WGFFT_OK;
+    }
+    if (mode == 2) {
+        for (auto& r : p->size_ranges) {
+            const int grid = (int)std::min<uint64_t>((uint64_t)p->grid, r.second - r.first);
+            if ((e = ragged_launch(p, d_in, d_out, inv, r.first, r.second, grid, s)) != hipSuccess)
+                return hip_fail(e, "ragged kernel launch");
+        }
+        return WGFFT_OK;
+    }
+    wgfft_ragged_plan::GraphKey key{d_in, d_out, dir, (void*)s};
+    auto it = p->graphs.find(key);
+    if (it == p->graphs.end()) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+            return hip_fail(e, "hipStreamBeginCapture");
+        hipError_t le = ragged_launch(p, d_in, d_out, inv, 0, p->nunits, p->grid, s);
+        e = hipStreamEndCapture(s, &g);
+        if (le != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return hip_fail(le, "ragged kernel capture");
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        if ((e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0)) != hipSuccess) {
+            hipGraphDestroy(g);
+            return hip_fail(e, "hipGraphInstantiate");
+        }
+        it = p->graphs.emplace(key, std::make_pair(g, ge)).first;
+    }
+    if ((e = hipGraphLaunch(it->second.second, s)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    return WGFFT_OK;
+}
+
+extern "C" uint64_t wgfft_ragged_samples(const wgfft_ragged_t* p) { return p ? p->total_samples : 0; }

@@ -13,9 +13,21 @@
 #include <algorithm>
 
 #include "../../include/wgfft.h"
+#include "kernels_adjoint.hpp"
 #include "runtime_internal.hpp"
 
 namespace wgfft {
+
+std::vector<AdjointVariant>& adjoint_registry() {
+    static std::vector<AdjointVariant> v;
+    return v;
+}
+
+static const AdjointVariant* find_adjoint(int log2n) {
+    for (const auto& v : adjoint_registry())
+        if (v.log2n == log2n) return &v;
+    return nullptr;
+}
 
 constexpr int kRowChunks = 64;  // batch chunks of the gw partial sums
 
@@ -100,6 +112,30 @@ static int vjp_finish(Handle* h, const float* d_x, const float* d_w, float* d_gx
     return WGFFT_OK;
 }
 
+// One fused launch (plus the fixed-order reductions).  fwd = true: loss
+// L = sum m |FFT(w x)|^2 with gY = 2 m Y formed in registers; fwd = false:
+// the given gY.
+static int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x, const float* d_w,
+                         const float* d_mask, const float* d_gy, float* d_y, float* d_gx, float* d_gw,
+                         float* d_loss, uint64_t n, uint64_t batch, const float2* tw, hipStream_t s) {
+    int grid = v->blocks_per_cu(fwd) * h->num_cus;
+    if ((uint64_t)grid > batch) grid = (int)batch;
+    int rc;
+    if ((rc = h->ensure(h->red, (uint64_t)grid * n * sizeof(float))) ||
+        (rc = h->ensure(h->lossbuf, (uint64_t)grid * sizeof(float) + 256)))
+        return rc;
+    AdjointArgs a{(const float2*)d_x, d_w, d_mask, (const float2*)d_gy, (float2*)d_y, (float2*)d_gx, tw,
+                  (float*)h->red.p, (float*)h->lossbuf.p, batch, grid, s};
+    hipError_t e = v->launch(fwd, a);
+    if (e != hipSuccess) return hip_fail(e, "adjoint kernel launch");
+    hipLaunchKernelGGL(k_sum_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const float*)h->red.p, d_gw,
+                       n, (uint64_t)grid);
+    if (fwd) hipLaunchKernelGGL(k_sum_all, dim3(1), dim3(256), 0, s, (const float*)h->lossbuf.p, d_loss, grid);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "adjoint reduction launch");
+    return WGFFT_OK;
+}
+
 }  // namespace wgfft
 
 using namespace wgfft;
@@ -128,6 +164,11 @@ extern "C" int wgfft_vjp_windowed_device(wgfft_t* h, const float* d_x, const flo
     }
     Plan* p;
     if ((rc = h->get_plan(n, &p))) return rc;
+    if (p->kind == Plan::kBatched) {
+        if (const AdjointVariant* v = find_adjoint(p->log2n))
+            return fused_adjoint(h, v, false, d_x, d_w, nullptr, d_gy, nullptr, d_gx, d_gw, nullptr, n, batch, p->tw,
+                                 s);
+    }
     // gu = F^H gY: inverse-sign transform with unit scale
     if ((rc = h->run(p, kInverse, (const float2*)d_gy, (float2*)d_gx, nullptr, batch, 1.0f, s))) return rc;
     return vjp_finish(h, d_x, d_w, d_gx, d_gw, n, batch, s);
@@ -153,6 +194,11 @@ extern "C" int wgfft_fwd_adjoint_masked_device(wgfft_t* h, const float* d_x, con
     }
     Plan* p;
     if ((rc = h->get_plan(n, &p))) return rc;
+    if (p->kind == Plan::kBatched) {
+        if (const AdjointVariant* v = find_adjoint(p->log2n))
+            return fused_adjoint(h, v, true, d_x, d_w, d_mask, nullptr, d_y, d_gx, d_gw, d_loss, n, batch, p->tw, s);
+    }
+    // lengths without a fused kernel: forward, mask, adjoint as separate passes
     const uint64_t total = n * batch;
     // Y (into d_y or scratch), then gY = 2 m Y in the scratch copy
     if ((rc = h->ensure(h->io_aux, total * sizeof(float2)))) return rc;

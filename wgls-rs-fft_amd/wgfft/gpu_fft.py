@@ -195,6 +195,55 @@ class GpuFft(FftExecutor):
         return buf.value.decode()
 
 
+class RaggedPlan:
+    """Mixed-length batch: transform i has length 2^log2n[i] at complex
+    offset offsets[i] of one contiguous buffer (wgfft_ragged_create)."""
+
+    MODE_PERSISTENT, MODE_GRAPH, MODE_PER_SIZE = 0, 1, 2
+
+    def __init__(self, gpu: GpuFft, offsets, log2n):
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        k = np.ascontiguousarray(log2n, dtype=np.uint32)
+        if off.shape != k.shape:
+            raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT, "offsets and log2n differ in length")
+        p = ctypes.c_void_p()
+        check(gpu._L.wgfft_ragged_create(gpu._h, off.ctypes.data, k.ctypes.data, len(off), ctypes.byref(p)))
+        self._p = p
+        self._gpu = gpu
+        self.samples = int(gpu._L.wgfft_ragged_samples(p))
+
+    def exec_device(self, x, out, inverse: bool = False, mode: int = 0, stream=None) -> None:
+        """x, out: contiguous complex64 CUDA tensors holding the packed signals."""
+        import torch
+
+        for t in (x, out):
+            if not (t.is_cuda and t.dtype == torch.complex64 and t.is_contiguous()):
+                raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT, "expected contiguous complex64 CUDA tensors")
+        check(self._gpu._L.wgfft_ragged_exec_device(self._p, x.data_ptr(), out.data_ptr(),
+                                                    WGFFT_INVERSE if inverse else WGFFT_FORWARD, int(mode),
+                                                    GpuFft._stream(stream)))
+
+    def close(self) -> None:
+        if getattr(self, "_p", None):
+            self._gpu._L.wgfft_ragged_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ragged_layout(log2n) -> tuple[np.ndarray, int]:
+    """Contiguous packing of lengths 2^log2n: (offsets, total samples)."""
+    lens = (np.uint64(1) << np.asarray(log2n, dtype=np.uint64))
+    offsets = np.zeros(len(lens), dtype=np.uint64)
+    if len(lens) > 1:
+        offsets[1:] = np.cumsum(lens)[:-1]
+    return offsets, int(lens.sum())
+
+
 def _check_tensor(t) -> None:
     import torch
 

@@ -2,12 +2,20 @@
 """Headline benchmark: batched complex64 forward FFT, N = 4096 x 4096.
 
 One step = one forward transform of 4096 signals of length 4096 (16.8 M
-samples, 128 MiB in + 128 MiB out) with inputs resident in HBM, through the
-C-ABI entry wgfft_exec_device on torch's current stream.  Also timed: the
-fused forward + adjoint of L = sum m |FFT(w x)|^2 on the same shape (ratio
-to the plain forward), and the reference algorithm on the host CPU (the
-oracle's restatement of the wgls-rs-fft f32 Stockham R4/R2 path) on a
-bounded sample.  Prints one JSON line.
+samples, 128 MiB in + 128 MiB out) with inputs resident in HBM, launched
+through the C-ABI entry wgfft_exec_device on torch's current stream and
+timed with HIP events on that stream.
+
+The same JSON line also carries, measured in the same run:
+  * roofline: algorithmic bytes (16 B/sample) / kernel time vs 8 TB/s, and
+    the bandwidth of a plain device copy of the same 256 MiB;
+  * fwd_adjoint: the fused forward + adjoint of L = sum m |FFT(w x)|^2 on the
+    same shape and its time relative to the plain forward;
+  * mixed_batch: 20,000 transforms of lengths 2^k, k ~ U{6..16}, as one
+    persistent launch, its hipGraph replay, and one launch per length;
+  * large_n: N = 2^22 x 16 and N = 2^26 x 1 forward transforms;
+  * cpu_baseline: the reference algorithm (the oracle's C restatement of the
+    wgls-rs-fft f32 Stockham R4/R2 path) on the host cores, bounded sample.
 """
 from __future__ import annotations
 
@@ -28,8 +36,6 @@ METRIC = "complex64 GSamples/s & %HBM-roofline, N=4096×4096; fwd+adjoint vs pla
 def cpu_baseline(seconds_target: float = 12.0) -> dict:
     """Reference algorithm (f32 Stockham R4/R2, restated in C) on host cores."""
     sys.path.insert(0, ROOT)
-    import numpy as np
-
     import oracle
 
     cores = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
@@ -43,8 +49,72 @@ def cpu_baseline(seconds_target: float = 12.0) -> dict:
         dt = time.perf_counter() - t0
         if dt >= seconds_target:
             break
-    return {"value": total * N / dt / 1e9, "unit": "GSamples/s", "cores": cores, "kind": "port",
+    return {"value": round(total * N / dt / 1e9, 4), "unit": "GSamples/s", "cores": cores, "kind": "port",
             "sample": f"{total} transforms of N={N} (f32 Stockham R4/R2 restatement, {dt:.1f} s)"}
+
+
+def timed(torch, fn, steps: int, warmup: int) -> float:
+    """Average ms per call of fn() with HIP events on the current stream."""
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def bench_mixed(torch, gpu, wgfft, steps: int) -> dict:
+    k = _ragged_log2(20000, 6, 16, 0x5EED0003)
+    offsets, total = wgfft.ragged_layout(k)
+    plan = wgfft.RaggedPlan(gpu, offsets, k)
+    x = (torch.rand(2 * total, device="cuda") * 2 - 1).view(torch.complex64)
+    y = torch.empty_like(x)
+    out = {"transforms": 20000, "samples": int(total), "bytes": 16 * int(total)}
+    for mode, name in ((0, "persistent"), (1, "graph"), (2, "per_length")):
+        ms = timed(torch, lambda: plan.exec_device(x, y, mode=mode), max(5, steps // 10), 3)
+        gbs = 16.0 * total / (ms * 1e-3) / 1e9
+        out[name] = {"ms": round(ms, 4), "gsamples_s": round(total / (ms * 1e-3) / 1e9, 2),
+                     "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    plan.close()
+    del x, y
+    return out
+
+
+def _ragged_log2(count, kmin, kmax, seed):
+    """log2 lengths kmin + (u32 >> 8) % span, one PCG32 draw per transform
+    (the draws of the parity tests), recovered exactly from the library's
+    uniform generator, which emits (u32 >> 8) * 2^-23 - 1 per draw."""
+    import numpy as np
+
+    import wgfft
+
+    raw = np.empty(count, dtype=np.float32)
+    wgfft._lib.load().wgfft_synth_uniform(raw.ctypes.data, count, seed)
+    v = np.round((raw.astype(np.float64) + 1.0) * 8388608.0).astype(np.int64)
+    return (kmin + v % (kmax - kmin + 1)).astype(np.uint32)
+
+
+def bench_large(torch, gpu, steps: int) -> dict:
+    out = {}
+    for log2n, batch in ((22, 16), (26, 1)):
+        n = 1 << log2n
+        x = (torch.rand(2 * n * batch, device="cuda") * 2 - 1).view(torch.complex64).view(batch, n)
+        y = torch.empty_like(x)
+        ms = timed(torch, lambda: gpu.exec_device(x, y), max(5, steps // 20), 2)
+        ms_inv = timed(torch, lambda: gpu.exec_device(y, x, inverse=True), max(5, steps // 20), 2)
+        s = n * batch
+        gbs = 16.0 * s / (ms * 1e-3) / 1e9
+        out[f"2^{log2n}x{batch}"] = {"fwd_ms": round(ms, 4), "inv_ms": round(ms_inv, 4),
+                                     "gsamples_s": round(s / (ms * 1e-3) / 1e9, 2),
+                                     "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                     "plan": gpu.describe(n)}
+        del x, y
+    return out
 
 
 def main() -> int:
@@ -53,6 +123,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--headline-only", action="store_true", help="only the N=4096x4096 forward (profiling)")
+    ap.add_argument("--no-tune", action="store_true", help="use the default kernel variants")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -64,49 +136,17 @@ def main() -> int:
     import wgfft
 
     gpu = wgfft.GpuFft.new()
+    tuned = None if args.no_tune else gpu.autotune(N, BATCH)
     x = torch.from_numpy(wgfft.synth_uniform((BATCH, N), 0x5EED0002)).cuda()
     y = torch.empty_like(x)
-    stream = torch.cuda.current_stream()
 
-    for _ in range(args.warmup):
-        gpu.exec_device(x, y)
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        gpu.exec_device(x, y)
-    ev1.record(stream)
-    torch.cuda.synchronize()
+    kern_ms = timed(torch, lambda: gpu.exec_device(x, y), args.steps, args.warmup)
     wall = time.perf_counter() - t0
-    kern_ms = ev0.elapsed_time(ev1) / args.steps
-    ms_per_step = max(kern_ms, wall * 1e3 / args.steps)
     samples = N * BATCH
+    ms_per_step = kern_ms
     value = samples / (ms_per_step * 1e-3) / 1e9
     achieved = 16.0 * samples / (kern_ms * 1e-3) / 1e9
-
-    # fused forward + adjoint on the same shape
-    w = torch.from_numpy(np.hanning(N).astype(np.float32)).cuda()
-    m = (torch.rand(N, device="cuda") < 0.3).float()
-    gx = torch.empty_like(x)
-    gw = torch.empty(N, dtype=torch.float32, device="cuda")
-    loss = torch.empty(1, dtype=torch.float32, device="cuda")
-    fa_steps = max(10, args.steps // 4)
-    for _ in range(max(3, args.warmup // 4)):
-        gpu.fwd_adjoint_masked_device(x, w, m, None, gx, gw, loss)
-    torch.cuda.synchronize()
-    ev0.record(stream)
-    for _ in range(fa_steps):
-        gpu.fwd_adjoint_masked_device(x, w, m, None, gx, gw, loss)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    fa_ms = ev0.elapsed_time(ev1) / fa_steps
-
-    traffic = None
-    prof = os.path.join(ROOT, "profiles", "pmc_fft4096x4096.json")
-    if os.path.exists(prof):
-        with open(prof) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
 
     out = {
         "metric": METRIC,
@@ -121,13 +161,36 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "complex64 (f32)",
         "data": "synthetic: re/im i.i.d. U[-1,1) from SplitMix64->PCG32, resident in HBM",
-        "config": {"workload": "batched forward FFT", "n": N, "batch": BATCH, "samples": samples,
-                   "plan": gpu.describe(N)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
-        "fwd_adjoint": {"ms": round(fa_ms, 5), "plain_fwd_ms": round(kern_ms, 5),
-                        "ratio": round(fa_ms / kern_ms, 3)},
+        "config": {"workload": "batched forward FFT, N=4096, batch 4096", "n": N, "batch": BATCH,
+                   "samples": samples, "plan": gpu.describe(N), "autotuned": tuned},
     }
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "pmc_fft4096x4096.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    copy_ms = timed(torch, lambda: y.copy_(x), 50, 5)
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                       "copy_gbs": round(2 * x.numel() * 8 / (copy_ms * 1e-3) / 1e9, 1),
+                       "wall_s": round(wall, 3)}
+
+    if not args.headline_only:
+        w = torch.from_numpy(np.hanning(N).astype(np.float32)).cuda()
+        m = (torch.rand(N, device="cuda") < 0.3).float()
+        gx = torch.empty_like(x)
+        gw = torch.empty(N, dtype=torch.float32, device="cuda")
+        loss = torch.empty(1, dtype=torch.float32, device="cuda")
+        fa_ms = timed(torch, lambda: gpu.fwd_adjoint_masked_device(x, w, m, None, gx, gw, loss),
+                      max(10, args.steps // 4), max(3, args.warmup // 4))
+        out["fwd_adjoint"] = {"ms": round(fa_ms, 5), "plain_fwd_ms": round(kern_ms, 5),
+                              "ratio": round(fa_ms / kern_ms, 3)}
+        del gx
+        x_keep, y_keep = x, y
+        del x, y
+        out["mixed_batch"] = bench_mixed(torch, gpu, wgfft, args.steps)
+        out["large_n"] = bench_large(torch, gpu, args.steps)
+        del x_keep, y_keep
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -12,10 +12,10 @@
  *   Return value: WGFFT_OK (0) or a negative WGFFT_ERR_* code; the message
  *   of the last failure on the calling thread is wgfft_last_error().
  *   Nothing aborts or throws across this boundary.
- *   Host entry points are synchronous.  Device entry points take device
- *   pointers (16-byte aligned) and a hipStream_t passed as void* (NULL = the
- *   handle's own stream) and are stream-ordered: they return once the work
- *   is enqueued.
+ *   Host entry points are synchronous (they use the handle's own stream).
+ *   Device entry points take device pointers (16-byte aligned) and a
+ *   hipStream_t passed as void* (NULL = HIP's default null stream) and are
+ *   stream-ordered: they return once the work is enqueued.
  *   A handle may be used by one thread at a time (calls are serialised by an
  *   internal mutex); distinct handles are independent.
  */

@@ -55,7 +55,11 @@ def test_fused_fwd_adjoint_vs_fp64(gpu, torch_cuda, c5):
 
 def test_fused_vs_finite_differences(gpu, torch_cuda, c5):
     """64 coordinates: 48 of x (re/im) and 16 of w; FP64 central differences
-    of the FP64 loss (numpy.fft) against the GPU's f32 gradients."""
+    of the FP64 loss (numpy.fft) against the GPU's f32 gradients.  L is
+    exactly quadratic in x and in w, so the central difference has no
+    truncation error and h = 0.01 keeps FP64 rounding noise (eps*L/h) far
+    below the f32 error.  Tolerance: 1e-5 of the gradient's RMS plus 1e-4
+    relative."""
     torch = torch_cuda
     from wgfft.autograd import masked_power_loss
 
@@ -72,20 +76,20 @@ def test_fused_vs_finite_differences(gpu, torch_cuda, c5):
         return float(np.sum(m64 * np.abs(y) ** 2))
 
     rng = np.random.default_rng(0x5EED)
+    h = 1e-2
     for _ in range(24):
         b, n = int(rng.integers(B)), int(rng.integers(N))
+        scale = float(np.sqrt(np.mean(np.abs(gx[b]) ** 2)))
         for part in (1.0, 1j):
-            h = 1e-6 * max(1.0, abs(x64[b, n]))
             xp, xm = x64[b].copy(), x64[b].copy()
             xp[n] += h * part
             xm[n] -= h * part
             fd = (row_loss(xp, w64) - row_loss(xm, w64)) / (2 * h)
             an = gx[b, n].real if part == 1.0 else gx[b, n].imag
-            assert abs(fd - an) <= 2e-4 * max(1.0, abs(fd)), (b, n, part, fd, an)
-    y0 = np.fft.fft(w64[None] * x64, axis=-1)
+            assert abs(fd - an) <= 1e-5 * scale + 1e-4 * abs(fd), (b, n, part, fd, an, scale)
+    scale_w = float(np.sqrt(np.mean(gw**2)))
     for _ in range(16):
         n = int(rng.integers(N))
-        h = 1e-6 * max(1.0, abs(w64[n]))
         losses = []
         for sgn in (1, -1):
             wv = w64.copy()
@@ -93,8 +97,7 @@ def test_fused_vs_finite_differences(gpu, torch_cuda, c5):
             y = np.fft.fft(wv[None] * x64, axis=-1)
             losses.append(float(np.sum(m64[None] * np.abs(y) ** 2)))
         fd = (losses[0] - losses[1]) / (2 * h)
-        assert abs(fd - gw[n]) <= 2e-4 * max(1.0, abs(fd)), (n, fd, gw[n])
-    del y0
+        assert abs(fd - gw[n]) <= 1e-5 * scale_w + 1e-4 * abs(fd), (n, fd, gw[n], scale_w)
 
 
 def test_autograd_harness(gpu, torch_cuda):

@@ -146,38 +146,52 @@ struct Dft<16, INV> {
     }
 };
 
-// v[m] *= W_{RL}^{m k} = W_N^{m k s}, with tw[j] = W_N^j (forward sign) and
-// s = N / (R L).  The table supplies W^{ks}, W^{2ks}, W^{4ks}, W^{8ks}; the
-// other powers are products of at most two table entries.
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+// v * w (forward) or v * conj(w) (inverse), w in forward-sign form
+template <bool INV>
+__device__ __forceinline__ float2 twmul(float2 v, float2 w) {
+    return INV ? cmulc(v, w) : cmul(v, w);
+}
+
+// v[m] *= W_{RL}^{m k} = W_N^{m k s} (conjugated for the inverse), with
+// tw[j] = W_N^j (forward sign) and base = k s.  The table supplies W^{base},
+// W^{2 base}, W^{4 base}, W^{8 base}; the other powers are products of at
+// most two table entries.  The powers are formed in forward-sign form and
+// conjugated inside the multiply, so forward and inverse transforms of the
+// same length share them (they do not depend on the transform index and are
+// kept in registers across the transforms a workgroup handles).
 template <int R, bool INV>
 __device__ __forceinline__ void apply_twiddles(float2 (&v)[R], const float2* __restrict__ tw,
                                                uint32_t base) {
-    float2 w1 = conj_if<INV>(tw[base]);
+    float2 w1 = tw[base];
     if constexpr (R == 2) {
-        v[1] = cmul(v[1], w1);
+        v[1] = twmul<INV>(v[1], w1);
     } else {
-        float2 w2 = conj_if<INV>(tw[2 * base]);
+        float2 w2 = tw[2 * base];
         float2 w3 = cmul(w1, w2);
-        v[1] = cmul(v[1], w1);
-        v[2] = cmul(v[2], w2);
-        v[3] = cmul(v[3], w3);
+        v[1] = twmul<INV>(v[1], w1);
+        v[2] = twmul<INV>(v[2], w2);
+        v[3] = twmul<INV>(v[3], w3);
         if constexpr (R >= 8) {
-            float2 w4 = conj_if<INV>(tw[4 * base]);
+            float2 w4 = tw[4 * base];
             float2 w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
-            v[4] = cmul(v[4], w4);
-            v[5] = cmul(v[5], w5);
-            v[6] = cmul(v[6], w6);
-            v[7] = cmul(v[7], w7);
+            v[4] = twmul<INV>(v[4], w4);
+            v[5] = twmul<INV>(v[5], w5);
+            v[6] = twmul<INV>(v[6], w6);
+            v[7] = twmul<INV>(v[7], w7);
             if constexpr (R == 16) {
-                float2 w8 = conj_if<INV>(tw[8 * base]);
-                v[8] = cmul(v[8], w8);
-                v[9] = cmul(v[9], cmul(w8, w1));
-                v[10] = cmul(v[10], cmul(w8, w2));
-                v[11] = cmul(v[11], cmul(w8, w3));
-                v[12] = cmul(v[12], cmul(w8, w4));
-                v[13] = cmul(v[13], cmul(w8, w5));
-                v[14] = cmul(v[14], cmul(w8, w6));
-                v[15] = cmul(v[15], cmul(w8, w7));
+                float2 w8 = tw[8 * base];
+                v[8] = twmul<INV>(v[8], w8);
+                v[9] = twmul<INV>(v[9], cmul(w8, w1));
+                v[10] = twmul<INV>(v[10], cmul(w8, w2));
+                v[11] = twmul<INV>(v[11], cmul(w8, w3));
+                v[12] = twmul<INV>(v[12], cmul(w8, w4));
+                v[13] = twmul<INV>(v[13], cmul(w8, w5));
+                v[14] = twmul<INV>(v[14], cmul(w8, w6));
+                v[15] = twmul<INV>(v[15], cmul(w8, w7));
             }
         }
     }

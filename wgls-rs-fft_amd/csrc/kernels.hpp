@@ -255,8 +255,8 @@ __device__ __forceinline__ void global_pass_group(const float2* __restrict__ src
         }
         if (!first) {
             const uint64_t k0 = (b0 + 2 * gp) & (L - 1);
-            a = cmul(a, conj_if<INV>(twN(((uint64_t)m * k0 * s) & (N - 1))));
-            b = cmul(b, conj_if<INV>(twN(((uint64_t)m * (k0 + 1) * s) & (N - 1))));
+            a = twmul<INV>(a, twN(((uint64_t)m * k0 * s) & (N - 1)));
+            b = twmul<INV>(b, twN(((uint64_t)m * (k0 + 1) * s) & (N - 1)));
         }
         const uint32_t pm = pad_idx<5, 0>(m);
         lds_all[(2 * gp) * SLOT + pm] = a;

@@ -41,7 +41,11 @@ __device__ __forceinline__ void load_real_pairs(float (&r)[E], const float* __re
     }
 }
 
-template <int N, int E, bool FWD, int... Rs>
+// KEEPX: hold x in registers from the load to the gw accumulation (one HBM
+// read, more registers) or re-read it after the adjoint transform (the
+// workgroup read it microseconds earlier, so it comes from cache; fewer
+// registers, higher occupancy).  The VJP (FWD = false) always re-reads.
+template <int N, int E, bool FWD, bool KEEPX, int... Rs>
 __global__ void __launch_bounds__(N / E)
 k_adjoint(AdjointArgs a) {
     constexpr int T = N / E;
@@ -58,16 +62,23 @@ k_adjoint(AdjointArgs a) {
     // w and m are re-read per transform (L1/L2 hits) to keep registers for
     // x, the transform data and the gw accumulators.
     for (uint64_t tr = blockIdx.x; tr < a.batch; tr += gridDim.x) {
-        float2 xs[E], d[E];
-        load_pairs<N, E, T>(xs, a.x + tr * N, t);
+        float2 d[E];
+        [[maybe_unused]] float2 xk[KEEPX && FWD ? E : 1];
         if constexpr (FWD) {
             {
+                float2 xs[E];
                 float wv[E];
+                load_pairs<N, E, T>(xs, a.x + tr * N, t);
                 load_real_pairs<N, E, T>(wv, a.w, t);
 #pragma unroll
-                for (int i = 0; i < E; ++i) d[i] = cscale(xs[i], wv[i]);
+                for (int i = 0; i < E; ++i) {
+                    if constexpr (KEEPX) xk[i] = xs[i];
+                    d[i] = cscale(xs[i], wv[i]);
+                }
             }
             Engine<N, E, T, false>::run(d, lds, a.tw, t, Radices<Rs...>{});
+            // keep the epilogue's loads out of the transform's register peak
+            __builtin_amdgcn_sched_barrier(0);
             if (a.y) store_pairs<N, E, T, false>(a.y + tr * N, d, t, 1.f);
             // gY = 2 m Y; loss += m |Y|^2
             float mv[E];
@@ -78,11 +89,20 @@ k_adjoint(AdjointArgs a) {
                 d[i] = cscale(d[i], 2.f * mv[i]);
             }
             WGFFT_LDS_SYNC();  // forward's last exchange read before the adjoint writes
+            __builtin_amdgcn_sched_barrier(0);
         } else {
             load_pairs<N, E, T>(d, a.gy + tr * N, t);
         }
         Engine<N, E, T, true>::run(d, lds, a.tw, t, Radices<Rs...>{});
+        __builtin_amdgcn_sched_barrier(0);
         // gu = d; gx = w gu; gw += Re(conj(gu) x)
+        float2 xs[E];
+        if constexpr (KEEPX && FWD) {
+#pragma unroll
+            for (int i = 0; i < E; ++i) xs[i] = xk[i];
+        } else {
+            load_pairs<N, E, T>(xs, a.x + tr * N, t);
+        }
         float wv[E];
         load_real_pairs<N, E, T>(wv, a.w, t);
 #pragma unroll
@@ -107,20 +127,21 @@ k_adjoint(AdjointArgs a) {
     }
 }
 
-template <int N, int E, int... Rs>
+template <int N, int E, bool KEEPX, int... Rs>
 struct AdjointKernel {
     static constexpr int THREADS = N / E;
     template <bool FWD>
     static hipError_t go(const AdjointArgs& a) {
-        hipLaunchKernelGGL((k_adjoint<N, E, FWD, Rs...>), dim3((unsigned)a.grid), dim3(THREADS), 0, a.stream, a);
+        hipLaunchKernelGGL((k_adjoint<N, E, FWD, KEEPX, Rs...>), dim3((unsigned)a.grid), dim3(THREADS), 0, a.stream,
+                           a);
         return hipGetLastError();
     }
     static hipError_t launch(bool fwd, const AdjointArgs& a) { return fwd ? go<true>(a) : go<false>(a); }
     template <bool FWD>
     static int occ() {
         int blocks = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_adjoint<N, E, FWD, Rs...>, THREADS,
-                                                         0) != hipSuccess)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_adjoint<N, E, FWD, KEEPX, Rs...>,
+                                                         THREADS, 0) != hipSuccess)
             return 1;
         return blocks > 0 ? blocks : 1;
     }
@@ -130,6 +151,7 @@ struct AdjointKernel {
 struct AdjointVariant {
     int log2n;
     int E;
+    int keepx;
     const char* plan;
     hipError_t (*launch)(bool, const AdjointArgs&);
     int (*blocks_per_cu)(bool);
@@ -141,9 +163,9 @@ struct AdjointRegistrar {
     explicit AdjointRegistrar(const AdjointVariant& v) { adjoint_registry().push_back(v); }
 };
 
-#define WGFFT_REGISTER_ADJOINT(N, E, PLAN, ...)                                                        \
-    static ::wgfft::AdjointRegistrar _wgfft_adj_##N##_##E##_##PLAN(::wgfft::AdjointVariant{           \
-        ::wgfft::ilog2(N), E, #PLAN, &::wgfft::AdjointKernel<N, E, __VA_ARGS__>::launch,               \
-        &::wgfft::AdjointKernel<N, E, __VA_ARGS__>::blocks_per_cu})
+#define WGFFT_REGISTER_ADJOINT(N, E, KEEPX, PLAN, ...)                                                 \
+    static ::wgfft::AdjointRegistrar _wgfft_adj_##N##_##E##_##KEEPX##_##PLAN(::wgfft::AdjointVariant{ \
+        ::wgfft::ilog2(N), E, KEEPX, #PLAN, &::wgfft::AdjointKernel<N, E, KEEPX, __VA_ARGS__>::launch, \
+        &::wgfft::AdjointKernel<N, E, KEEPX, __VA_ARGS__>::blocks_per_cu})
 
 }  // namespace wgfft

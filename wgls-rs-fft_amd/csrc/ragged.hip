@@ -336,7 +336,7 @@ extern "C" int wgfft_ragged_exec_device(wgfft_ragged_t* p, const float* d_in, fl
     std::lock_guard<std::mutex> lk(h->mu);
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    hipStream_t s = (hipStream_t)stream;  // NULL: HIP default stream
     const bool inv = dir == WGFFT_INVERSE;
     if (mode == 0) {
         if ((e = ragged_launch(p, d_in, d_out, inv, 0, p->nunits, p->grid, s)) != hipSuccess)

@@ -287,9 +287,9 @@ static int device_count() {
     return g_device_count;
 }
 
-static hipStream_t pick_stream(wgfft_t* h, void* stream) {
-    return stream ? (hipStream_t)stream : h->stream;
-}
+// Device entry points run on the caller's stream; NULL is HIP's default
+// (null) stream, as for any HIP library call.
+static hipStream_t pick_stream(wgfft_t*, void* stream) { return (hipStream_t)stream; }
 
 static int check_dir(int dir) {
     if (dir != WGFFT_FORWARD && dir != WGFFT_INVERSE)
@@ -371,7 +371,7 @@ void wgfft_destroy(wgfft_t* h) {
         hipStreamSynchronize(h->stream);
         for (auto& t : h->tables) hipFree(t.p);
         h->tables.clear();
-        for (DevBuf* b : {&h->io_in, &h->io_out, &h->io_w, &h->io_aux, &h->scratch, &h->scratch2, &h->red, &h->lossbuf})
+        for (DevBuf* b : {&h->io_in, &h->io_out, &h->io_w, &h->io_aux, &h->scratch, &h->scratch2, &h->red, &h->red2, &h->lossbuf})
             h->release(*b);
         hipStreamDestroy(h->stream);
     }
@@ -472,7 +472,7 @@ int wgfft_bench_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, 
     int rc;
     hipError_t e;
     for (int i = 0; i < warmup; ++i) {
-        if ((rc = wgfft_exec_device(h, d_in, d_out, n, batch, dir, nullptr))) return rc;
+        if ((rc = wgfft_exec_device(h, d_in, d_out, n, batch, dir, (void*)h->stream))) return rc;
         if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     }
     hipEvent_t a, b;
@@ -483,7 +483,8 @@ int wgfft_bench_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, 
     }
     rc = WGFFT_OK;
     hipEventRecord(a, h->stream);
-    for (int i = 0; i < iters && rc == WGFFT_OK; ++i) rc = wgfft_exec_device(h, d_in, d_out, n, batch, dir, nullptr);
+    for (int i = 0; i < iters && rc == WGFFT_OK; ++i)
+        rc = wgfft_exec_device(h, d_in, d_out, n, batch, dir, (void*)h->stream);
     hipEventRecord(b, h->stream);
     if (rc == WGFFT_OK) {
         if ((e = hipEventSynchronize(b)) != hipSuccess) {

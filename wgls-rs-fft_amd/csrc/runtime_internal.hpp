@@ -44,7 +44,14 @@ struct Plan {
 };
 
 struct Handle;
+struct AdjointVariant;
 void load_tune_file(Handle* h);
+const AdjointVariant* find_adjoint(int log2n);
+const std::vector<const AdjointVariant*> adjoint_variants(int log2n);
+int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x, const float* d_w,
+                  const float* d_mask, const float* d_gy, float* d_y, float* d_gx, float* d_gw, float* d_loss,
+                  uint64_t n, uint64_t batch, const float2* tw, hipStream_t s);
+const char* adjoint_describe(const AdjointVariant* v, char* buf, uint64_t len);
 void synth_uniform(float* out, uint64_t nfloats, uint64_t seed);
 const BatchedVariant* default_variant(int log2n);
 const BatchedVariant* find_variant(int log2n, const char* plan, int E, int G);
@@ -56,9 +63,10 @@ struct Handle {
     std::mutex mu;
     std::map<uint64_t, Plan> plans;
     std::map<int, const BatchedVariant*> tuned;  // log2n -> autotuned variant
+    std::map<int, const AdjointVariant*> tuned_adj;
     std::map<std::pair<const BatchedVariant*, int>, int> occ_cache;
     std::vector<DevBuf> tables;
-    DevBuf io_in, io_out, io_w, io_aux, scratch, scratch2, red, lossbuf;
+    DevBuf io_in, io_out, io_w, io_aux, scratch, scratch2, red, red2, lossbuf;
     uint64_t held = 0;
     uint64_t mp_chunk_bytes = 64ull << 20;
 

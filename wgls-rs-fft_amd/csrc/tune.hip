@@ -14,6 +14,7 @@
 #include <string>
 
 #include "../../include/wgfft.h"
+#include "kernels_adjoint.hpp"
 #include "runtime_internal.hpp"
 
 namespace wgfft {
@@ -146,10 +147,59 @@ extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc
     }
     hipEventDestroy(a);
     hipEventDestroy(b);
+    // fused forward + adjoint variants for this length
+    std::string adj_desc;
+    const auto adjs = adjoint_variants(p->log2n);
+    if (adjs.size() > 1) {
+        if ((rc = h->ensure(h->io_aux, bytes)) || (rc = h->ensure(h->io_w, 2 * n * sizeof(float) + 64))) return rc;
+        if ((e = hipMemsetAsync(h->io_w.p, 0, 2 * n * sizeof(float) + 64, h->stream)) != hipSuccess)
+            return hip_fail(e, "hipMemsetAsync");
+        float* w = (float*)h->io_w.p;
+        float* m = w + n;
+        float* loss = m + n;
+        float* gw = (float*)h->io_out.p;  // scratch: gx goes to io_aux
+        const AdjointVariant* best_adj = nullptr;
+        float best_adj_ms = 1e30f;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        for (const AdjointVariant* v : adjs) {
+            for (int i = 0; i < 2; ++i)
+                fused_adjoint(h, v, true, (const float*)h->io_in.p, w, m, nullptr, nullptr, (float*)h->io_aux.p, gw,
+                              loss, n, batch, p->tw, h->stream);
+            const int iters = 5;
+            hipEventRecord(a, h->stream);
+            for (int i = 0; i < iters; ++i)
+                fused_adjoint(h, v, true, (const float*)h->io_in.p, w, m, nullptr, nullptr, (float*)h->io_aux.p, gw,
+                              loss, n, batch, p->tw, h->stream);
+            hipEventRecord(b, h->stream);
+            if ((e = hipEventSynchronize(b)) != hipSuccess) {
+                hipEventDestroy(a);
+                hipEventDestroy(b);
+                return hip_fail(e, "autotune timing");
+            }
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, a, b);
+            if (ms < best_adj_ms) {
+                best_adj_ms = ms;
+                best_adj = v;
+            }
+        }
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+        if (best_adj) {
+            h->tuned_adj[p->log2n] = best_adj;
+            char buf[128];
+            adj_desc = std::string("; ") + adjoint_describe(best_adj, buf, sizeof buf);
+        }
+    }
     if (best) {
         h->tuned[p->log2n] = best;
         p->var = best;
         describe(best, desc, desc_len);
+        if (desc && desc_len && !adj_desc.empty()) {
+            std::string full = std::string(desc) + adj_desc;
+            std::snprintf(desc, desc_len, "%s", full.c_str());
+        }
         if (const char* path = std::getenv("WGFFT_TUNE_FILE")) {
             if (FILE* f = std::fopen(path, "a")) {
                 std::fprintf(f, "%d %s %d %d\n", best->log2n, best->plan, best->E, best->G);

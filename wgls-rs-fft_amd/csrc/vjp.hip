@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 
 #include "../../include/wgfft.h"
 #include "kernels_adjoint.hpp"
@@ -23,10 +24,23 @@ std::vector<AdjointVariant>& adjoint_registry() {
     return v;
 }
 
-static const AdjointVariant* find_adjoint(int log2n) {
+const AdjointVariant* find_adjoint(int log2n) {
     for (const auto& v : adjoint_registry())
         if (v.log2n == log2n) return &v;
     return nullptr;
+}
+
+const std::vector<const AdjointVariant*> adjoint_variants(int log2n) {
+    std::vector<const AdjointVariant*> out;
+    for (const auto& v : adjoint_registry())
+        if (v.log2n == log2n) out.push_back(&v);
+    return out;
+}
+
+const char* adjoint_describe(const AdjointVariant* v, char* buf, uint64_t len) {
+    if (buf && len)
+        std::snprintf(buf, len, "fused N=%d plan=%s E=%d keep_x=%d", 1 << v->log2n, v->plan, v->E, v->keepx);
+    return buf;
 }
 
 constexpr int kRowChunks = 64;  // batch chunks of the gw partial sums
@@ -50,6 +64,25 @@ k_vjp_finish(const float2* __restrict__ x, const float* __restrict__ w, float2* 
         gx[e] = make_float2(wi * u.x, wi * u.y);
     }
     part[c * n + i] = acc;
+}
+
+// Column sums of a rows x n matrix in two fixed-order stages: blocks of 64
+// columns x 4 row lanes reduce a chunk of rows each (coalesced 256-byte row
+// segments), then k_sum_rows folds the chunk sums.
+__global__ void __launch_bounds__(256)
+k_sum_rows_chunks(const float* __restrict__ part, float* __restrict__ out, uint64_t n, uint64_t rows,
+                  uint64_t rows_per_chunk) {
+    __shared__ float red[4][64];
+    const uint32_t lane = threadIdx.x & 63, rl = threadIdx.x >> 6;
+    const uint64_t col = (uint64_t)blockIdx.x * 64 + lane;
+    const uint64_t r0 = (uint64_t)blockIdx.y * rows_per_chunk;
+    const uint64_t r1 = min(rows, r0 + rows_per_chunk);
+    float acc = 0.f;
+    if (col < n)
+        for (uint64_t r = r0 + rl; r < r1; r += 4) acc += part[r * n + col];
+    red[rl][lane] = acc;
+    __syncthreads();
+    if (rl == 0 && col < n) out[(uint64_t)blockIdx.y * n + col] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
 
 __global__ void __launch_bounds__(256)
@@ -95,6 +128,21 @@ __global__ void __launch_bounds__(256) k_sum_all(const float* __restrict__ part,
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
+// out[i] = sum_r part[r][i] (deterministic order).
+static int reduce_rows(Handle* h, const float* part, uint64_t rows, uint64_t n, float* out, hipStream_t s) {
+    const uint64_t chunks = std::min<uint64_t>(rows, 16);
+    const uint64_t per = (rows + chunks - 1) / chunks;
+    int rc;
+    if ((rc = h->ensure(h->red2, chunks * n * sizeof(float)))) return rc;
+    dim3 grid((unsigned)((n + 63) / 64), (unsigned)chunks);
+    hipLaunchKernelGGL(k_sum_rows_chunks, grid, dim3(256), 0, s, part, (float*)h->red2.p, n, rows, per);
+    hipLaunchKernelGGL(k_sum_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const float*)h->red2.p, out, n,
+                       chunks);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "row reduction launch");
+    return WGFFT_OK;
+}
+
 // gx, gw from gu (already in d_gx).
 static int vjp_finish(Handle* h, const float* d_x, const float* d_w, float* d_gx, float* d_gw, uint64_t n,
                       uint64_t batch, hipStream_t s) {
@@ -105,17 +153,15 @@ static int vjp_finish(Handle* h, const float* d_x, const float* d_w, float* d_gx
     dim3 grid((unsigned)((n + 255) / 256), (unsigned)chunks);
     hipLaunchKernelGGL(k_vjp_finish, grid, dim3(256), 0, s, (const float2*)d_x, d_w, (float2*)d_gx,
                        (float*)h->red.p, n, batch, rows);
-    hipLaunchKernelGGL(k_sum_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const float*)h->red.p, d_gw,
-                       n, chunks);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "vjp finish launch");
-    return WGFFT_OK;
+    return reduce_rows(h, (const float*)h->red.p, chunks, n, d_gw, s);
 }
 
 // One fused launch (plus the fixed-order reductions).  fwd = true: loss
 // L = sum m |FFT(w x)|^2 with gY = 2 m Y formed in registers; fwd = false:
 // the given gY.
-static int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x, const float* d_w,
+int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x, const float* d_w,
                          const float* d_mask, const float* d_gy, float* d_y, float* d_gx, float* d_gw,
                          float* d_loss, uint64_t n, uint64_t batch, const float2* tw, hipStream_t s) {
     int grid = v->blocks_per_cu(fwd) * h->num_cus;
@@ -128,12 +174,10 @@ static int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const flo
                   (float*)h->red.p, (float*)h->lossbuf.p, batch, grid, s};
     hipError_t e = v->launch(fwd, a);
     if (e != hipSuccess) return hip_fail(e, "adjoint kernel launch");
-    hipLaunchKernelGGL(k_sum_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const float*)h->red.p, d_gw,
-                       n, (uint64_t)grid);
     if (fwd) hipLaunchKernelGGL(k_sum_all, dim3(1), dim3(256), 0, s, (const float*)h->lossbuf.p, d_loss, grid);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "adjoint reduction launch");
-    return WGFFT_OK;
+    return reduce_rows(h, (const float*)h->red.p, (uint64_t)grid, n, d_gw, s);
 }
 
 }  // namespace wgfft
@@ -157,7 +201,7 @@ extern "C" int wgfft_vjp_windowed_device(wgfft_t* h, const float* d_x, const flo
     std::lock_guard<std::mutex> lk(h->mu);
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    hipStream_t s = (hipStream_t)stream;  // NULL: HIP default stream
     if (batch == 0) {
         hipMemsetAsync(d_gw, 0, n * sizeof(float), s);
         return WGFFT_OK;
@@ -165,7 +209,8 @@ extern "C" int wgfft_vjp_windowed_device(wgfft_t* h, const float* d_x, const flo
     Plan* p;
     if ((rc = h->get_plan(n, &p))) return rc;
     if (p->kind == Plan::kBatched) {
-        if (const AdjointVariant* v = find_adjoint(p->log2n))
+        const AdjointVariant* v = h->tuned_adj.count(p->log2n) ? h->tuned_adj[p->log2n] : find_adjoint(p->log2n);
+        if (v)
             return fused_adjoint(h, v, false, d_x, d_w, nullptr, d_gy, nullptr, d_gx, d_gw, nullptr, n, batch, p->tw,
                                  s);
     }
@@ -186,7 +231,7 @@ extern "C" int wgfft_fwd_adjoint_masked_device(wgfft_t* h, const float* d_x, con
     std::lock_guard<std::mutex> lk(h->mu);
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    hipStream_t s = (hipStream_t)stream;  // NULL: HIP default stream
     if (batch == 0) {
         hipMemsetAsync(d_gw, 0, n * sizeof(float), s);
         hipMemsetAsync(d_loss, 0, sizeof(float), s);
@@ -195,7 +240,8 @@ extern "C" int wgfft_fwd_adjoint_masked_device(wgfft_t* h, const float* d_x, con
     Plan* p;
     if ((rc = h->get_plan(n, &p))) return rc;
     if (p->kind == Plan::kBatched) {
-        if (const AdjointVariant* v = find_adjoint(p->log2n))
+        const AdjointVariant* v = h->tuned_adj.count(p->log2n) ? h->tuned_adj[p->log2n] : find_adjoint(p->log2n);
+        if (v)
             return fused_adjoint(h, v, true, d_x, d_w, d_mask, nullptr, d_y, d_gx, d_gw, d_loss, n, batch, p->tw, s);
     }
     // lengths without a fused kernel: forward, mask, adjoint as separate passes

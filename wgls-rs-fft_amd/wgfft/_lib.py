@@ -51,6 +51,15 @@ def load() -> ctypes.CDLL:
     global _lib
     if _lib is not None:
         return _lib
+    # One HIP runtime per process: torch ships its own libamdhip64 (soname
+    # libamdhip64.so.7, but linked by the unversioned name), so when torch is
+    # present it is imported first and libwgfft.so binds to that runtime;
+    # loading libwgfft.so first would make torch load a second runtime that
+    # sees no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python wgls-rs-fft_amd/build.py` "

@@ -351,15 +351,17 @@ extern "C" int wgfft_ragged_exec_device(wgfft_ragged_t* p, const float* d_in, fl
         }
         return WGFFT_OK;
     }
-    wgfft_ragged_plan::GraphKey key{d_in, d_out, dir, (void*)s};
+    // capture once on the handle's private stream (the legacy default stream
+    // cannot be captured), replay on the caller's stream
+    wgfft_ragged_plan::GraphKey key{d_in, d_out, dir, nullptr};
     auto it = p->graphs.find(key);
     if (it == p->graphs.end()) {
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
-        if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        if ((e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
             return hip_fail(e, "hipStreamBeginCapture");
-        hipError_t le = ragged_launch(p, d_in, d_out, inv, 0, p->nunits, p->grid, s);
-        e = hipStreamEndCapture(s, &g);
+        hipError_t le = ragged_launch(p, d_in, d_out, inv, 0, p->nunits, p->grid, h->stream);
+        e = hipStreamEndCapture(h->stream, &g);
         if (le != hipSuccess) {
             if (g) hipGraphDestroy(g);
             return hip_fail(le, "ragged kernel capture");

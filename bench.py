@@ -99,12 +99,14 @@ def _ragged_log2(count, kmin, kmax, seed):
     return (kmin + v % (kmax - kmin + 1)).astype(np.uint32)
 
 
-def bench_large(torch, gpu, steps: int) -> dict:
+def bench_large(torch, gpu, steps: int, tune: bool) -> dict:
     out = {}
     for log2n, batch in ((22, 16), (26, 1)):
         n = 1 << log2n
         x = (torch.rand(2 * n * batch, device="cuda") * 2 - 1).view(torch.complex64).view(batch, n)
         y = torch.empty_like(x)
+        if tune:
+            gpu.autotune(n, batch)
         ms = timed(torch, lambda: gpu.exec_device(x, y), max(5, steps // 20), 2)
         ms_inv = timed(torch, lambda: gpu.exec_device(y, x, inverse=True), max(5, steps // 20), 2)
         s = n * batch
@@ -189,7 +191,7 @@ def main() -> int:
         x_keep, y_keep = x, y
         del x, y
         out["mixed_batch"] = bench_mixed(torch, gpu, wgfft, args.steps)
-        out["large_n"] = bench_large(torch, gpu, args.steps)
+        out["large_n"] = bench_large(torch, gpu, args.steps, not args.no_tune)
         del x_keep, y_keep
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

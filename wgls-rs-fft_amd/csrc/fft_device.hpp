@@ -25,6 +25,16 @@ namespace wgfft {
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
 
+// Returns p unchanged but opaque to the optimiser.  Called once per
+// iteration of a persistent loop, it keeps the per-thread twiddle loads (which
+// do not depend on the transform) inside the loop instead of being hoisted
+// and held in registers for every pass of every length the loop handles.
+template <class P>
+__device__ __forceinline__ P* launder(P* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "fft_device.hpp"
@@ -52,12 +53,13 @@ struct GlobalPassArgs {
     uint64_t L;           // stride of this pass (product of earlier radices)
     uint64_t batch;
     float scale;
+    int num_cus;
     hipStream_t stream;
 };
 
 struct GlobalPassVariant {
     int log2r;
-    int E, G, threads;
+    int E, G, prefetch, threads;
     int lds_bytes;
     const char* plan;
     hipError_t (*launch)(Mode, const GlobalPassArgs&);
@@ -222,14 +224,34 @@ struct TwOneLevel {
 // radix R done as an LDS sub-FFT: butterflies b0 .. b0+G-1 (G <= L, so both
 // the strided loads and the strided stores are runs of G contiguous
 // elements); when L == 1 each butterfly's R outputs are contiguous and are
-// stored straight from registers.  src/dst point at the transform.  The
-// caller provides G * SLOT complex elements of LDS and the LDS barrier
-// before the next group.
+// stored straight from registers.  src/dst point at the transform.
+//
+// Split in two so callers can software-pipeline: gp_load issues the 16-byte
+// loads of the R x G tile into E registers per thread (lane i of step q holds
+// rows m = (tid + THREADS q) / (G/2), butterfly pair (tid + THREADS q) mod
+// (G/2)); gp_compute applies the pass twiddles (or the window), stages the
+// tile through LDS into the sub-FFT's pair layout, runs it and stores.  The
+// caller provides G * SLOT complex elements of LDS and the LDS barrier before
+// the next gp_compute.
+template <int R, int E, int G>
+__device__ __forceinline__ void gp_load(float2 (&v)[E], const float2* __restrict__ src, uint64_t NR, uint64_t b0) {
+    constexpr int THREADS = G * (R / E);
+    const uint32_t tid = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < E / 2; ++q) {
+        const int it = tid + THREADS * q;
+        const int gp = it % (G / 2);
+        const int m = it / (G / 2);
+        float4 x = *reinterpret_cast<const float4*>(src + b0 + 2 * gp + (uint64_t)m * NR);
+        v[2 * q] = make_float2(x.x, x.y);
+        v[2 * q + 1] = make_float2(x.z, x.w);
+    }
+}
+
 template <int R, int E, int G, int MODE, class TW, int... Rs>
-__device__ __forceinline__ void global_pass_group(const float2* __restrict__ src, float2* __restrict__ dst,
-                                                  const float2* __restrict__ tw_r, const TW& twN,
-                                                  const float* __restrict__ win, uint64_t N, uint64_t L,
-                                                  uint64_t b0, float sc, float2* __restrict__ lds_all) {
+__device__ __forceinline__ void gp_compute(float2 (&v)[E], float2* __restrict__ dst, const float2* __restrict__ tw_r,
+                                           const TW& twN, const float* __restrict__ win, uint64_t N, uint64_t L,
+                                           uint64_t b0, float sc, float2* __restrict__ lds_all) {
     constexpr int T = R / E;
     constexpr int THREADS = G * T;
     constexpr bool INV = MODE == kInverse;
@@ -239,16 +261,15 @@ __device__ __forceinline__ void global_pass_group(const float2* __restrict__ src
     const uint64_t s = N / ((uint64_t)R * L);  // W_{RL} = W_N^s
     const bool first = (L == 1);
 
-    // staged load of the R x G tile (rows m, adjacent butterfly pairs per lane)
-    constexpr int PAIRS = R * (G / 2);
-    for (int it = tid; it < PAIRS; it += THREADS) {
+#pragma unroll
+    for (int q = 0; q < E / 2; ++q) {
+        const int it = tid + THREADS * q;
         const int gp = it % (G / 2);
         const int m = it / (G / 2);
-        const uint64_t e = b0 + 2 * gp + (uint64_t)m * NR;
-        float4 v = *reinterpret_cast<const float4*>(src + e);
-        float2 a = make_float2(v.x, v.y), b = make_float2(v.z, v.w);
+        float2 a = v[2 * q], b = v[2 * q + 1];
         if constexpr (MODE == kWindowed) {
             if (first) {
+                const uint64_t e = b0 + 2 * gp + (uint64_t)m * NR;
                 a = cscale(a, win[e]);
                 b = cscale(b, win[e + 1]);
             }
@@ -261,6 +282,8 @@ __device__ __forceinline__ void global_pass_group(const float2* __restrict__ src
         const uint32_t pm = pad_idx<5, 0>(m);
         lds_all[(2 * gp) * SLOT + pm] = a;
         lds_all[(2 * gp + 1) * SLOT + pm] = b;
+        // bound the twiddle-table loads in flight (16 registers per 4 steps)
+        if ((q & 3) == 3) __builtin_amdgcn_sched_barrier(0);
     }
     WGFFT_LDS_SYNC();
 
@@ -288,7 +311,9 @@ __device__ __forceinline__ void global_pass_group(const float2* __restrict__ src
     }
     WGFFT_LDS_SYNC();
     const uint64_t bj = b0 / L, k0base = b0 & (L - 1);
-    for (int it = tid; it < PAIRS; it += THREADS) {
+#pragma unroll
+    for (int q = 0; q < E / 2; ++q) {
+        const int it = tid + THREADS * q;
         const int gp = it % (G / 2);
         const int m = it / (G / 2);
         const uint32_t pm = pad_idx<5, 0>(m);
@@ -299,7 +324,13 @@ __device__ __forceinline__ void global_pass_group(const float2* __restrict__ src
     }
 }
 
-template <int R, int E, int G, int MODE, int... Rs>
+// Persistent multi-pass kernel: workgroups loop over the groups of all
+// transforms in the batch with the next group's tile prefetched into
+// registers while the current one is transformed.  Consecutive workgroup ids
+// are dealt round-robin over the 8 XCDs; the logical numbering makes the
+// workgroups of one XCD take adjacent column groups at the same time, so the
+// narrow column runs of neighbouring groups share that XCD's L2 lines.
+template <int R, int E, int G, bool PF, int MODE, int... Rs>
 __global__ void __launch_bounds__(G*(R / E))
 k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw_r,
               const float2* __restrict__ tw_hi, const float2* __restrict__ tw_lo,
@@ -309,38 +340,65 @@ k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const flo
     __shared__ __align__(16) float2 lds_all[G * SLOT];
 
     const uint64_t N = 1ull << log2n;
-    const uint64_t groups_per_tr = (N / R) / G;
+    const uint64_t NR = N / R;
+    const uint64_t groups_per_tr = NR / G;
     const uint64_t total = groups_per_tr * batch;
-
-    // XCD-aware remap: block ids are dealt round-robin over the 8 XCDs; map
-    // consecutive logical groups onto one XCD so neighbouring column runs
-    // share that XCD's L2 lines.
+    const uint64_t nb = gridDim.x;  // a multiple of 8
     const uint64_t pid = blockIdx.x;
-    const uint64_t nb = gridDim.x;
-    uint64_t lid = pid;
-    if ((nb & 7) == 0) lid = (pid & 7) * (nb >> 3) + (pid >> 3);
-    if (lid >= total) return;
-    const uint64_t tr = lid / groups_per_tr;
-    const uint64_t b0 = (lid % groups_per_tr) * G;
+    const uint64_t lid0 = (pid & 7) * (nb >> 3) + (pid >> 3);
     const bool last = (L * R == N);
-    global_pass_group<R, E, G, MODE, TwTwoLevel, Rs...>(in + tr * N, out + tr * N, tw_r, TwTwoLevel{tw_hi, tw_lo, h},
-                                                        win, N, L, b0, last ? scale : 1.0f, lds_all);
+    const TwTwoLevel twN{tw_hi, tw_lo, h};
+    float2 cur[E];
+    if (PF && lid0 < total) {
+        const uint64_t tr = lid0 / groups_per_tr, b0 = (lid0 % groups_per_tr) * G;
+        gp_load<R, E, G>(cur, in + tr * N, NR, b0);
+    }
+    for (uint64_t lid = lid0; lid < total; lid += nb) {
+        const uint64_t tr = lid / groups_per_tr, b0 = (lid % groups_per_tr) * G;
+        if constexpr (PF) {
+            // next group's tile in flight while this one is transformed
+            float2 nxt[E];
+            const uint64_t nl = lid + nb;
+            if (nl < total) {
+                const uint64_t ntr = nl / groups_per_tr, nb0 = (nl % groups_per_tr) * G;
+                gp_load<R, E, G>(nxt, in + ntr * N, NR, nb0);
+            }
+            gp_compute<R, E, G, MODE, TwTwoLevel, Rs...>(cur, out + tr * N, tw_r, twN, win, N, L, b0,
+                                                         last ? scale : 1.0f, lds_all);
+#pragma unroll
+            for (int i = 0; i < E; ++i) cur[i] = nxt[i];
+        } else {
+            gp_load<R, E, G>(cur, in + tr * N, NR, b0);
+            gp_compute<R, E, G, MODE, TwTwoLevel, Rs...>(cur, out + tr * N, tw_r, twN, win, N, L, b0,
+                                                         last ? scale : 1.0f, lds_all);
+        }
+        WGFFT_LDS_SYNC();
+    }
 }
 
-template <int R, int E, int G, int... Rs>
+template <int R, int E, int G, bool PF, int... Rs>
 struct GlobalPassKernel {
     static constexpr int T = R / E;
     static constexpr int THREADS = G * T;
     static constexpr int LDS_BYTES = G * GlobalPassSlot<R, E, Rs...>::len * 8;
 
     template <int MODE>
+    static int occ() {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_global_pass<R, E, G, PF, MODE, Rs...>,
+                                                         THREADS, 0) != hipSuccess)
+            return 1;
+        return blocks > 0 ? blocks : 1;
+    }
+    template <int MODE>
     static hipError_t go(const GlobalPassArgs& a) {
         const uint64_t N = 1ull << a.log2n;
         const uint64_t groups = (N / R) / G * a.batch;
         if (groups == 0) return hipSuccess;
-        uint64_t grid = groups;
-        if (grid % 8) grid += 8 - grid % 8;  // keep the XCD remap a bijection; extra blocks exit
-        hipLaunchKernelGGL((k_global_pass<R, E, G, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0,
+        static const int per_cu = occ<MODE>();
+        uint64_t grid = std::min<uint64_t>(groups, (uint64_t)per_cu * (uint64_t)a.num_cus);
+        if (grid % 8) grid += 8 - grid % 8;  // keep the XCD numbering a bijection; extra blocks exit
+        hipLaunchKernelGGL((k_global_pass<R, E, G, PF, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0,
                            a.stream, a.in, a.out, a.tw_r, a.tw_hi, a.tw_lo, a.win, a.log2n, a.h, a.L,
                            a.batch, a.scale);
         return hipGetLastError();
@@ -354,10 +412,10 @@ struct GlobalPassKernel {
     }
 };
 
-#define WGFFT_REGISTER_GLOBAL_PASS(R, E, G, PLAN, ...)                                                  \
-    static ::wgfft::Registrar _wgfft_gp_##R##_##E##_##G##_##PLAN(::wgfft::GlobalPassVariant{           \
-        ::wgfft::ilog2(R), E, G, ::wgfft::GlobalPassKernel<R, E, G, __VA_ARGS__>::THREADS,               \
-        ::wgfft::GlobalPassKernel<R, E, G, __VA_ARGS__>::LDS_BYTES, #PLAN,                              \
-        &::wgfft::GlobalPassKernel<R, E, G, __VA_ARGS__>::launch})
+#define WGFFT_REGISTER_GLOBAL_PASS(R, E, G, PF, PLAN, ...)                                              \
+    static ::wgfft::Registrar _wgfft_gp_##R##_##E##_##G##_##PF##_##PLAN(::wgfft::GlobalPassVariant{   \
+        ::wgfft::ilog2(R), E, G, PF, ::wgfft::GlobalPassKernel<R, E, G, PF, __VA_ARGS__>::THREADS,       \
+        ::wgfft::GlobalPassKernel<R, E, G, PF, __VA_ARGS__>::LDS_BYTES, #PLAN,                          \
+        &::wgfft::GlobalPassKernel<R, E, G, PF, __VA_ARGS__>::launch})
 
 }  // namespace wgfft

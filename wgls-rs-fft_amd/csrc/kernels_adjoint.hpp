@@ -7,8 +7,9 @@
 //                 in registers across the transforms this workgroup handles.
 //  k_vjp          given gY: adjoint FFT of gY, gx = w * gu, the same gw
 //                 accumulation.
-// Both write per-workgroup partial sums (gw: N floats, loss: 1 float) that
-// k_reduce_rows folds in a fixed order, so results are deterministic.
+// Both write per-workgroup partial sums (gw: N floats, loss: 1 float) into
+// one row of a partial matrix that a single column-reduction launch folds in
+// a fixed order, so results are deterministic.
 #pragma once
 
 #include "kernels.hpp"
@@ -23,8 +24,8 @@ struct AdjointArgs {
     float2* y;          // fwd_adjoint, optional
     float2* gx;
     const float2* tw;
-    float* part_gw;     // [grid][N]
-    float* part_loss;   // [grid]
+    float* part;        // [grid][ld]: gw partials in columns < N, loss partial in column N
+    uint64_t ld;
     uint64_t batch;
     int grid;
     hipStream_t stream;
@@ -113,7 +114,7 @@ k_adjoint(AdjointArgs a) {
         store_pairs<N, E, T, false>(a.gx + tr * N, d, t, 1.f);
         WGFFT_LDS_SYNC();
     }
-    float2* pg = reinterpret_cast<float2*>(a.part_gw + (uint64_t)blockIdx.x * N);
+    float2* pg = reinterpret_cast<float2*>(a.part + (uint64_t)blockIdx.x * a.ld);
 #pragma unroll
     for (int q = 0; q < E / 2; ++q) pg[t + T * q] = make_float2(acc[2 * q], acc[2 * q + 1]);
     if constexpr (FWD) {
@@ -122,7 +123,7 @@ k_adjoint(AdjointArgs a) {
         if (t == 0) {
             float s = 0.f;
             for (int i = 0; i < T; ++i) s += red[i];
-            a.part_loss[blockIdx.x] = s;
+            a.part[(uint64_t)blockIdx.x * a.ld + N] = s;
         }
     }
 }

@@ -52,8 +52,10 @@ constexpr int kRaggedLds = cmax(
               cmax(direct_lds<4096, 32, 16, 16, 16>(), direct_lds<8192, 32, 16, 16, 16, 2>())),
          cmax(pass_lds<128, 32, 16, 8>(), pass_lds<256, 32, 16, 16>())));
 
+// Unit handlers are separate (non-inlined) functions so the kernel's
+// register budget is the largest handler's, not the union of all of them.
 template <int N, int E, bool INV, int... Rs>
-__device__ __forceinline__ void ragged_direct(const float2* __restrict__ in, float2* __restrict__ out,
+__device__ __attribute__((noinline)) void ragged_direct(const float2* __restrict__ in, float2* __restrict__ out,
                                               const uint64_t* __restrict__ offs, uint32_t first, uint32_t count,
                                               const float2* __restrict__ tw_all, float2* __restrict__ lds_raw) {
     constexpr int T = N / E;
@@ -70,38 +72,66 @@ __device__ __forceinline__ void ragged_direct(const float2* __restrict__ in, flo
 
 // Two-pass transform of one length-N signal (N = R1 * R2) by one workgroup:
 // pass 1 (L = 1) src -> scratch, pass 2 (L = R1) scratch -> dst.
-template <int LOGN, int R1, int E1, class P1, int R2, int E2, class P2, bool INV>
+// PF: prefetch the next group's tile into registers while the current one is
+// transformed (more registers, more overlap).
+template <int LOGN, int R1, int E1, class P1, int R2, int E2, class P2, bool INV, bool PF>
 struct RaggedBig;
 
-template <int LOGN, int R1, int E1, int... Rs1, int R2, int E2, int... Rs2, bool INV>
-struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV> {
+template <int LOGN, int R1, int E1, int... Rs1, int R2, int E2, int... Rs2, bool INV, bool PF>
+struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV, PF> {
     static constexpr uint64_t N = 1ull << LOGN;
     static constexpr int G1 = kRaggedThreads / (R1 / E1);
     static constexpr int G2 = kRaggedThreads / (R2 / E2);
     static_assert(R1 * R2 == (1 << LOGN), "radices must multiply to N");
     static_assert(G2 <= R1, "second pass needs G <= L");
-    __device__ __forceinline__ static void run(const float2* __restrict__ src, float2* __restrict__ dst,
-                                               float2* __restrict__ scratch, const float2* __restrict__ tw_all,
-                                               float2* __restrict__ lds) {
+    __device__ __attribute__((noinline)) static void run(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                         float2* __restrict__ scratch,
+                                                         const float2* __restrict__ tw_all, float2* __restrict__ lds) {
         constexpr int M = INV ? kInverse : kForward;
         const TwOneLevel twN{tw_all + N};
+        static_assert(E1 == E2, "both passes share the prefetch registers");
+        // pass 1 (L = 1): src -> scratch
+        float2 cur[E1];
+        if (PF) gp_load<R1, E1, G1>(cur, src, N / R1, 0);
         for (uint64_t b0 = 0; b0 < N / R1; b0 += G1) {
-            global_pass_group<R1, E1, G1, M, TwOneLevel, Rs1...>(src, scratch, tw_all + R1, twN, nullptr, N, 1, b0,
-                                                                 1.0f, lds);
+            if constexpr (PF) {
+                float2 nxt[E1];
+                if (b0 + G1 < N / R1) gp_load<R1, E1, G1>(nxt, src, N / R1, b0 + G1);
+                gp_compute<R1, E1, G1, M, TwOneLevel, Rs1...>(cur, scratch, tw_all + R1, twN, nullptr, N, 1, b0,
+                                                              1.0f, lds);
+#pragma unroll
+                for (int i = 0; i < E1; ++i) cur[i] = nxt[i];
+            } else {
+                gp_load<R1, E1, G1>(cur, src, N / R1, b0);
+                gp_compute<R1, E1, G1, M, TwOneLevel, Rs1...>(cur, scratch, tw_all + R1, twN, nullptr, N, 1, b0,
+                                                              1.0f, lds);
+            }
             WGFFT_LDS_SYNC();
         }
         __syncthreads();  // scratch stores of the whole workgroup visible to its loads
+        // pass 2 (L = R1): scratch -> dst
         const float sc = INV ? 1.0f / (float)N : 1.0f;
+        if (PF) gp_load<R2, E2, G2>(cur, scratch, N / R2, 0);
         for (uint64_t b0 = 0; b0 < N / R2; b0 += G2) {
-            global_pass_group<R2, E2, G2, M, TwOneLevel, Rs2...>(scratch, dst, tw_all + R2, twN, nullptr, N, R1, b0,
-                                                                 sc, lds);
+            if constexpr (PF) {
+                float2 nxt[E2];
+                if (b0 + G2 < N / R2) gp_load<R2, E2, G2>(nxt, scratch, N / R2, b0 + G2);
+                gp_compute<R2, E2, G2, M, TwOneLevel, Rs2...>(cur, dst, tw_all + R2, twN, nullptr, N, R1, b0, sc,
+                                                              lds);
+#pragma unroll
+                for (int i = 0; i < E2; ++i) cur[i] = nxt[i];
+            } else {
+                gp_load<R2, E2, G2>(cur, scratch, N / R2, b0);
+                gp_compute<R2, E2, G2, M, TwOneLevel, Rs2...>(cur, dst, tw_all + R2, twN, nullptr, N, R1, b0, sc,
+                                                              lds);
+            }
             WGFFT_LDS_SYNC();
         }
         __syncthreads();  // the next unit reuses this scratch slot
     }
 };
 
-template <bool INV>
+template <bool INV, bool PF>
 __global__ void __launch_bounds__(kRaggedThreads)
 k_ragged(const float2* __restrict__ in, float2* __restrict__ out, const RaggedUnit* __restrict__ units,
          uint32_t u_begin, uint32_t u_end, const uint64_t* __restrict__ offs, const float2* __restrict__ tw_all,
@@ -131,19 +161,19 @@ k_ragged(const float2* __restrict__ in, float2* __restrict__ out, const RaggedUn
                 break;
             case 14: {
                 const uint64_t o = offs[un.first];
-                RaggedBig<14, 128, 32, Radices<16, 8>, 128, 32, Radices<16, 8>, INV>::run(in + o, out + o, my_scratch,
+                RaggedBig<14, 128, 32, Radices<16, 8>, 128, 32, Radices<16, 8>, INV, PF>::run(in + o, out + o, my_scratch,
                                                                                          tw_all, lds);
                 break;
             }
             case 15: {
                 const uint64_t o = offs[un.first];
-                RaggedBig<15, 256, 32, Radices<16, 16>, 128, 32, Radices<16, 8>, INV>::run(in + o, out + o,
+                RaggedBig<15, 256, 32, Radices<16, 16>, 128, 32, Radices<16, 8>, INV, PF>::run(in + o, out + o,
                                                                                           my_scratch, tw_all, lds);
                 break;
             }
             default: {
                 const uint64_t o = offs[un.first];
-                RaggedBig<16, 256, 32, Radices<16, 16>, 256, 32, Radices<16, 16>, INV>::run(in + o, out + o,
+                RaggedBig<16, 256, 32, Radices<16, 16>, 256, 32, Radices<16, 16>, INV, PF>::run(in + o, out + o,
                                                                                            my_scratch, tw_all, lds);
                 break;
             }
@@ -160,9 +190,9 @@ k_ragged(const float2* __restrict__ in, float2* __restrict__ out, const RaggedUn
     }
 }
 
-static int ragged_blocks_per_cu(bool inv) {
+static int ragged_blocks_per_cu(bool pf) {
     int b = 0;
-    const void* f = inv ? (const void*)k_ragged<true> : (const void*)k_ragged<false>;
+    const void* f = pf ? (const void*)k_ragged<false, true> : (const void*)k_ragged<false, false>;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, f, kRaggedThreads, 0) != hipSuccess || b < 1) b = 1;
     return b;
 }
@@ -182,7 +212,9 @@ struct wgfft_ragged_plan {
     void* d_tw = nullptr;
     void* d_counters = nullptr;
     void* d_scratch = nullptr;
-    int grid = 0;
+    int grid = 0;      // workgroups of the chosen kernel variant
+    int grid_max = 0;  // scratch slots allocated (largest grid of the variants)
+    int pf = -1;       // big-unit tile prefetch: -1 undecided (timed on first use)
     struct GraphKey {
         const void* in;
         void* out;
@@ -261,7 +293,8 @@ extern "C" int wgfft_ragged_create(wgfft_t* h, const uint64_t* offsets, const ui
     p->total_samples = samples;
     p->size_ranges = ranges;
     p->nunits = (uint32_t)units.size();
-    p->grid = std::max(ragged_blocks_per_cu(false), 1) * h->num_cus;
+    p->grid_max = std::max(ragged_blocks_per_cu(false), ragged_blocks_per_cu(true)) * h->num_cus;
+    p->grid = p->grid_max;
     // twiddles: tw_all[N + j] = W_N^j for N = 2 .. 2^16
     const uint64_t tw_n = 2ull << kMaxLog2;
     std::vector<float2> tw(tw_n);
@@ -273,7 +306,7 @@ extern "C" int wgfft_ragged_create(wgfft_t* h, const uint64_t* offsets, const ui
         }
     }
     tw[0] = make_float2(1.f, 0.f);
-    const uint64_t scratch_bytes = (uint64_t)p->grid * (1ull << kMaxLog2) * sizeof(float2);
+    const uint64_t scratch_bytes = (uint64_t)p->grid_max * (1ull << kMaxLog2) * sizeof(float2);
     struct {
         void** dst;
         const void* src;
@@ -309,17 +342,64 @@ extern "C" void wgfft_ragged_destroy(wgfft_ragged_t* p) {
     ragged_free(p);
 }
 
+template <bool INV, bool PF>
+static void ragged_launch_t(wgfft_ragged_t* p, const float* in, float* out, uint32_t u0, uint32_t u1, int grid,
+                            hipStream_t s) {
+    hipLaunchKernelGGL((k_ragged<INV, PF>), dim3(grid), dim3(kRaggedThreads), 0, s, (const float2*)in, (float2*)out,
+                       (const RaggedUnit*)p->d_units, u0, u1, (const uint64_t*)p->d_offs, (const float2*)p->d_tw,
+                       (float2*)p->d_scratch, (uint32_t*)p->d_counters);
+}
+
+static hipError_t ragged_launch_pf(wgfft_ragged_t* p, const float* in, float* out, bool inv, bool pf, uint32_t u0,
+                                   uint32_t u1, int grid, hipStream_t s) {
+    if (inv) {
+        if (pf) ragged_launch_t<true, true>(p, in, out, u0, u1, grid, s);
+        else ragged_launch_t<true, false>(p, in, out, u0, u1, grid, s);
+    } else {
+        if (pf) ragged_launch_t<false, true>(p, in, out, u0, u1, grid, s);
+        else ragged_launch_t<false, false>(p, in, out, u0, u1, grid, s);
+    }
+    return hipGetLastError();
+}
+
+// First out-of-place use: time both kernel variants over the whole queue
+// (each launch rewrites the same output from the untouched input) and keep
+// the faster; the grid follows the chosen variant's occupancy.  An in-place
+// first call keeps the default variant.
+static hipError_t ragged_choose(wgfft_ragged_t* p, const float* in, float* out, bool inv, hipStream_t s) {
+    if (p->pf >= 0) return hipSuccess;
+    if ((const void*)in == (const void*)out) {
+        p->pf = 0;
+        p->grid = std::max(ragged_blocks_per_cu(false), 1) * p->h->num_cus;
+        return hipSuccess;
+    }
+    float ms[2] = {0.f, 0.f};
+    hipEvent_t a, b;
+    hipError_t e = hipEventCreate(&a);
+    if (e != hipSuccess) return e;
+    if ((e = hipEventCreate(&b)) != hipSuccess) {
+        hipEventDestroy(a);
+        return e;
+    }
+    for (int pf = 0; pf < 2 && e == hipSuccess; ++pf) {
+        const int grid = std::max(ragged_blocks_per_cu(pf != 0), 1) * p->h->num_cus;
+        ragged_launch_pf(p, in, out, inv, pf != 0, 0, p->nunits, grid, s);  // warm
+        hipEventRecord(a, s);
+        ragged_launch_pf(p, in, out, inv, pf != 0, 0, p->nunits, grid, s);
+        hipEventRecord(b, s);
+        if ((e = hipEventSynchronize(b)) == hipSuccess) hipEventElapsedTime(&ms[pf], a, b);
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (e != hipSuccess) return e;
+    p->pf = ms[1] < ms[0] ? 1 : 0;
+    p->grid = std::max(ragged_blocks_per_cu(p->pf != 0), 1) * p->h->num_cus;
+    return hipSuccess;
+}
+
 static hipError_t ragged_launch(wgfft_ragged_t* p, const float* in, float* out, bool inv, uint32_t u0, uint32_t u1,
                                 int grid, hipStream_t s) {
-    if (inv)
-        hipLaunchKernelGGL(k_ragged<true>, dim3(grid), dim3(kRaggedThreads), 0, s, (const float2*)in, (float2*)out,
-                           (const RaggedUnit*)p->d_units, u0, u1, (const uint64_t*)p->d_offs,
-                           (const float2*)p->d_tw, (float2*)p->d_scratch, (uint32_t*)p->d_counters);
-    else
-        hipLaunchKernelGGL(k_ragged<false>, dim3(grid), dim3(kRaggedThreads), 0, s, (const float2*)in, (float2*)out,
-                           (const RaggedUnit*)p->d_units, u0, u1, (const uint64_t*)p->d_offs,
-                           (const float2*)p->d_tw, (float2*)p->d_scratch, (uint32_t*)p->d_counters);
-    return hipGetLastError();
+    return ragged_launch_pf(p, in, out, inv, p->pf > 0, u0, u1, grid, s);
 }
 
 extern "C" int wgfft_ragged_exec_device(wgfft_ragged_t* p, const float* d_in, float* d_out, int dir, int mode,
@@ -338,6 +418,7 @@ extern "C" int wgfft_ragged_exec_device(wgfft_ragged_t* p, const float* d_in, fl
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipStream_t s = (hipStream_t)stream;  // NULL: HIP default stream
     const bool inv = dir == WGFFT_INVERSE;
+    if ((e = ragged_choose(p, d_in, d_out, inv, s)) != hipSuccess) return hip_fail(e, "ragged variant selection");
     if (mode == 0) {
         if ((e = ragged_launch(p, d_in, d_out, inv, 0, p->nunits, p->grid, s)) != hipSuccess)
             return hip_fail(e, "ragged kernel launch");

@@ -137,15 +137,18 @@ const BatchedVariant* find_variant(int log2n, const char* plan, int E, int G) {
     return nullptr;
 }
 
-static const GlobalPassVariant* pick_global_pass(int log2r, uint64_t L, uint64_t nr) {
-    const GlobalPassVariant* best = nullptr;
+// Global-pass variants usable for a pass of radix 2^log2r at stride L over
+// nr butterflies (G <= nr, and G <= L after the first pass); the first is
+// the default.
+std::vector<const GlobalPassVariant*> global_pass_candidates(int log2r, uint64_t L, uint64_t nr) {
+    std::vector<const GlobalPassVariant*> out;
     for (const auto& v : global_pass_registry()) {
         if (v.log2r != log2r) continue;
         if ((uint64_t)v.G > nr) continue;
         if (L > 1 && (uint64_t)v.G > L) continue;
-        if (!best || v.G > best->G) best = &v;
+        out.push_back(&v);
     }
-    return best;
+    return out;
 }
 
 int Handle::get_plan(uint64_t n, Plan** out) {
@@ -174,7 +177,8 @@ int Handle::get_plan(uint64_t n, Plan** out) {
         for (int i = 0; i < passes; ++i) {
             int r = (rem + (passes - i) - 1) / (passes - i);  // larger radices first
             rem -= r;
-            const GlobalPassVariant* v = pick_global_pass(r, L, n >> r);
+            auto cands = global_pass_candidates(r, L, n >> r);
+            const GlobalPassVariant* v = cands.empty() ? nullptr : cands.front();
             if (!v) return fail(WGFFT_ERR_INVALID_LENGTH, "no global-pass variant for radix 2^" + std::to_string(r));
             p.pass_log2r.push_back(r);
             p.pass_var.push_back(v);
@@ -252,8 +256,8 @@ int Handle::run(Plan* p, Mode mode, const float2* in, float2* out, const float* 
                 if ((const void*)dst == (const void*)src) dst = (dst == a) ? b : a;
             }
             const GlobalPassVariant* v = p->pass_var[i];
-            GlobalPassArgs a{src,     dst,     p->tw_r[p->pass_log2r[i]], p->tw_hi, p->tw_lo, win,
-                             (uint32_t)p->log2n, p->h, L, nb, (i == P - 1) ? scale : 1.0f, s};
+            GlobalPassArgs a{src, dst, p->tw_r[p->pass_log2r[i]], p->tw_hi, p->tw_lo, win, (uint32_t)p->log2n, p->h, L,
+                             nb, (i == P - 1) ? scale : 1.0f, num_cus, s};
             Mode m = mode;
             if (mode == kWindowed && i > 0) m = kForward;
             e = v->launch(m, a);

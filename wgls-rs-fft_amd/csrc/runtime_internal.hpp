@@ -55,6 +55,7 @@ const char* adjoint_describe(const AdjointVariant* v, char* buf, uint64_t len);
 void synth_uniform(float* out, uint64_t nfloats, uint64_t seed);
 const BatchedVariant* default_variant(int log2n);
 const BatchedVariant* find_variant(int log2n, const char* plan, int E, int G);
+std::vector<const GlobalPassVariant*> global_pass_candidates(int log2r, uint64_t L, uint64_t nr);
 
 struct Handle {
     int device = 0;

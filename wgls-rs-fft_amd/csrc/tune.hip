@@ -8,6 +8,7 @@
 // (lines: "log2n plan E G").
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -73,6 +74,69 @@ static void describe(const BatchedVariant* v, char* desc, uint64_t len) {
                   v->threads, v->lds_bytes);
 }
 
+static std::string describe_multipass(const Plan* p) {
+    std::string s = "multipass N=" + std::to_string(p->n) + " passes=";
+    for (size_t i = 0; i < p->pass_var.size(); ++i) {
+        const GlobalPassVariant* v = p->pass_var[i];
+        s += (i ? "," : "") + std::string("R") + std::to_string(1 << v->log2r) + "(" + v->plan + ",G" +
+             std::to_string(v->G) + (v->prefetch ? ",prefetch" : "") + ")";
+    }
+    return s;
+}
+
+// Time every combination of global-pass variants for a multipass plan (a
+// few per pass) on a scratch batch and keep the fastest.
+static int tune_multipass(Handle* h, Plan* p, uint64_t batch, char* desc, uint64_t desc_len) {
+    const uint64_t bytes = p->n * batch * sizeof(float2);
+    int rc;
+    if ((rc = h->ensure(h->io_in, bytes)) || (rc = h->ensure(h->io_out, bytes))) return rc;
+    hipError_t e;
+    if ((e = hipMemsetAsync(h->io_in.p, 0, bytes, h->stream)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+    std::vector<std::vector<const GlobalPassVariant*>> cands;
+    uint64_t L = 1;
+    for (size_t i = 0; i < p->pass_log2r.size(); ++i) {
+        cands.push_back(global_pass_candidates(p->pass_log2r[i], L, p->n >> p->pass_log2r[i]));
+        L <<= p->pass_log2r[i];
+    }
+    std::vector<size_t> idx(cands.size(), 0), best_idx = idx;
+    float best_ms = 1e30f;
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    for (;;) {
+        for (size_t i = 0; i < cands.size(); ++i) p->pass_var[i] = cands[i][idx[i]];
+        h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream);
+        float ms = 1e30f;
+        for (int round = 0; round < 3; ++round) {
+            hipEventRecord(a, h->stream);
+            for (int it = 0; it < 3; ++it)
+                h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f,
+                       h->stream);
+            hipEventRecord(b, h->stream);
+            if ((e = hipEventSynchronize(b)) != hipSuccess) {
+                hipEventDestroy(a);
+                hipEventDestroy(b);
+                return hip_fail(e, "autotune timing");
+            }
+            float r = 0.f;
+            hipEventElapsedTime(&r, a, b);
+            ms = std::min(ms, r);
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_idx = idx;
+        }
+        size_t k = 0;
+        while (k < idx.size() && ++idx[k] == cands[k].size()) idx[k++] = 0;
+        if (k == idx.size()) break;
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    for (size_t i = 0; i < cands.size(); ++i) p->pass_var[i] = cands[i][best_idx[i]];
+    if (desc && desc_len) std::snprintf(desc, desc_len, "%s", describe_multipass(p).c_str());
+    return WGFFT_OK;
+}
+
 }  // namespace wgfft
 
 using namespace wgfft;
@@ -91,13 +155,7 @@ extern "C" int wgfft_plan_describe(wgfft_t* h, uint64_t n, char* desc, uint64_t 
     } else if (p->kind == Plan::kBatched) {
         describe(p->var, desc, desc_len);
     } else {
-        std::string s = "multipass N=" + std::to_string(n) + " passes=";
-        for (size_t i = 0; i < p->pass_var.size(); ++i) {
-            const GlobalPassVariant* v = p->pass_var[i];
-            s += (i ? "," : "") + std::string("R") + std::to_string(1 << v->log2r) + "(" + v->plan + ",G" +
-                 std::to_string(v->G) + ")";
-        }
-        std::snprintf(desc, desc_len, "%s", s.c_str());
+        std::snprintf(desc, desc_len, "%s", describe_multipass(p).c_str());
     }
     return WGFFT_OK;
 }
@@ -112,11 +170,12 @@ extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc
     Plan* p;
     int rc;
     if ((rc = h->get_plan(n, &p))) return rc;
+    const uint64_t bytes = n * batch * sizeof(float2);
+    if (p->kind == Plan::kMultiPass) return tune_multipass(h, p, batch, desc, desc_len);
     if (p->kind != Plan::kBatched) {
         if (desc && desc_len) std::snprintf(desc, desc_len, "no alternatives for N=%llu", (unsigned long long)n);
         return WGFFT_OK;
     }
-    const uint64_t bytes = n * batch * sizeof(float2);
     if ((rc = h->ensure(h->io_in, bytes)) || (rc = h->ensure(h->io_out, bytes))) return rc;
     if ((e = hipMemsetAsync(h->io_in.p, 0, bytes, h->stream)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
     hipEvent_t a, b;
@@ -128,18 +187,24 @@ extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc
         if (v.log2n != p->log2n) continue;
         for (int i = 0; i < 2; ++i)
             h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream, &v);
+        // best of three rounds of ten launches
         const int iters = 10;
-        hipEventRecord(a, h->stream);
-        for (int i = 0; i < iters; ++i)
-            h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream, &v);
-        hipEventRecord(b, h->stream);
-        if ((e = hipEventSynchronize(b)) != hipSuccess) {
-            hipEventDestroy(a);
-            hipEventDestroy(b);
-            return hip_fail(e, "autotune timing");
+        float ms = 1e30f;
+        for (int round = 0; round < 3; ++round) {
+            hipEventRecord(a, h->stream);
+            for (int i = 0; i < iters; ++i)
+                h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream,
+                       &v);
+            hipEventRecord(b, h->stream);
+            if ((e = hipEventSynchronize(b)) != hipSuccess) {
+                hipEventDestroy(a);
+                hipEventDestroy(b);
+                return hip_fail(e, "autotune timing");
+            }
+            float r = 0.f;
+            hipEventElapsedTime(&r, a, b);
+            ms = std::min(ms, r);
         }
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, a, b);
         if (ms < best_ms) {
             best_ms = ms;
             best = &v;
@@ -167,18 +232,22 @@ extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc
                 fused_adjoint(h, v, true, (const float*)h->io_in.p, w, m, nullptr, nullptr, (float*)h->io_aux.p, gw,
                               loss, n, batch, p->tw, h->stream);
             const int iters = 5;
-            hipEventRecord(a, h->stream);
-            for (int i = 0; i < iters; ++i)
-                fused_adjoint(h, v, true, (const float*)h->io_in.p, w, m, nullptr, nullptr, (float*)h->io_aux.p, gw,
-                              loss, n, batch, p->tw, h->stream);
-            hipEventRecord(b, h->stream);
-            if ((e = hipEventSynchronize(b)) != hipSuccess) {
-                hipEventDestroy(a);
-                hipEventDestroy(b);
-                return hip_fail(e, "autotune timing");
+            float ms = 1e30f;
+            for (int round = 0; round < 3; ++round) {
+                hipEventRecord(a, h->stream);
+                for (int i = 0; i < iters; ++i)
+                    fused_adjoint(h, v, true, (const float*)h->io_in.p, w, m, nullptr, nullptr, (float*)h->io_aux.p,
+                                  gw, loss, n, batch, p->tw, h->stream);
+                hipEventRecord(b, h->stream);
+                if ((e = hipEventSynchronize(b)) != hipSuccess) {
+                    hipEventDestroy(a);
+                    hipEventDestroy(b);
+                    return hip_fail(e, "autotune timing");
+                }
+                float r = 0.f;
+                hipEventElapsedTime(&r, a, b);
+                ms = std::min(ms, r);
             }
-            float ms = 0.f;
-            hipEventElapsedTime(&ms, a, b);
             if (ms < best_adj_ms) {
                 best_adj_ms = ms;
                 best_adj = v;

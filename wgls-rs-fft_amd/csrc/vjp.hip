@@ -128,6 +128,29 @@ __global__ void __launch_bounds__(256) k_sum_all(const float* __restrict__ part,
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
+// Column sums of the rows x ld partial matrix in one launch: block = 64
+// columns x 4 row lanes, each lane strides over the rows (coalesced 256-byte
+// row segments), fixed-order combine.  Columns < n go to gw; when
+// loss != NULL, column n holds per-row loss partials and is summed too.
+__global__ void __launch_bounds__(256)
+k_reduce_cols(const float* __restrict__ part, uint64_t ld, uint64_t rows, uint64_t n, float* __restrict__ gw,
+              float* __restrict__ loss) {
+    __shared__ float red[4][64];
+    const uint32_t lane = threadIdx.x & 63, rl = threadIdx.x >> 6;
+    const uint64_t col = (uint64_t)blockIdx.x * 64 + lane;
+    const uint64_t ncols = loss ? n + 1 : n;
+    float acc = 0.f;
+    if (col < ncols)
+        for (uint64_t r = rl; r < rows; r += 4) acc += part[r * ld + col];
+    red[rl][lane] = acc;
+    __syncthreads();
+    if (rl == 0 && col < ncols) {
+        const float v = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+        if (col < n) gw[col] = v;
+        else loss[0] = v;
+    }
+}
+
 // out[i] = sum_r part[r][i] (deterministic order).
 static int reduce_rows(Handle* h, const float* part, uint64_t rows, uint64_t n, float* out, hipStream_t s) {
     const uint64_t chunks = std::min<uint64_t>(rows, 16);
@@ -167,17 +190,20 @@ int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x
     int grid = v->blocks_per_cu(fwd) * h->num_cus;
     if ((uint64_t)grid > batch) grid = (int)batch;
     int rc;
-    if ((rc = h->ensure(h->red, (uint64_t)grid * n * sizeof(float))) ||
-        (rc = h->ensure(h->lossbuf, (uint64_t)grid * sizeof(float) + 256)))
-        return rc;
+    // partial matrix: grid rows of n + 4 floats (gw partials, then the loss
+    // partial in column n; rows kept 16-byte aligned)
+    const uint64_t ld = n + 4;
+    if ((rc = h->ensure(h->red, (uint64_t)grid * ld * sizeof(float)))) return rc;
     AdjointArgs a{(const float2*)d_x, d_w, d_mask, (const float2*)d_gy, (float2*)d_y, (float2*)d_gx, tw,
-                  (float*)h->red.p, (float*)h->lossbuf.p, batch, grid, s};
+                  (float*)h->red.p, ld, batch, grid, s};
     hipError_t e = v->launch(fwd, a);
     if (e != hipSuccess) return hip_fail(e, "adjoint kernel launch");
-    if (fwd) hipLaunchKernelGGL(k_sum_all, dim3(1), dim3(256), 0, s, (const float*)h->lossbuf.p, d_loss, grid);
+    const uint64_t ncols = fwd ? n + 1 : n;
+    hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)((ncols + 63) / 64)), dim3(256), 0, s, (const float*)h->red.p, ld,
+                       (uint64_t)grid, n, d_gw, fwd ? d_loss : nullptr);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "adjoint reduction launch");
-    return reduce_rows(h, (const float*)h->red.p, (uint64_t)grid, n, d_gw, s);
+    return WGFFT_OK;
 }
 
 }  // namespace wgfft

@@ -3,11 +3,11 @@
 // src/lib.rs:527-531).
 //
 // The host sorts the transforms by length (largest first) and cuts them into
-// work units: for N <= 8192 a unit is G transforms of one length that a
-// 256-thread workgroup transforms together in LDS (T*G = 256); for
-// N = 2^14 .. 2^16 a unit is one transform that the workgroup does as a
-// two-pass Stockham (radix R1 then R2 LDS sub-FFTs) through its own slot of
-// global scratch.  Units never depend on each other, so the only shared
+// work units: for N <= 4096 a unit is G transforms of one length that a
+// 256-thread workgroup transforms together in LDS (T*G = 256, 16 elements
+// per thread); for N = 2^13 .. 2^16 a unit is one transform that the
+// workgroup does as a two-pass Stockham (radix R1 then R2 LDS sub-FFTs)
+// through its own slot of global scratch.  Units never depend on each other, so the only shared
 // state is the queue head.  A persistent grid (occupancy x CUs) dequeues
 // units with one atomic per unit; the last workgroup to leave resets the
 // counters, so the launch can be captured once into a hipGraph and replayed.
@@ -33,10 +33,11 @@ struct RaggedUnit {
 };
 
 constexpr int kRaggedThreads = 256;
-constexpr int kMinLog2 = 6, kMaxLog2 = 16, kDirectMaxLog2 = 13;
+constexpr int kMinLog2 = 6, kMaxLog2 = 16, kDirectMaxLog2 = 12;
+constexpr int kRaggedE = 16;  // elements per thread in every unit
 
-// transforms per direct unit for length 2^k with E = 32
-constexpr int direct_g(int log2n) { return kRaggedThreads / ((1 << log2n) / 32); }
+// transforms per direct unit for length 2^k
+constexpr int direct_g(int log2n) { return kRaggedThreads / ((1 << log2n) / kRaggedE); }
 
 // LDS footprint (complex elements) of every unit kind
 template <int N, int E, int... Rs>
@@ -46,11 +47,11 @@ constexpr int pass_lds() { return (kRaggedThreads / (R / E)) * GlobalPassSlot<R,
 
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int kRaggedLds = cmax(
-    cmax(cmax(direct_lds<64, 32, 16, 4>(), direct_lds<128, 32, 16, 8>()),
-         cmax(direct_lds<256, 32, 16, 16>(), direct_lds<512, 32, 16, 16, 2>())),
-    cmax(cmax(cmax(direct_lds<1024, 32, 16, 16, 4>(), direct_lds<2048, 32, 16, 16, 8>()),
-              cmax(direct_lds<4096, 32, 16, 16, 16>(), direct_lds<8192, 32, 16, 16, 16, 2>())),
-         cmax(pass_lds<128, 32, 16, 8>(), pass_lds<256, 32, 16, 16>())));
+    cmax(cmax(direct_lds<64, 16, 8, 8>(), direct_lds<128, 16, 4, 4, 8>()),
+         cmax(direct_lds<256, 16, 8, 4, 8>(), direct_lds<512, 16, 8, 8, 8>())),
+    cmax(cmax(direct_lds<1024, 16, 8, 16, 8>(), direct_lds<2048, 16, 8, 4, 8, 8>()),
+         cmax(direct_lds<4096, 16, 8, 8, 8, 8>(),
+              cmax(cmax(pass_lds<64, 16, 8, 8>(), pass_lds<128, 16, 4, 4, 8>()), pass_lds<256, 16, 8, 4, 8>()))));
 
 // Unit handlers are separate (non-inlined) functions so the kernel's
 // register budget is the largest handler's, not the union of all of them.
@@ -149,32 +150,35 @@ k_ragged(const float2* __restrict__ in, float2* __restrict__ out, const RaggedUn
         if (u >= u_end) break;
         const RaggedUnit un = units[u];
         switch (un.log2n) {
-            case 6: ragged_direct<64, 32, INV, 16, 4>(in, out, offs, un.first, un.count, tw_all, lds); break;
-            case 7: ragged_direct<128, 32, INV, 16, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
-            case 8: ragged_direct<256, 32, INV, 16, 16>(in, out, offs, un.first, un.count, tw_all, lds); break;
-            case 9: ragged_direct<512, 32, INV, 16, 16, 2>(in, out, offs, un.first, un.count, tw_all, lds); break;
-            case 10: ragged_direct<1024, 32, INV, 16, 16, 4>(in, out, offs, un.first, un.count, tw_all, lds); break;
-            case 11: ragged_direct<2048, 32, INV, 16, 16, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
-            case 12: ragged_direct<4096, 32, INV, 16, 16, 16>(in, out, offs, un.first, un.count, tw_all, lds); break;
-            case 13:
-                ragged_direct<8192, 32, INV, 16, 16, 16, 2>(in, out, offs, un.first, un.count, tw_all, lds);
+            case 6: ragged_direct<64, 16, INV, 8, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
+            case 7: ragged_direct<128, 16, INV, 4, 4, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
+            case 8: ragged_direct<256, 16, INV, 8, 4, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
+            case 9: ragged_direct<512, 16, INV, 8, 8, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
+            case 10: ragged_direct<1024, 16, INV, 8, 16, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
+            case 11: ragged_direct<2048, 16, INV, 8, 4, 8, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
+            case 12: ragged_direct<4096, 16, INV, 8, 8, 8, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
+            case 13: {
+                const uint64_t o = offs[un.first];
+                RaggedBig<13, 128, 16, Radices<4, 4, 8>, 64, 16, Radices<8, 8>, INV, PF>::run(in + o, out + o,
+                                                                                          my_scratch, tw_all, lds);
                 break;
+            }
             case 14: {
                 const uint64_t o = offs[un.first];
-                RaggedBig<14, 128, 32, Radices<16, 8>, 128, 32, Radices<16, 8>, INV, PF>::run(in + o, out + o, my_scratch,
-                                                                                         tw_all, lds);
+                RaggedBig<14, 128, 16, Radices<4, 4, 8>, 128, 16, Radices<4, 4, 8>, INV, PF>::run(in + o, out + o,
+                                                                                              my_scratch, tw_all, lds);
                 break;
             }
             case 15: {
                 const uint64_t o = offs[un.first];
-                RaggedBig<15, 256, 32, Radices<16, 16>, 128, 32, Radices<16, 8>, INV, PF>::run(in + o, out + o,
-                                                                                          my_scratch, tw_all, lds);
+                RaggedBig<15, 256, 16, Radices<8, 4, 8>, 128, 16, Radices<4, 4, 8>, INV, PF>::run(in + o, out + o,
+                                                                                              my_scratch, tw_all, lds);
                 break;
             }
             default: {
                 const uint64_t o = offs[un.first];
-                RaggedBig<16, 256, 32, Radices<16, 16>, 256, 32, Radices<16, 16>, INV, PF>::run(in + o, out + o,
-                                                                                           my_scratch, tw_all, lds);
+                RaggedBig<16, 256, 16, Radices<8, 4, 8>, 256, 16, Radices<8, 4, 8>, INV, PF>::run(in + o, out + o,
+                                                                                              my_scratch, tw_all, lds);
                 break;
             }
         }

@@ -1,20 +1,25 @@
 // variants_global.hip -- sub-FFT variants for the passes of N > 16384:
-// radix R as an LDS sub-FFT over G butterflies per workgroup, with (PF = 1)
-// or without the next group's tile prefetched into registers.  For each R
-// the first registration is the default; wgfft_autotune times the others.
+// radix R as an LDS sub-FFT over G butterflies per workgroup with E
+// elements per thread, with (PF = 1) or without the next group's tile
+// prefetched into registers.  For each R the first registration is the
+// default; wgfft_autotune times the others.  E = 16 halves the registers of
+// the sub-FFT (more workgroups per CU), E = 32 halves its LDS passes.
 #include "kernels.hpp"
 
+WGFFT_REGISTER_GLOBAL_PASS(128, 16, 16, 0, 4x4x8, 4, 4, 8);
 WGFFT_REGISTER_GLOBAL_PASS(128, 32, 16, 0, 16x8, 16, 8);
+WGFFT_REGISTER_GLOBAL_PASS(256, 16, 16, 0, 8x4x8, 8, 4, 8);
 WGFFT_REGISTER_GLOBAL_PASS(256, 32, 16, 0, 16x16, 16, 16);
+WGFFT_REGISTER_GLOBAL_PASS(512, 16, 8, 0, 8x8x8, 8, 8, 8);
 WGFFT_REGISTER_GLOBAL_PASS(512, 32, 16, 0, 16x16x2, 16, 16, 2);
-WGFFT_REGISTER_GLOBAL_PASS(512, 32, 8, 0, 16x16x2, 16, 16, 2);
+WGFFT_REGISTER_GLOBAL_PASS(1024, 16, 4, 0, 8x16x8, 8, 16, 8);
 WGFFT_REGISTER_GLOBAL_PASS(1024, 32, 8, 0, 16x16x4, 16, 16, 4);
-WGFFT_REGISTER_GLOBAL_PASS(1024, 32, 4, 0, 16x16x4, 16, 16, 4);
-WGFFT_REGISTER_GLOBAL_PASS(2048, 32, 4, 0, 16x16x8, 16, 16, 8);
+WGFFT_REGISTER_GLOBAL_PASS(2048, 16, 2, 0, 8x4x8x8, 8, 4, 8, 8);
+WGFFT_REGISTER_GLOBAL_PASS(2048, 16, 4, 0, 8x4x8x8, 8, 4, 8, 8);
 WGFFT_REGISTER_GLOBAL_PASS(2048, 32, 8, 0, 16x16x8, 16, 16, 8);
-WGFFT_REGISTER_GLOBAL_PASS(2048, 32, 2, 0, 16x16x8, 16, 16, 8);
 WGFFT_REGISTER_GLOBAL_PASS(2048, 32, 4, 1, 16x16x8, 16, 16, 8);
-WGFFT_REGISTER_GLOBAL_PASS(4096, 32, 2, 0, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_GLOBAL_PASS(4096, 16, 2, 0, 8x8x8x8, 8, 8, 8, 8);
 WGFFT_REGISTER_GLOBAL_PASS(4096, 32, 4, 0, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_GLOBAL_PASS(8192, 16, 2, 0, 8x16x8x8, 8, 16, 8, 8);
 WGFFT_REGISTER_GLOBAL_PASS(8192, 32, 2, 0, 16x16x16x2, 16, 16, 16, 2);
 WGFFT_REGISTER_GLOBAL_PASS(8192, 32, 2, 1, 16x16x16x2, 16, 16, 16, 2);

@@ -140,8 +140,17 @@ k_reduce_cols(const float* __restrict__ part, uint64_t ld, uint64_t rows, uint64
     const uint64_t col = (uint64_t)blockIdx.x * 64 + lane;
     const uint64_t ncols = loss ? n + 1 : n;
     float acc = 0.f;
-    if (col < ncols)
-        for (uint64_t r = rl; r < rows; r += 4) acc += part[r * ld + col];
+    if (col < ncols) {
+        // eight independent loads in flight per step, summed in a fixed order
+        uint64_t r = rl;
+        for (; r + 28 < rows; r += 32) {
+            float v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = part[(r + 4 * i) * ld + col];
+            acc += ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+        }
+        for (; r < rows; r += 4) acc += part[r * ld + col];
+    }
     red[rl][lane] = acc;
     __syncthreads();
     if (rl == 0 && col < ncols) {

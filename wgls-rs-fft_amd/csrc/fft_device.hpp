@@ -34,6 +34,12 @@ __device__ __forceinline__ P* launder(P* p) {
     asm volatile("" : "+s"(p));
     return p;
 }
+// The same for a per-lane value (the thread index of a persistent loop whose
+// body dispatches to many code paths): nothing derived from it is hoisted.
+__device__ __forceinline__ uint32_t launder_lane(uint32_t v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }

@@ -234,9 +234,9 @@ struct TwOneLevel {
 // caller provides G * SLOT complex elements of LDS and the LDS barrier before
 // the next gp_compute.
 template <int R, int E, int G>
-__device__ __forceinline__ void gp_load(float2 (&v)[E], const float2* __restrict__ src, uint64_t NR, uint64_t b0) {
+__device__ __forceinline__ void gp_load(float2 (&v)[E], const float2* __restrict__ src, uint64_t NR, uint64_t b0,
+                                        uint32_t tid) {
     constexpr int THREADS = G * (R / E);
-    const uint32_t tid = threadIdx.x;
 #pragma unroll
     for (int q = 0; q < E / 2; ++q) {
         const int it = tid + THREADS * q;
@@ -251,13 +251,12 @@ __device__ __forceinline__ void gp_load(float2 (&v)[E], const float2* __restrict
 template <int R, int E, int G, int MODE, class TW, int... Rs>
 __device__ __forceinline__ void gp_compute(float2 (&v)[E], float2* __restrict__ dst, const float2* __restrict__ tw_r,
                                            const TW& twN, const float* __restrict__ win, uint64_t N, uint64_t L,
-                                           uint64_t b0, float sc, float2* __restrict__ lds_all) {
+                                           uint64_t b0, float sc, float2* __restrict__ lds_all, uint32_t tid) {
     constexpr int T = R / E;
     constexpr int THREADS = G * T;
     constexpr bool INV = MODE == kInverse;
     constexpr int SLOT = GlobalPassSlot<R, E, Rs...>::len;
     const uint64_t NR = N / R;
-    const uint32_t tid = threadIdx.x;
     const uint64_t s = N / ((uint64_t)R * L);  // W_{RL} = W_N^s
     const bool first = (L == 1);
 
@@ -351,26 +350,27 @@ k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const flo
     float2 cur[E];
     if (PF && lid0 < total) {
         const uint64_t tr = lid0 / groups_per_tr, b0 = (lid0 % groups_per_tr) * G;
-        gp_load<R, E, G>(cur, in + tr * N, NR, b0);
+        gp_load<R, E, G>(cur, in + tr * N, NR, b0, threadIdx.x);
     }
     for (uint64_t lid = lid0; lid < total; lid += nb) {
         const uint64_t tr = lid / groups_per_tr, b0 = (lid % groups_per_tr) * G;
+        const uint32_t tid = launder_lane(threadIdx.x);
         if constexpr (PF) {
             // next group's tile in flight while this one is transformed
             float2 nxt[E];
             const uint64_t nl = lid + nb;
             if (nl < total) {
                 const uint64_t ntr = nl / groups_per_tr, nb0 = (nl % groups_per_tr) * G;
-                gp_load<R, E, G>(nxt, in + ntr * N, NR, nb0);
+                gp_load<R, E, G>(nxt, in + ntr * N, NR, nb0, tid);
             }
             gp_compute<R, E, G, MODE, TwTwoLevel, Rs...>(cur, out + tr * N, tw_r, twN, win, N, L, b0,
-                                                         last ? scale : 1.0f, lds_all);
+                                                         last ? scale : 1.0f, lds_all, tid);
 #pragma unroll
             for (int i = 0; i < E; ++i) cur[i] = nxt[i];
         } else {
-            gp_load<R, E, G>(cur, in + tr * N, NR, b0);
+            gp_load<R, E, G>(cur, in + tr * N, NR, b0, tid);
             gp_compute<R, E, G, MODE, TwTwoLevel, Rs...>(cur, out + tr * N, tw_r, twN, win, N, L, b0,
-                                                         last ? scale : 1.0f, lds_all);
+                                                         last ? scale : 1.0f, lds_all, tid);
         }
         WGFFT_LDS_SYNC();
     }

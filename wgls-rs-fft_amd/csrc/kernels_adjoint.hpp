@@ -54,7 +54,6 @@ k_adjoint(AdjointArgs a) {
     __shared__ __align__(16) float2 lds[LEN];
     __shared__ float red[T];
 
-    const uint32_t t = threadIdx.x;
     float acc[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) acc[i] = 0.f;
@@ -63,6 +62,7 @@ k_adjoint(AdjointArgs a) {
     // w and m are re-read per transform (L1/L2 hits) to keep registers for
     // x, the transform data and the gw accumulators.
     for (uint64_t tr = blockIdx.x; tr < a.batch; tr += gridDim.x) {
+        const uint32_t t = launder_lane(threadIdx.x);  // keep per-thread addressing inside the loop
         float2 d[E];
         [[maybe_unused]] float2 xk[KEEPX && FWD ? E : 1];
         if constexpr (FWD) {
@@ -114,6 +114,7 @@ k_adjoint(AdjointArgs a) {
         store_pairs<N, E, T, false>(a.gx + tr * N, d, t, 1.f);
         WGFFT_LDS_SYNC();
     }
+    const uint32_t t = threadIdx.x;
     float2* pg = reinterpret_cast<float2*>(a.part + (uint64_t)blockIdx.x * a.ld);
 #pragma unroll
     for (int q = 0; q < E / 2; ++q) pg[t + T * q] = make_float2(acc[2 * q], acc[2 * q + 1]);

@@ -53,16 +53,18 @@ constexpr int kRaggedLds = cmax(
          cmax(direct_lds<4096, 16, 8, 8, 8, 8>(),
               cmax(cmax(pass_lds<64, 16, 8, 8>(), pass_lds<128, 16, 4, 4, 8>()), pass_lds<256, 16, 8, 4, 8>()))));
 
-// Unit handlers are separate (non-inlined) functions so the kernel's
-// register budget is the largest handler's, not the union of all of them.
+// Unit handlers take the thread index through launder_lane() once per unit,
+// so no handler's index arithmetic is hoisted out of the persistent loop and
+// the kernel's register budget stays the largest handler's.
 template <int N, int E, bool INV, int... Rs>
-__device__ __attribute__((noinline)) void ragged_direct(const float2* __restrict__ in, float2* __restrict__ out,
+__device__ __forceinline__ void ragged_direct(const float2* __restrict__ in, float2* __restrict__ out,
                                               const uint64_t* __restrict__ offs, uint32_t first, uint32_t count,
-                                              const float2* __restrict__ tw_all, float2* __restrict__ lds_raw) {
+                                              const float2* __restrict__ tw_all, float2* __restrict__ lds_raw,
+                                              uint32_t tid) {
     constexpr int T = N / E;
     constexpr int LEN = LdsLayout<N, E, Rs...>::len;
     static_assert(T * direct_g(ilog2(N)) == kRaggedThreads, "direct unit must fill the workgroup");
-    const uint32_t g = threadIdx.x / T, t = threadIdx.x % T;
+    const uint32_t g = tid / T, t = tid % T;
     const bool active = g < count;
     const uint64_t off = active ? offs[first + g] : 0;
     float2 d[E];
@@ -85,46 +87,48 @@ struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV, PF
     static constexpr int G2 = kRaggedThreads / (R2 / E2);
     static_assert(R1 * R2 == (1 << LOGN), "radices must multiply to N");
     static_assert(G2 <= R1, "second pass needs G <= L");
-    __device__ __attribute__((noinline)) static void run(const float2* __restrict__ src, float2* __restrict__ dst,
-                                                         float2* __restrict__ scratch,
-                                                         const float2* __restrict__ tw_all, float2* __restrict__ lds) {
+    __device__ __forceinline__ static void run(const float2* __restrict__ src, float2* __restrict__ dst,
+                                               float2* __restrict__ scratch, const float2* __restrict__ tw_all,
+                                               float2* __restrict__ lds, uint32_t tid) {
         constexpr int M = INV ? kInverse : kForward;
         const TwOneLevel twN{tw_all + N};
         static_assert(E1 == E2, "both passes share the prefetch registers");
         // pass 1 (L = 1): src -> scratch
         float2 cur[E1];
-        if (PF) gp_load<R1, E1, G1>(cur, src, N / R1, 0);
+        if (PF) gp_load<R1, E1, G1>(cur, src, N / R1, 0, tid);
+#pragma unroll 1
         for (uint64_t b0 = 0; b0 < N / R1; b0 += G1) {
             if constexpr (PF) {
                 float2 nxt[E1];
-                if (b0 + G1 < N / R1) gp_load<R1, E1, G1>(nxt, src, N / R1, b0 + G1);
+                if (b0 + G1 < N / R1) gp_load<R1, E1, G1>(nxt, src, N / R1, b0 + G1, tid);
                 gp_compute<R1, E1, G1, M, TwOneLevel, Rs1...>(cur, scratch, tw_all + R1, twN, nullptr, N, 1, b0,
-                                                              1.0f, lds);
+                                                              1.0f, lds, tid);
 #pragma unroll
                 for (int i = 0; i < E1; ++i) cur[i] = nxt[i];
             } else {
-                gp_load<R1, E1, G1>(cur, src, N / R1, b0);
+                gp_load<R1, E1, G1>(cur, src, N / R1, b0, tid);
                 gp_compute<R1, E1, G1, M, TwOneLevel, Rs1...>(cur, scratch, tw_all + R1, twN, nullptr, N, 1, b0,
-                                                              1.0f, lds);
+                                                              1.0f, lds, tid);
             }
             WGFFT_LDS_SYNC();
         }
         __syncthreads();  // scratch stores of the whole workgroup visible to its loads
         // pass 2 (L = R1): scratch -> dst
         const float sc = INV ? 1.0f / (float)N : 1.0f;
-        if (PF) gp_load<R2, E2, G2>(cur, scratch, N / R2, 0);
+        if (PF) gp_load<R2, E2, G2>(cur, scratch, N / R2, 0, tid);
+#pragma unroll 1
         for (uint64_t b0 = 0; b0 < N / R2; b0 += G2) {
             if constexpr (PF) {
                 float2 nxt[E2];
-                if (b0 + G2 < N / R2) gp_load<R2, E2, G2>(nxt, scratch, N / R2, b0 + G2);
+                if (b0 + G2 < N / R2) gp_load<R2, E2, G2>(nxt, scratch, N / R2, b0 + G2, tid);
                 gp_compute<R2, E2, G2, M, TwOneLevel, Rs2...>(cur, dst, tw_all + R2, twN, nullptr, N, R1, b0, sc,
-                                                              lds);
+                                                              lds, tid);
 #pragma unroll
                 for (int i = 0; i < E2; ++i) cur[i] = nxt[i];
             } else {
-                gp_load<R2, E2, G2>(cur, scratch, N / R2, b0);
+                gp_load<R2, E2, G2>(cur, scratch, N / R2, b0, tid);
                 gp_compute<R2, E2, G2, M, TwOneLevel, Rs2...>(cur, dst, tw_all + R2, twN, nullptr, N, R1, b0, sc,
-                                                              lds);
+                                                              lds, tid);
             }
             WGFFT_LDS_SYNC();
         }
@@ -149,36 +153,37 @@ k_ragged(const float2* __restrict__ in, float2* __restrict__ out, const RaggedUn
         __syncthreads();
         if (u >= u_end) break;
         const RaggedUnit un = units[u];
+        const uint32_t tid = launder_lane(threadIdx.x);
         switch (un.log2n) {
-            case 6: ragged_direct<64, 16, INV, 8, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
-            case 7: ragged_direct<128, 16, INV, 4, 4, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
-            case 8: ragged_direct<256, 16, INV, 8, 4, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
-            case 9: ragged_direct<512, 16, INV, 8, 8, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
-            case 10: ragged_direct<1024, 16, INV, 8, 16, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
-            case 11: ragged_direct<2048, 16, INV, 8, 4, 8, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
-            case 12: ragged_direct<4096, 16, INV, 8, 8, 8, 8>(in, out, offs, un.first, un.count, tw_all, lds); break;
+            case 6: ragged_direct<64, 16, INV, 8, 8>(in, out, offs, un.first, un.count, tw_all, lds, tid); break;
+            case 7: ragged_direct<128, 16, INV, 4, 4, 8>(in, out, offs, un.first, un.count, tw_all, lds, tid); break;
+            case 8: ragged_direct<256, 16, INV, 8, 4, 8>(in, out, offs, un.first, un.count, tw_all, lds, tid); break;
+            case 9: ragged_direct<512, 16, INV, 8, 8, 8>(in, out, offs, un.first, un.count, tw_all, lds, tid); break;
+            case 10: ragged_direct<1024, 16, INV, 8, 16, 8>(in, out, offs, un.first, un.count, tw_all, lds, tid); break;
+            case 11: ragged_direct<2048, 16, INV, 8, 4, 8, 8>(in, out, offs, un.first, un.count, tw_all, lds, tid); break;
+            case 12: ragged_direct<4096, 16, INV, 8, 8, 8, 8>(in, out, offs, un.first, un.count, tw_all, lds, tid); break;
             case 13: {
                 const uint64_t o = offs[un.first];
                 RaggedBig<13, 128, 16, Radices<4, 4, 8>, 64, 16, Radices<8, 8>, INV, PF>::run(in + o, out + o,
-                                                                                          my_scratch, tw_all, lds);
+                                                                                          my_scratch, tw_all, lds, tid);
                 break;
             }
             case 14: {
                 const uint64_t o = offs[un.first];
                 RaggedBig<14, 128, 16, Radices<4, 4, 8>, 128, 16, Radices<4, 4, 8>, INV, PF>::run(in + o, out + o,
-                                                                                              my_scratch, tw_all, lds);
+                                                                                              my_scratch, tw_all, lds, tid);
                 break;
             }
             case 15: {
                 const uint64_t o = offs[un.first];
                 RaggedBig<15, 256, 16, Radices<8, 4, 8>, 128, 16, Radices<4, 4, 8>, INV, PF>::run(in + o, out + o,
-                                                                                              my_scratch, tw_all, lds);
+                                                                                              my_scratch, tw_all, lds, tid);
                 break;
             }
             default: {
                 const uint64_t o = offs[un.first];
                 RaggedBig<16, 256, 16, Radices<8, 4, 8>, 256, 16, Radices<8, 4, 8>, INV, PF>::run(in + o, out + o,
-                                                                                              my_scratch, tw_all, lds);
+                                                                                              my_scratch, tw_all, lds, tid);
                 break;
             }
         }

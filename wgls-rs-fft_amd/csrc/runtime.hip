@@ -227,8 +227,7 @@ int Handle::run(Plan* p, Mode mode, const float2* in, float2* out, const float* 
         if (e != hipSuccess) return hip_fail(e, "batched kernel launch");
         return WGFFT_OK;
     }
-    // multi-pass: chunk the batch so the intermediate stays modest (and, at
-    // 2^22, inside the 256 MB last-level cache between the two passes)
+    // multi-pass: chunk the batch so the scratch intermediate stays bounded
     const uint64_t tbytes = p->n * sizeof(float2);
     uint64_t chunk = std::max<uint64_t>(1, mp_chunk_bytes / tbytes);
     chunk = std::min(chunk, batch);
@@ -375,7 +374,7 @@ void wgfft_destroy(wgfft_t* h) {
         hipStreamSynchronize(h->stream);
         for (auto& t : h->tables) hipFree(t.p);
         h->tables.clear();
-        for (DevBuf* b : {&h->io_in, &h->io_out, &h->io_w, &h->io_aux, &h->scratch, &h->scratch2, &h->red, &h->red2, &h->lossbuf})
+        for (DevBuf* b : {&h->io_in, &h->io_out, &h->io_w, &h->io_aux, &h->scratch, &h->scratch2, &h->red, &h->red2, &h->lossbuf, &h->counters})
             h->release(*b);
         hipStreamDestroy(h->stream);
     }

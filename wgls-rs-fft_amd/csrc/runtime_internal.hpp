@@ -67,9 +67,9 @@ struct Handle {
     std::map<int, const AdjointVariant*> tuned_adj;
     std::map<std::pair<const BatchedVariant*, int>, int> occ_cache;
     std::vector<DevBuf> tables;
-    DevBuf io_in, io_out, io_w, io_aux, scratch, scratch2, red, red2, lossbuf;
+    DevBuf io_in, io_out, io_w, io_aux, scratch, scratch2, red, red2, lossbuf, counters;
     uint64_t held = 0;
-    uint64_t mp_chunk_bytes = 64ull << 20;
+    uint64_t mp_chunk_bytes = 2ull << 30;  // batch chunk of the multipass path (bounds its scratch)
 
     int ensure(DevBuf& b, uint64_t bytes);
     void release(DevBuf& b);

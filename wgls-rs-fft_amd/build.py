@@ -52,6 +52,13 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     shutil.move(tmp, LIB)
+    # C++ mirror tests, linked against the library with an $ORIGIN-relative rpath
+    cxx_tests = os.path.join(OUT_DIR, "gpu_fft_tests")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", cxx_tests, os.path.join(HERE, "cpp", "gpu_fft_tests.cpp"),
+           "-L", OUT_DIR, "-lwgfft", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ mirror tests failed to build:\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB

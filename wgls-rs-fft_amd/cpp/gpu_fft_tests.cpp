@@ -1,0 +1,161 @@
+// gpu_fft_tests.cpp -- the reference's own tests restated against the C++
+// mirror (tests/gpu_fft_test.rs, tests/ifft_test.rs, tests/windowed_fft_test.rs,
+// src/lib.rs:207-227), checked against an in-file FP64 DFT.
+//   gpu_fft_tests cpu   error contract without a device
+//   gpu_fft_tests gpu   transforms on the GPU
+// Prints one line per check and exits non-zero on the first failure.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gpu_fft.hpp"
+
+using wgls_rs_fft::Batch;
+using wgls_rs_fft::Complex;
+using wgls_rs_fft::FftError;
+using wgls_rs_fft::GpuFft;
+
+static int failures = 0;
+#define CHECK(cond, what)                                                  \
+    do {                                                                   \
+        bool ok_ = (cond);                                                 \
+        std::printf("%s %s\n", ok_ ? "PASS" : "FAIL", what);               \
+        if (!ok_) ++failures;                                              \
+    } while (0)
+
+// Direct FP64 DFT (sign -1 forward, +1 inverse without scale), exact index reduction.
+static std::vector<std::complex<double>> dft(const std::vector<Complex>& x, int sign) {
+    const size_t n = x.size();
+    std::vector<std::complex<double>> y(n);
+    for (size_t k = 0; k < n; ++k) {
+        std::complex<double> acc = 0.0;
+        for (size_t j = 0; j < n; ++j) {
+            const double a = 2.0 * M_PI * (double)((k * j) % n) / (double)n;
+            acc += std::complex<double>(x[j]) * std::complex<double>(std::cos(a), sign * std::sin(a));
+        }
+        y[k] = acc;
+    }
+    return y;
+}
+
+static std::vector<Complex> make_input(size_t n) {  // tests/gpu_fft_test.rs:8-15
+    std::vector<Complex> v(n);
+    for (size_t i = 0; i < n; ++i) v[i] = Complex(std::sin((float)i * 0.1f), std::cos((float)i * 0.07f));
+    return v;
+}
+
+static std::vector<Complex> generate_test_signal(size_t n) {  // tests/test_utils.rs:21-34
+    std::vector<Complex> v(n);
+    const float pi = 3.14159265358979323846f;
+    for (size_t i = 0; i < n; ++i) {
+        const float t = (float)i / (float)n;
+        v[i] = Complex(0.7f * std::sin(2.0f * pi * 10.0f * t) + 0.3f * std::sin(2.0f * pi * 50.0f * t), 0.0f);
+    }
+    return v;
+}
+
+static void apply_hann_window(std::vector<Complex>& s) {  // tests/test_utils.rs:39-50
+    const size_t n = s.size();
+    const float pi = 3.14159265358979323846f;
+    for (size_t i = 0; i < n; ++i) s[i] *= 0.5f * (1.0f - std::cos(2.0f * pi * (float)i / (float)(n - 1)));
+}
+
+static float max_diff(const std::vector<Complex>& a, const std::vector<Complex>& b) {
+    float m = 0.f;
+    for (size_t i = 0; i < a.size(); ++i) m = std::max(m, std::abs(a[i] - b[i]));
+    return m;
+}
+
+static int run_cpu() {
+    CHECK(!GpuFft::is_gpu_available(), "no device in this process");
+    try {
+        GpuFft::create();
+        CHECK(false, "create() without a device throws");
+    } catch (const FftError& e) {
+        CHECK(e.code() == WGFFT_ERR_NO_DEVICE, "create() without a device throws WGFFT_ERR_NO_DEVICE");
+    }
+    uint64_t n = 0;
+    uint64_t ragged[2] = {8, 16}, bad[1] = {12};
+    CHECK(wgfft_validate_lengths(ragged, 2, &n) == WGFFT_ERR_RAGGED_BATCH &&
+              std::string(wgfft_last_error()) == "All input vectors in a batch must have the same length",
+          "ragged batch message");
+    CHECK(wgfft_validate_lengths(bad, 1, &n) == WGFFT_ERR_INVALID_LENGTH &&
+              std::string(wgfft_last_error()) == "Transform length must be a non-zero power of two",
+          "bad length message");
+    return failures ? 1 : 0;
+}
+
+static int run_gpu() {
+    GpuFft fft = GpuFft::create();
+    CHECK(!fft.name().empty(), "name()");
+    // tests/gpu_fft_test.rs:17-42
+    {
+        auto x = make_input(1024);
+        Batch out = fft.fft({x});
+        auto ref = dft(x, -1);
+        float m = 0.f;
+        for (size_t i = 0; i < 1024; ++i) m = std::max(m, (float)std::abs(std::complex<double>(out[0][i]) - ref[i]));
+        CHECK(out.size() == 1 && out[0].size() == 1024 && m < 1e-3f, "gpu_fft_matches_rustfft (1e-3 vs FP64)");
+    }
+    // tests/ifft_test.rs:22-107
+    {
+        auto x = make_input(1024);
+        Batch spec = fft.fft({x});
+        Batch rec = fft.ifft(spec);
+        CHECK(max_diff(rec[0], x) < 1e-3f, "ifft roundtrip");
+        Batch spec2 = fft.fft(rec);
+        CHECK(max_diff(spec2[0], spec[0]) < 1e-3f, "ifft->fft roundtrip");
+    }
+    // src/lib.rs:207-227
+    {
+        std::vector<Complex> x(1024);
+        for (size_t i = 0; i < 1024; ++i) x[i] = Complex((float)i * 0.1f, 0.f);
+        Batch rec = fft.ifft(fft.fft({x}));
+        CHECK(max_diff(rec[0], x) < 1e-4f, "roundtrip_consistency (1e-4)");
+    }
+    // tests/windowed_fft_test.rs:10-133 (N = 2048)
+    {
+        auto sig = generate_test_signal(2048);
+        auto win = sig;
+        apply_hann_window(win);
+        Batch spec = fft.fft({win});
+        CHECK(max_diff(fft.ifft(spec)[0], win) < 1e-3f, "windowed roundtrip");
+        std::vector<float> p(2048);
+        for (size_t i = 0; i < 2048; ++i) p[i] = std::norm(spec[0][i]);
+        bool p10 = false, p50 = false;
+        for (size_t i = 1; i < 1024; ++i)
+            if (p[i] > p[i - 1] && p[i] > p[i + 1] && p[i] > 10.f) {
+                p10 |= (i >= 8 && i < 12);
+                p50 |= (i >= 45 && i < 55);
+            }
+        CHECK(p10 && p50, "frequency detection at 10 and 50");
+        std::vector<float> w(2048);
+        const float pi = 3.14159265358979323846f;
+        for (size_t i = 0; i < 2048; ++i) w[i] = 0.5f * (1.0f - std::cos(2.0f * pi * (float)i / 2047.0f));
+        Batch fused = fft.fft_windowed({sig}, w);
+        CHECK(max_diff(fused[0], spec[0]) < 1e-3f, "fft_windowed equals window-then-fft");
+    }
+    // error contract on a live executor
+    CHECK(fft.fft({}).empty(), "empty batch -> empty result");
+    try {
+        fft.fft({std::vector<Complex>(8), std::vector<Complex>(16)});
+        CHECK(false, "ragged batch throws");
+    } catch (const FftError& e) {
+        CHECK(e.code() == WGFFT_ERR_RAGGED_BATCH, "ragged batch throws");
+    }
+    try {
+        fft.ifft({std::vector<Complex>(12)});
+        CHECK(false, "non-power-of-two throws");
+    } catch (const FftError& e) {
+        CHECK(e.code() == WGFFT_ERR_INVALID_LENGTH, "non-power-of-two throws");
+    }
+    return failures ? 1 : 0;
+}
+
+int main(int argc, char** argv) {
+    const std::string mode = argc > 1 ? argv[1] : "cpu";
+    return mode == "gpu" ? run_gpu() : run_cpu();
+}

@@ -34,7 +34,7 @@ struct BatchedArgs {
 
 struct BatchedVariant {
     int log2n;
-    int E, G, threads;
+    int E, G, nt, threads;
     int lds_bytes;
     const char* plan;  // radix split, e.g. "16x16x16"
     hipError_t (*launch)(Mode, const BatchedArgs&);
@@ -99,7 +99,9 @@ __device__ __forceinline__ void window_pairs(float2 (&d)[E], const float* __rest
     }
 }
 
-template <int N, int E, int T, bool SCALE>
+// NT: non-temporal (streaming) stores -- the output is not re-read by the
+// kernel, so it need not displace the input stream in L2.
+template <int N, int E, int T, bool SCALE, bool NT = false>
 __device__ __forceinline__ void store_pairs(float2* __restrict__ dst, const float2 (&d)[E], uint32_t t,
                                             float scale) {
     float4* d4 = reinterpret_cast<float4*>(dst);
@@ -110,12 +112,18 @@ __device__ __forceinline__ void store_pairs(float2* __restrict__ dst, const floa
             a = cscale(a, scale);
             b = cscale(b, scale);
         }
-        d4[t + T * q] = make_float4(a.x, a.y, b.x, b.y);
+        if constexpr (NT) {
+            typedef float f4v __attribute__((ext_vector_type(4)));
+            const f4v v = {a.x, a.y, b.x, b.y};
+            __builtin_nontemporal_store(v, reinterpret_cast<f4v*>(d4) + t + T * q);
+        } else {
+            d4[t + T * q] = make_float4(a.x, a.y, b.x, b.y);
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-template <int N, int E, int G, int MODE, int... Rs>
+template <int N, int E, int G, bool NT, int MODE, int... Rs>
 __global__ void __launch_bounds__(G*(N / E))
 k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
           const float* __restrict__ win, uint64_t batch, float scale) {
@@ -143,12 +151,12 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
         if (ngrp < ngroups && ngrp * G + g < batch) load_pairs<N, E, T>(nxt, in + (ngrp * G + g) * N, t);
         if constexpr (MODE == kWindowed) window_pairs<N, E, T>(d, win, t);
         Engine<N, E, T, INV>::run(d, lds, tw, t, Radices<Rs...>{});
-        if (active) store_pairs<N, E, T, INV>(out + tr * N, d, t, scale);
+        if (active) store_pairs<N, E, T, INV, NT>(out + tr * N, d, t, scale);
         WGFFT_LDS_SYNC();  // the next group's first pass overwrites the exchange buffer
     }
 }
 
-template <int N, int E, int G, int... Rs>
+template <int N, int E, int G, bool NT, int... Rs>
 struct BatchedKernel {
     static constexpr int T = N / E;
     static constexpr int THREADS = G * T;
@@ -160,7 +168,7 @@ struct BatchedKernel {
         uint64_t grid = a.grid > 0 ? (uint64_t)a.grid : ngroups;
         if (grid > ngroups) grid = ngroups;
         if (grid == 0) return hipSuccess;
-        hipLaunchKernelGGL((k_batched<N, E, G, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0,
+        hipLaunchKernelGGL((k_batched<N, E, G, NT, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0,
                            a.stream, a.in, a.out, a.tw, a.win, a.batch, a.scale);
         return hipGetLastError();
     }
@@ -174,7 +182,7 @@ struct BatchedKernel {
     template <int MODE>
     static int occ() {
         int blocks = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_batched<N, E, G, MODE, Rs...>,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_batched<N, E, G, NT, MODE, Rs...>,
                                                          THREADS, 0) != hipSuccess)
             return 1;
         return blocks > 0 ? blocks : 1;
@@ -188,12 +196,13 @@ struct BatchedKernel {
     }
 };
 
-#define WGFFT_REGISTER_BATCHED(N, E, G, PLAN, ...)                                                      \
-    static ::wgfft::Registrar _wgfft_reg_##N##_##E##_##G##_##PLAN(::wgfft::BatchedVariant{                                     \
-        ::wgfft::ilog2(N), E, G, ::wgfft::BatchedKernel<N, E, G, __VA_ARGS__>::THREADS,                  \
-        ::wgfft::BatchedKernel<N, E, G, __VA_ARGS__>::LDS_BYTES, #PLAN,                                 \
-        &::wgfft::BatchedKernel<N, E, G, __VA_ARGS__>::launch,                                          \
-        &::wgfft::BatchedKernel<N, E, G, __VA_ARGS__>::blocks_per_cu})
+#define WGFFT_REGISTER_BATCHED_NT(N, E, G, NT, PLAN, ...)                                               \
+    static ::wgfft::Registrar _wgfft_reg_##N##_##E##_##G##_##NT##_##PLAN(::wgfft::BatchedVariant{      \
+        ::wgfft::ilog2(N), E, G, NT, ::wgfft::BatchedKernel<N, E, G, NT, __VA_ARGS__>::THREADS,          \
+        ::wgfft::BatchedKernel<N, E, G, NT, __VA_ARGS__>::LDS_BYTES, #PLAN,                             \
+        &::wgfft::BatchedKernel<N, E, G, NT, __VA_ARGS__>::launch,                                      \
+        &::wgfft::BatchedKernel<N, E, G, NT, __VA_ARGS__>::blocks_per_cu})
+#define WGFFT_REGISTER_BATCHED(N, E, G, PLAN, ...) WGFFT_REGISTER_BATCHED_NT(N, E, G, 0, PLAN, __VA_ARGS__)
 
 // LDS slot (complex elements) per butterfly of a global pass: the larger of
 // the sub-FFT's exchange footprint and the staging layout (R + R/32), kept

@@ -9,8 +9,9 @@
 // workgroup does as a two-pass Stockham (radix R1 then R2 LDS sub-FFTs)
 // through its own slot of global scratch.  Units never depend on each other, so the only shared
 // state is the queue head.  A persistent grid (occupancy x CUs) dequeues
-// units with one atomic per unit; the last workgroup to leave resets the
-// counters, so the launch can be captured once into a hipGraph and replayed.
+// units with one atomic ticket per unit; the workgroup holding the launch's
+// last ticket resets the counter, so the launch can be captured once into a
+// hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -146,8 +147,16 @@ k_ragged(const float2* __restrict__ in, float2* __restrict__ out, const RaggedUn
     float2* my_scratch = scratch + (uint64_t)blockIdx.x * (1ull << kMaxLog2);
 
     for (;;) {
-        if (threadIdx.x == 0)
-            s_unit = u_begin + __hip_atomic_fetch_add(&counters[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0) {
+            // one ticket per dequeue; every workgroup draws exactly one ticket past the end
+            const uint32_t ticket = __hip_atomic_fetch_add(&counters[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_unit = u_begin + ticket;
+            // the last ticket ever drawn in this launch resets the queue for the
+            // next launch or graph replay: all other draws are ordered before it
+            // on this one location, so no fence is needed
+            if (ticket == (u_end - u_begin) + gridDim.x - 1)
+                __hip_atomic_exchange(&counters[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         __syncthreads();
         const uint32_t u = s_unit;
         __syncthreads();
@@ -189,14 +198,7 @@ k_ragged(const float2* __restrict__ in, float2* __restrict__ out, const RaggedUn
         }
         WGFFT_LDS_SYNC();
     }
-    // the last workgroup out resets the queue for the next launch / replay
-    if (threadIdx.x == 0) {
-        const uint32_t prev = __hip_atomic_fetch_add(&counters[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == gridDim.x - 1) {
-            __hip_atomic_exchange(&counters[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_exchange(&counters[1], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+}
 }
 
 static int ragged_blocks_per_cu(bool pf) {

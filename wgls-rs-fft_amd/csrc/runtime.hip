@@ -131,9 +131,9 @@ const BatchedVariant* default_variant(int log2n) {
     return nullptr;
 }
 
-const BatchedVariant* find_variant(int log2n, const char* plan, int E, int G) {
+const BatchedVariant* find_variant(int log2n, const char* plan, int E, int G, int nt) {
     for (const auto& v : batched_registry())
-        if (v.log2n == log2n && v.E == E && v.G == G && std::strcmp(v.plan, plan) == 0) return &v;
+        if (v.log2n == log2n && v.E == E && v.G == G && v.nt == nt && std::strcmp(v.plan, plan) == 0) return &v;
     return nullptr;
 }
 
@@ -374,7 +374,7 @@ void wgfft_destroy(wgfft_t* h) {
         hipStreamSynchronize(h->stream);
         for (auto& t : h->tables) hipFree(t.p);
         h->tables.clear();
-        for (DevBuf* b : {&h->io_in, &h->io_out, &h->io_w, &h->io_aux, &h->scratch, &h->scratch2, &h->red, &h->red2, &h->lossbuf, &h->counters})
+        for (DevBuf* b : {&h->io_in, &h->io_out, &h->io_w, &h->io_aux, &h->scratch, &h->scratch2, &h->red, &h->red2, &h->lossbuf})
             h->release(*b);
         hipStreamDestroy(h->stream);
     }

@@ -54,7 +54,7 @@ int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x
 const char* adjoint_describe(const AdjointVariant* v, char* buf, uint64_t len);
 void synth_uniform(float* out, uint64_t nfloats, uint64_t seed);
 const BatchedVariant* default_variant(int log2n);
-const BatchedVariant* find_variant(int log2n, const char* plan, int E, int G);
+const BatchedVariant* find_variant(int log2n, const char* plan, int E, int G, int nt);
 std::vector<const GlobalPassVariant*> global_pass_candidates(int log2r, uint64_t L, uint64_t nr);
 
 struct Handle {
@@ -67,7 +67,7 @@ struct Handle {
     std::map<int, const AdjointVariant*> tuned_adj;
     std::map<std::pair<const BatchedVariant*, int>, int> occ_cache;
     std::vector<DevBuf> tables;
-    DevBuf io_in, io_out, io_w, io_aux, scratch, scratch2, red, red2, lossbuf, counters;
+    DevBuf io_in, io_out, io_w, io_aux, scratch, scratch2, red, red2, lossbuf;
     uint64_t held = 0;
     uint64_t mp_chunk_bytes = 2ull << 30;  // batch chunk of the multipass path (bounds its scratch)
 

@@ -4,3 +4,5 @@
 WGFFT_REGISTER_BATCHED(4096, 32, 1, 16x16x16, 16, 16, 16);
 WGFFT_REGISTER_BATCHED(4096, 32, 2, 16x16x16, 16, 16, 16);
 WGFFT_REGISTER_BATCHED(4096, 16, 1, 8x8x8x8, 8, 8, 8, 8);
+WGFFT_REGISTER_BATCHED_NT(4096, 32, 1, 1, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_BATCHED_NT(4096, 16, 1, 1, 8x8x8x8, 8, 8, 8, 8);

@@ -128,64 +128,41 @@ __global__ void __launch_bounds__(256) k_sum_all(const float* __restrict__ part,
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
-// Column sums of the rows x ld partial matrix in one launch.  Block (x, y)
-// reduces 64 columns over row chunk y (4 row lanes striding the chunk with
-// eight loads in flight, fixed-order combine) into chunk_out[y][col]; the
-// last of the kRowChunks blocks of column block x to finish (counted with an
-// agent-scope atomic) folds the chunk sums in chunk order and resets its
-// counter for the next launch.  Columns < n go to gw; when loss != NULL,
-// column n holds per-row loss partials and is summed into loss[0].
-constexpr int kRowChunks2 = 8;
+// Column sums of the rows x ld partial matrix in one launch: a block is 64
+// columns x 16 row lanes, each lane striding the rows with eight loads in
+// flight; lanes combine in a fixed order through LDS (no cross-block
+// ordering, so no device-scope fences).  Columns < n go to gw; when
+// loss != NULL, column n holds per-row loss partials summed into loss[0].
+constexpr int kRowLanes = 16;
 
-__global__ void __launch_bounds__(256) k_zero_u32(uint32_t* __restrict__ p, uint64_t count) {
-    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-    if (i < count) p[i] = 0u;
-}
-
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64 * kRowLanes)
 k_reduce_cols(const float* __restrict__ part, uint64_t ld, uint64_t rows, uint64_t n, float* __restrict__ gw,
-              float* __restrict__ loss, float* __restrict__ chunk_out, uint32_t* __restrict__ counters) {
-    __shared__ float red[4][64];
-    __shared__ uint32_t s_last;
+              float* __restrict__ loss) {
+    __shared__ float red[kRowLanes][64];
     const uint32_t lane = threadIdx.x & 63, rl = threadIdx.x >> 6;
     const uint64_t col = (uint64_t)blockIdx.x * 64 + lane;
     const uint64_t ncols = loss ? n + 1 : n;
-    const uint64_t per = (rows + kRowChunks2 - 1) / kRowChunks2;
-    const uint64_t r0 = (uint64_t)blockIdx.y * per;
-    const uint64_t r1 = min(rows, r0 + per);
     float acc = 0.f;
     if (col < ncols) {
-        uint64_t r = r0 + rl;
-        for (; r + 28 < r1; r += 32) {
+        uint64_t r = rl;
+        for (; r + 7 * kRowLanes < rows; r += 8 * kRowLanes) {
             float v[8];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) v[i] = part[(r + 4 * i) * ld + col];
+            for (int i = 0; i < 8; ++i) v[i] = part[(r + (uint64_t)kRowLanes * i) * ld + col];
             acc += ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
         }
-        for (; r < r1; r += 4) acc += part[r * ld + col];
+        for (; r < rows; r += kRowLanes) acc += part[r * ld + col];
     }
     red[rl][lane] = acc;
     __syncthreads();
-    if (rl == 0 && col < ncols)
-        chunk_out[(uint64_t)blockIdx.y * ld + col] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t prev =
-            __hip_atomic_fetch_add(&counters[blockIdx.x], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = prev == kRowChunks2 - 1;
+    for (int w = kRowLanes / 2; w > 0; w >>= 1) {
+        if ((int)rl < w) red[rl][lane] += red[rl + w][lane];
+        __syncthreads();
     }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
     if (rl == 0 && col < ncols) {
-        float v = 0.f;
-        for (int c = 0; c < kRowChunks2; ++c) v += __builtin_nontemporal_load(&chunk_out[(uint64_t)c * ld + col]);
-        if (col < n) gw[col] = v;
-        else loss[0] = v;
+        if (col < n) gw[col] = red[0][lane];
+        else loss[0] = red[0][lane];
     }
-    if (threadIdx.x == 0)
-        __hip_atomic_store(&counters[blockIdx.x], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // out[i] = sum_r part[r][i] (deterministic order).
@@ -236,19 +213,8 @@ int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x
     hipError_t e = v->launch(fwd, a);
     if (e != hipSuccess) return hip_fail(e, "adjoint kernel launch");
     const uint64_t ncols = fwd ? n + 1 : n;
-    const uint64_t cblocks = (ncols + 63) / 64;
-    if ((rc = h->ensure(h->red2, kRowChunks2 * ld * sizeof(float)))) return rc;
-    if (h->counters.bytes < cblocks * sizeof(uint32_t)) {
-        if ((rc = h->ensure(h->counters, cblocks * sizeof(uint32_t)))) return rc;
-        // zeroed once (a kernel, not a memset node, so the call can be
-        // captured into a graph); every reduction leaves its counters at zero
-        const uint64_t cnt = h->counters.bytes / sizeof(uint32_t);
-        hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
-                           (uint32_t*)h->counters.p, cnt);
-    }
-    hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)cblocks, kRowChunks2), dim3(256), 0, s, (const float*)h->red.p,
-                       ld, (uint64_t)grid, n, d_gw, fwd ? d_loss : nullptr, (float*)h->red2.p,
-                       (uint32_t*)h->counters.p);
+    hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)((ncols + 63) / 64)), dim3(64 * kRowLanes), 0, s,
+                       (const float*)h->red.p, ld, (uint64_t)grid, n, d_gw, fwd ? d_loss : nullptr);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "adjoint reduction launch");
     return WGFFT_OK;

@@ -199,7 +199,6 @@ k_ragged(const float2* __restrict__ in, float2* __restrict__ out, const RaggedUn
         WGFFT_LDS_SYNC();
     }
 }
-}
 
 static int ragged_blocks_per_cu(bool pf) {
     int b = 0;

@@ -244,29 +244,44 @@ struct RadixInfo<R, Rest...> {
     static constexpr int count = 1 + RadixInfo<Rest...>::count;
 };
 
+// Global I/O layout of a plan.  PAIR (first and last radix <= E/2): thread t
+// owns element pairs, register slot 2q + delta = element 2t + delta + 2T q,
+// so global traffic is 16-byte vectors.  SINGLE (first radix = last radix =
+// R, E/R butterflies per thread): slot i R + m = element t + T i + m N/R,
+// 8-byte lanes, one radix-R butterfly per slot group with no extra
+// exchange; it lets E = 16 run radix-16 passes.
+template <int E, int... Rs>
+struct IoKind {
+    static constexpr int RF = RadixInfo<Rs...>::first;
+    static constexpr int RL = RadixInfo<Rs...>::last;
+    static constexpr bool PAIR = (E % (2 * RF) == 0) && (E % (2 * RL) == 0);
+    static_assert(PAIR || RF == RL, "single-layout plans need equal first and last radix");
+};
+
 // Exchange padding written after a pass of radix R at stride L.
-//  first pass (pairs of butterflies per thread, output b R + m): lanes are
-//  2R apart, so one pad element per 2R staggers them over all banks;
+//  first pass, pair layout (output b R + m, lanes 2R apart): one pad element
+//  per 2R staggers the lanes over all banks; single layout (lanes R apart):
+//  one per R;
 //  later passes with L < 32 write runs of L contiguous elements spaced R L
 //  apart, so a pad of L per R L staggers the runs;
 //  L >= 32 writes are already contiguous.
-template <int R, int L, bool FIRST>
+template <int R, int L, bool FIRST, bool PAIR>
 struct ExchangePad {
-    static constexpr int A = FIRST ? ilog2(2 * R) : (L < 32 ? ilog2(R * L) : 30);
+    static constexpr int A = FIRST ? ilog2(PAIR ? 2 * R : R) : (L < 32 ? ilog2(R * L) : 30);
     static constexpr int B = FIRST ? 0 : (L < 32 ? ilog2(L) : 0);
 };
 
 template <int N, int E, int... Rs>
 struct LdsLayout;
 // Max padded length over the exchanges of a plan (the LDS footprint).
-template <int N, int E>
+template <int N, int E, bool PAIR>
 struct LdsLayoutAcc {
     template <int L, bool FIRST, int R, int... Rest>
     static constexpr int max_len() {
         if constexpr (sizeof...(Rest) == 0) {
             return N;
         } else {
-            using P = ExchangePad<R, L, FIRST>;
+            using P = ExchangePad<R, L, FIRST, PAIR>;
             constexpr int here = padded_len(N, P::A, P::B);
             constexpr int rest = max_len<L * R, false, Rest...>();
             return here > rest ? here : rest;
@@ -276,14 +291,14 @@ struct LdsLayoutAcc {
 
 template <int N, int E, int... Rs>
 struct LdsLayout {
-    static constexpr int raw = LdsLayoutAcc<N, E>::template max_len<1, true, Rs...>();
+    static constexpr int raw =
+        LdsLayoutAcc<N, E, IoKind<E, Rs...>::PAIR>::template max_len<1, true, Rs...>();
     // keep every transform's LDS slot 16-byte aligned
     static constexpr int len = (raw + 1) & ~1;
 };
 
 // Stockham pass engine.  d[] holds the thread's E elements; on entry (first
-// pass) and on exit (last pass) in the pair layout d[2q + delta] =
-// X[2t + delta + 2T q].
+// pass) and on exit (last pass) in the plan's global I/O layout (IoKind).
 template <int N, int E, int T, bool INV>
 struct Engine {
     static_assert(N % E == 0, "E must divide N");
@@ -292,11 +307,11 @@ struct Engine {
     template <int R>
     static constexpr int pair_slots() { return E / (2 * R); }
 
-    template <int L, int PA, int PB, bool FIRST, int R, int... Rest>
+    template <bool PAIR, int L, int PA, int PB, bool FIRST, int R, int... Rest>
     __device__ __forceinline__ static void passes(float2 (&d)[E], float2* __restrict__ lds,
                                                   const float2* __restrict__ tw, uint32_t t) {
         constexpr bool LAST = sizeof...(Rest) == 0;
-        constexpr bool IO = FIRST || LAST;
+        constexpr bool IO = (FIRST || LAST) && PAIR;
         constexpr int NR = N / R;
         static_assert(E % R == 0, "radix must divide E");
         static_assert(!IO || (E % (2 * R) == 0), "first/last radix must satisfy E >= 2R");
@@ -352,9 +367,11 @@ struct Engine {
                 float2 v[R];
 #pragma unroll
                 for (int m = 0; m < R; ++m) v[m] = d[i * R + m];
-                uint32_t b = t + T * i;
-                uint32_t k = b & (L - 1);
-                apply_twiddles<R, INV>(v, tw, k * (N / (R * L)));
+                if constexpr (L > 1) {
+                    uint32_t b = t + T * i;
+                    uint32_t k = b & (L - 1);
+                    apply_twiddles<R, INV>(v, tw, k * (N / (R * L)));
+                }
                 Dft<R, INV>::run(v);
 #pragma unroll
                 for (int m = 0; m < R; ++m) d[i * R + m] = v[m];
@@ -362,7 +379,7 @@ struct Engine {
         }
 
         if constexpr (!LAST) {
-            using P = ExchangePad<R, L, FIRST>;
+            using P = ExchangePad<R, L, FIRST, PAIR>;
             if constexpr (!FIRST) WGFFT_LDS_SYNC();  // everyone has read the previous exchange
             if constexpr (IO) {
 #pragma unroll
@@ -386,7 +403,7 @@ struct Engine {
                 }
             }
             WGFFT_LDS_SYNC();
-            passes<L * R, P::A, P::B, false, Rest...>(d, lds, tw, t);
+            passes<PAIR, L * R, P::A, P::B, false, Rest...>(d, lds, tw, t);
         }
     }
 
@@ -395,7 +412,7 @@ struct Engine {
                                                const float2* __restrict__ tw, uint32_t t,
                                                Radices<Rs...>) {
         static_assert(RadixInfo<Rs...>::product == N, "radices must multiply to N");
-        passes<1, 30, 0, true, Rs...>(d, lds, tw, t);
+        passes<IoKind<E, Rs...>::PAIR, 1, 30, 0, true, Rs...>(d, lds, tw, t);
     }
 };
 

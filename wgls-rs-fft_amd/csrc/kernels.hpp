@@ -122,6 +122,70 @@ __device__ __forceinline__ void store_pairs(float2* __restrict__ dst, const floa
     }
 }
 
+// Layout-generic global I/O of one transform (see IoKind): element index of
+// register slot s of thread t, and complex / real loads and stores.
+template <int N, int E, int... Rs>
+struct Io {
+    static constexpr int T = N / E;
+    static constexpr bool PAIR = IoKind<E, Rs...>::PAIR;
+    static constexpr int R = IoKind<E, Rs...>::RF;
+    __device__ __forceinline__ static uint32_t elem(int s, uint32_t t) {
+        if constexpr (PAIR) return 2 * t + (s & 1) + 2 * T * (s >> 1);
+        else return t + T * (s / R) + (s % R) * (N / R);
+    }
+    __device__ __forceinline__ static void load(float2 (&d)[E], const float2* __restrict__ src, uint32_t t) {
+        if constexpr (PAIR) {
+            load_pairs<N, E, T>(d, src, t);
+        } else {
+#pragma unroll
+            for (int s = 0; s < E; ++s) d[s] = src[elem(s, t)];
+        }
+    }
+    template <bool SCALE, bool NT = false>
+    __device__ __forceinline__ static void store(float2* __restrict__ dst, const float2 (&d)[E], uint32_t t,
+                                                 float scale) {
+        if constexpr (PAIR) {
+            store_pairs<N, E, T, SCALE, NT>(dst, d, t, scale);
+        } else {
+#pragma unroll
+            for (int s = 0; s < E; ++s) {
+                float2 v = SCALE ? cscale(d[s], scale) : d[s];
+                if constexpr (NT) {
+                    typedef float f2v __attribute__((ext_vector_type(2)));
+                    const f2v nv = {v.x, v.y};
+                    __builtin_nontemporal_store(nv, reinterpret_cast<f2v*>(dst) + elem(s, t));
+                } else {
+                    dst[elem(s, t)] = v;
+                }
+            }
+        }
+    }
+    __device__ __forceinline__ static void load_real(float (&r)[E], const float* __restrict__ src, uint32_t t) {
+        if constexpr (PAIR) {
+            const float2* s2 = reinterpret_cast<const float2*>(src);
+#pragma unroll
+            for (int q = 0; q < E / 2; ++q) {
+                float2 v = s2[t + T * q];
+                r[2 * q] = v.x;
+                r[2 * q + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < E; ++s) r[s] = src[elem(s, t)];
+        }
+    }
+    __device__ __forceinline__ static void store_real(float* __restrict__ dst, const float (&r)[E], uint32_t t) {
+        if constexpr (PAIR) {
+            float2* d2 = reinterpret_cast<float2*>(dst);
+#pragma unroll
+            for (int q = 0; q < E / 2; ++q) d2[t + T * q] = make_float2(r[2 * q], r[2 * q + 1]);
+        } else {
+#pragma unroll
+            for (int s = 0; s < E; ++s) dst[elem(s, t)] = r[s];
+        }
+    }
+};
+
 // ---------------------------------------------------------------------------
 template <int N, int E, int G, bool NT, int MODE, int... Rs>
 __global__ void __launch_bounds__(G*(N / E))
@@ -137,9 +201,10 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
     float2* lds = lds_all + g * LEN;
     const uint64_t ngroups = (batch + G - 1) / G;
 
+    using IO = Io<N, E, Rs...>;
     uint64_t grp = blockIdx.x;
     float2 nxt[E];
-    if (grp < ngroups && grp * G + g < batch) load_pairs<N, E, T>(nxt, in + (grp * G + g) * N, t);
+    if (grp < ngroups && grp * G + g < batch) IO::load(nxt, in + (grp * G + g) * N, t);
 
     for (; grp < ngroups; grp += gridDim.x) {
         const uint64_t tr = grp * G + g;
@@ -148,10 +213,15 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
 #pragma unroll
         for (int i = 0; i < E; ++i) d[i] = nxt[i];
         const uint64_t ngrp = grp + gridDim.x;
-        if (ngrp < ngroups && ngrp * G + g < batch) load_pairs<N, E, T>(nxt, in + (ngrp * G + g) * N, t);
-        if constexpr (MODE == kWindowed) window_pairs<N, E, T>(d, win, t);
+        if (ngrp < ngroups && ngrp * G + g < batch) IO::load(nxt, in + (ngrp * G + g) * N, t);
+        if constexpr (MODE == kWindowed) {
+            float w[E];
+            IO::load_real(w, win, t);
+#pragma unroll
+            for (int i = 0; i < E; ++i) d[i] = cscale(d[i], w[i]);
+        }
         Engine<N, E, T, INV>::run(d, lds, tw, t, Radices<Rs...>{});
-        if (active) store_pairs<N, E, T, INV, NT>(out + tr * N, d, t, scale);
+        if (active) IO::template store<INV, NT>(out + tr * N, d, t, scale);
         WGFFT_LDS_SYNC();  // the next group's first pass overwrites the exchange buffer
     }
 }
@@ -265,6 +335,7 @@ __device__ __forceinline__ void gp_compute(float2 (&v)[E], float2* __restrict__ 
     constexpr int THREADS = G * T;
     constexpr bool INV = MODE == kInverse;
     constexpr int SLOT = GlobalPassSlot<R, E, Rs...>::len;
+    static_assert(IoKind<E, Rs...>::PAIR, "global-pass sub-FFTs use the pair layout");
     const uint64_t NR = N / R;
     const uint64_t s = N / ((uint64_t)R * L);  // W_{RL} = W_N^s
     const bool first = (L == 1);

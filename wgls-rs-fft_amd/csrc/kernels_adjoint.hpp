@@ -51,6 +51,7 @@ __global__ void __launch_bounds__(N / E)
 k_adjoint(AdjointArgs a) {
     constexpr int T = N / E;
     constexpr int LEN = LdsLayout<N, E, Rs...>::len;
+    using IO = Io<N, E, Rs...>;
     __shared__ __align__(16) float2 lds[LEN];
     __shared__ float red[T];
 
@@ -69,8 +70,8 @@ k_adjoint(AdjointArgs a) {
             {
                 float2 xs[E];
                 float wv[E];
-                load_pairs<N, E, T>(xs, a.x + tr * N, t);
-                load_real_pairs<N, E, T>(wv, a.w, t);
+                IO::load(xs, a.x + tr * N, t);
+                IO::load_real(wv, a.w, t);
 #pragma unroll
                 for (int i = 0; i < E; ++i) {
                     if constexpr (KEEPX) xk[i] = xs[i];
@@ -80,10 +81,10 @@ k_adjoint(AdjointArgs a) {
             Engine<N, E, T, false>::run(d, lds, a.tw, t, Radices<Rs...>{});
             // keep the epilogue's loads out of the transform's register peak
             __builtin_amdgcn_sched_barrier(0);
-            if (a.y) store_pairs<N, E, T, false>(a.y + tr * N, d, t, 1.f);
+            if (a.y) IO::template store<false>(a.y + tr * N, d, t, 1.f);
             // gY = 2 m Y; loss += m |Y|^2
             float mv[E];
-            load_real_pairs<N, E, T>(mv, a.mask, t);
+            IO::load_real(mv, a.mask, t);
 #pragma unroll
             for (int i = 0; i < E; ++i) {
                 lacc += mv[i] * (d[i].x * d[i].x + d[i].y * d[i].y);
@@ -92,7 +93,7 @@ k_adjoint(AdjointArgs a) {
             WGFFT_LDS_SYNC();  // forward's last exchange read before the adjoint writes
             __builtin_amdgcn_sched_barrier(0);
         } else {
-            load_pairs<N, E, T>(d, a.gy + tr * N, t);
+            IO::load(d, a.gy + tr * N, t);
         }
         Engine<N, E, T, true>::run(d, lds, a.tw, t, Radices<Rs...>{});
         __builtin_amdgcn_sched_barrier(0);
@@ -102,22 +103,20 @@ k_adjoint(AdjointArgs a) {
 #pragma unroll
             for (int i = 0; i < E; ++i) xs[i] = xk[i];
         } else {
-            load_pairs<N, E, T>(xs, a.x + tr * N, t);
+            IO::load(xs, a.x + tr * N, t);
         }
         float wv[E];
-        load_real_pairs<N, E, T>(wv, a.w, t);
+        IO::load_real(wv, a.w, t);
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             acc[i] += d[i].x * xs[i].x + d[i].y * xs[i].y;
             d[i] = cscale(d[i], wv[i]);
         }
-        store_pairs<N, E, T, false>(a.gx + tr * N, d, t, 1.f);
+        IO::template store<false>(a.gx + tr * N, d, t, 1.f);
         WGFFT_LDS_SYNC();
     }
     const uint32_t t = threadIdx.x;
-    float2* pg = reinterpret_cast<float2*>(a.part + (uint64_t)blockIdx.x * a.ld);
-#pragma unroll
-    for (int q = 0; q < E / 2; ++q) pg[t + T * q] = make_float2(acc[2 * q], acc[2 * q + 1]);
+    IO::store_real(a.part + (uint64_t)blockIdx.x * a.ld, acc, t);
     if constexpr (FWD) {
         red[t] = lacc;
         __syncthreads();

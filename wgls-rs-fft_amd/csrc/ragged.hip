@@ -69,9 +69,9 @@ __device__ __forceinline__ void ragged_direct(const float2* __restrict__ in, flo
     const bool active = g < count;
     const uint64_t off = active ? offs[first + g] : 0;
     float2 d[E];
-    if (active) load_pairs<N, E, T>(d, in + off, t);
+    if (active) Io<N, E, Rs...>::load(d, in + off, t);
     Engine<N, E, T, INV>::run(d, lds_raw + g * LEN, tw_all + N, t, Radices<Rs...>{});
-    if (active) store_pairs<N, E, T, INV>(out + off, d, t, 1.0f / (float)N);
+    if (active) Io<N, E, Rs...>::template store<INV>(out + off, d, t, 1.0f / (float)N);
 }
 
 // Two-pass transform of one length-N signal (N = R1 * R2) by one workgroup:

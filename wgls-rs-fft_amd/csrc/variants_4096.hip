@@ -1,8 +1,12 @@
 // variants_4096.hip -- LDS-resident variants for N = 4096 (the headline length).
+// 16x16x16 with E = 16 uses the single I/O layout (8-byte lanes, two LDS
+// exchanges, ~half the registers of E = 32).
 #include "kernels.hpp"
 
 WGFFT_REGISTER_BATCHED(4096, 32, 1, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_BATCHED(4096, 16, 1, 16x16x16, 16, 16, 16);
 WGFFT_REGISTER_BATCHED(4096, 32, 2, 16x16x16, 16, 16, 16);
 WGFFT_REGISTER_BATCHED(4096, 16, 1, 8x8x8x8, 8, 8, 8, 8);
 WGFFT_REGISTER_BATCHED_NT(4096, 32, 1, 1, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_BATCHED_NT(4096, 16, 1, 1, 16x16x16, 16, 16, 16);
 WGFFT_REGISTER_BATCHED_NT(4096, 16, 1, 1, 8x8x8x8, 8, 8, 8, 8);

@@ -11,6 +11,7 @@ The same JSON line also carries, measured in the same run:
     the bandwidth of a plain device copy of the same 256 MiB;
   * fwd_adjoint: the fused forward + adjoint of L = sum m |FFT(w x)|^2 on the
     same shape and its time relative to the plain forward;
+  * other_shapes: the same 16.8 M samples as 256 x 65536 and 16384 x 1024;
   * mixed_batch: 20,000 transforms of lengths 2^k, k ~ U{6..16}, as one
     persistent launch, its hipGraph replay, and one launch per length;
   * large_n: N = 2^22 x 16 and N = 2^26 x 1 forward transforms;
@@ -97,6 +98,24 @@ def _ragged_log2(count, kmin, kmax, seed):
     wgfft._lib.load().wgfft_synth_uniform(raw.ctypes.data, count, seed)
     v = np.round((raw.astype(np.float64) + 1.0) * 8388608.0).astype(np.int64)
     return (kmin + v % (kmax - kmin + 1)).astype(np.uint32)
+
+
+def bench_shapes(torch, gpu, steps: int, tune: bool) -> dict:
+    """The same 16.8 M samples as 256 x 65536 and 16384 x 1024."""
+    out = {}
+    for n, batch in ((256, 65536), (16384, 1024)):
+        x = (torch.rand(2 * n * batch, device="cuda") * 2 - 1).view(torch.complex64).view(batch, n)
+        y = torch.empty_like(x)
+        if tune:
+            gpu.autotune(n, batch)
+        ms = timed(torch, lambda: gpu.exec_device(x, y), max(10, steps // 4), 5)
+        s = n * batch
+        gbs = 16.0 * s / (ms * 1e-3) / 1e9
+        out[f"{n}x{batch}"] = {"ms": round(ms, 4), "gsamples_s": round(s / (ms * 1e-3) / 1e9, 2),
+                               "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                               "plan": gpu.describe(n)}
+        del x, y
+    return out
 
 
 def bench_large(torch, gpu, steps: int, tune: bool) -> dict:
@@ -190,6 +209,7 @@ def main() -> int:
         del gx
         x_keep, y_keep = x, y
         del x, y
+        out["other_shapes"] = bench_shapes(torch, gpu, args.steps, not args.no_tune)
         out["mixed_batch"] = bench_mixed(torch, gpu, wgfft, args.steps)
         out["large_n"] = bench_large(torch, gpu, args.steps, not args.no_tune)
         del x_keep, y_keep

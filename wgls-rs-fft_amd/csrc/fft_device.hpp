@@ -172,34 +172,34 @@ __device__ __forceinline__ float2 twmul(float2 v, float2 w) {
     return INV ? cmulc(v, w) : cmul(v, w);
 }
 
-// v[m] *= W_{RL}^{m k} = W_N^{m k s} (conjugated for the inverse), with
-// tw[j] = W_N^j (forward sign) and base = k s.  The table supplies W^{base},
-// W^{2 base}, W^{4 base}, W^{8 base}; the other powers are products of at
-// most two table entries.  The powers are formed in forward-sign form and
+// Number of base powers w, w^2, w^4, w^8 a radix-R butterfly's twiddles are
+// formed from.
+constexpr int tw_bases(int R) { return R <= 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4; }
+
+// v[m] *= w^m (w^-m for the inverse), given wb[i] = w^(2^i) in forward-sign
+// form; the other powers are products of at most two of them.  The powers are
 // conjugated inside the multiply, so forward and inverse transforms of the
-// same length share them (they do not depend on the transform index and are
-// kept in registers across the transforms a workgroup handles).
+// same length share them.
 template <int R, bool INV>
-__device__ __forceinline__ void apply_twiddles(float2 (&v)[R], const float2* __restrict__ tw,
-                                               uint32_t base) {
-    float2 w1 = tw[base];
+__device__ __forceinline__ void apply_twiddle_powers(float2 (&v)[R], const float2* wb) {
+    const float2 w1 = wb[0];
     if constexpr (R == 2) {
         v[1] = twmul<INV>(v[1], w1);
     } else {
-        float2 w2 = tw[2 * base];
-        float2 w3 = cmul(w1, w2);
+        const float2 w2 = wb[1];
+        const float2 w3 = cmul(w1, w2);
         v[1] = twmul<INV>(v[1], w1);
         v[2] = twmul<INV>(v[2], w2);
         v[3] = twmul<INV>(v[3], w3);
         if constexpr (R >= 8) {
-            float2 w4 = tw[4 * base];
-            float2 w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
+            const float2 w4 = wb[2];
+            const float2 w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
             v[4] = twmul<INV>(v[4], w4);
             v[5] = twmul<INV>(v[5], w5);
             v[6] = twmul<INV>(v[6], w6);
             v[7] = twmul<INV>(v[7], w7);
             if constexpr (R == 16) {
-                float2 w8 = tw[8 * base];
+                const float2 w8 = wb[3];
                 v[8] = twmul<INV>(v[8], w8);
                 v[9] = twmul<INV>(v[9], cmul(w8, w1));
                 v[10] = twmul<INV>(v[10], cmul(w8, w2));
@@ -211,6 +211,18 @@ __device__ __forceinline__ void apply_twiddles(float2 (&v)[R], const float2* __r
             }
         }
     }
+}
+
+// v[m] *= W_{RL}^{m k} = W_N^{m k s} (conjugated for the inverse), with
+// tw[j] = W_N^j (forward sign) and base = k s: the base powers are read from
+// the table at base, 2 base, 4 base, 8 base.
+template <int R, bool INV>
+__device__ __forceinline__ void apply_twiddles(float2 (&v)[R], const float2* __restrict__ tw,
+                                               uint32_t base) {
+    float2 wb[tw_bases(R)];
+#pragma unroll
+    for (int i = 0; i < tw_bases(R); ++i) wb[i] = tw[base << i];
+    apply_twiddle_powers<R, INV>(v, wb);
 }
 
 // LDS index padding for one exchange: phys = i + ((i >> A) << B).
@@ -307,9 +319,11 @@ struct Engine {
     template <int R>
     static constexpr int pair_slots() { return E / (2 * R); }
 
-    template <bool PAIR, int L, int PA, int PB, bool FIRST, int R, int... Rest>
+    // HOIST: the twiddle base powers come from twr[] (loaded once by
+    // load_twiddles, TO = this pass's first entry) instead of the table.
+    template <bool PAIR, bool HOIST, int L, int PA, int PB, int TO, bool FIRST, int R, int... Rest>
     __device__ __forceinline__ static void passes(float2 (&d)[E], float2* __restrict__ lds,
-                                                  const float2* __restrict__ tw, uint32_t t) {
+                                                  const float2* __restrict__ tw, const float2* twr, uint32_t t) {
         constexpr bool LAST = sizeof...(Rest) == 0;
         constexpr bool IO = (FIRST || LAST) && PAIR;
         constexpr int NR = N / R;
@@ -352,9 +366,13 @@ struct Engine {
 #pragma unroll
                     for (int m = 0; m < R; ++m) v[m] = d[2 * (ip + m * pair_slots<R>()) + dl];
                     if constexpr (L > 1) {
-                        uint32_t b = 2 * t + dl + 2 * T * ip;
-                        uint32_t k = b & (L - 1);
-                        apply_twiddles<R, INV>(v, tw, k * (N / (R * L)));
+                        if constexpr (HOIST) {
+                            apply_twiddle_powers<R, INV>(v, twr + TO + (2 * ip + dl) * tw_bases(R));
+                        } else {
+                            uint32_t b = 2 * t + dl + 2 * T * ip;
+                            uint32_t k = b & (L - 1);
+                            apply_twiddles<R, INV>(v, tw, k * (N / (R * L)));
+                        }
                     }
                     Dft<R, INV>::run(v);
 #pragma unroll
@@ -368,9 +386,13 @@ struct Engine {
 #pragma unroll
                 for (int m = 0; m < R; ++m) v[m] = d[i * R + m];
                 if constexpr (L > 1) {
-                    uint32_t b = t + T * i;
-                    uint32_t k = b & (L - 1);
-                    apply_twiddles<R, INV>(v, tw, k * (N / (R * L)));
+                    if constexpr (HOIST) {
+                        apply_twiddle_powers<R, INV>(v, twr + TO + i * tw_bases(R));
+                    } else {
+                        uint32_t b = t + T * i;
+                        uint32_t k = b & (L - 1);
+                        apply_twiddles<R, INV>(v, tw, k * (N / (R * L)));
+                    }
                 }
                 Dft<R, INV>::run(v);
 #pragma unroll
@@ -403,16 +425,56 @@ struct Engine {
                 }
             }
             WGFFT_LDS_SYNC();
-            passes<PAIR, L * R, P::A, P::B, false, Rest...>(d, lds, tw, t);
+            passes<PAIR, HOIST, L * R, P::A, P::B, FIRST ? TO : TO + (E / R) * tw_bases(R), false, Rest...>(
+                d, lds, tw, twr, t);
         }
     }
 
+    template <bool PAIR, int L, int TO, bool FIRST, int R, int... Rest>
+    __device__ __forceinline__ static void tw_load(float2* twr, const float2* __restrict__ tw, uint32_t t) {
+        constexpr bool LAST = sizeof...(Rest) == 0;
+        constexpr bool IO = (FIRST || LAST) && PAIR;
+        if constexpr (!FIRST) {
+#pragma unroll
+            for (int j = 0; j < E / R; ++j) {
+                const uint32_t b = IO ? 2 * t + (j & 1) + 2 * T * (j >> 1) : t + T * j;
+                const uint32_t base = (b & (L - 1)) * (N / (R * L));
+#pragma unroll
+                for (int i = 0; i < tw_bases(R); ++i) twr[TO + j * tw_bases(R) + i] = tw[base << i];
+            }
+        }
+        if constexpr (!LAST)
+            tw_load<PAIR, L * R, FIRST ? TO : TO + (E / R) * tw_bases(R), false, Rest...>(twr, tw, t);
+    }
+
+    // Transform d[] (thread t's elements) reading twiddles from the table.
     template <int... Rs>
     __device__ __forceinline__ static void run(float2 (&d)[E], float2* __restrict__ lds,
                                                const float2* __restrict__ tw, uint32_t t,
                                                Radices<Rs...>) {
         static_assert(RadixInfo<Rs...>::product == N, "radices must multiply to N");
-        passes<IoKind<E, Rs...>::PAIR, 1, 30, 0, true, Rs...>(d, lds, tw, t);
+        passes<IoKind<E, Rs...>::PAIR, false, 1, 30, 0, 0, true, Rs...>(d, lds, tw, nullptr, t);
+    }
+
+    // Base twiddle powers thread t needs for a plan: tw_bases(R) for each of
+    // the E/R butterflies of every pass after the first.  They depend only on
+    // t, so a persistent kernel loads them once (load_twiddles) and keeps them
+    // in registers; no per-pass table load then has to wait behind the
+    // prefetched global loads of the next transform (vmcnt retires in order).
+    template <int R0, int... Rest>
+    static constexpr int hoisted_count(Radices<R0, Rest...>) {
+        return (0 + ... + ((E / Rest) * tw_bases(Rest)));
+    }
+    template <int... Rs>
+    __device__ __forceinline__ static void load_twiddles(float2* twr, const float2* __restrict__ tw, uint32_t t,
+                                                         Radices<Rs...>) {
+        tw_load<IoKind<E, Rs...>::PAIR, 1, 0, true, Rs...>(twr, tw, t);
+    }
+    template <int... Rs>
+    __device__ __forceinline__ static void run_hoisted(float2 (&d)[E], float2* __restrict__ lds, const float2* twr,
+                                                       uint32_t t, Radices<Rs...>) {
+        static_assert(RadixInfo<Rs...>::product == N, "radices must multiply to N");
+        passes<IoKind<E, Rs...>::PAIR, true, 1, 30, 0, 0, true, Rs...>(d, lds, nullptr, twr, t);
     }
 };
 

@@ -32,9 +32,15 @@ struct BatchedArgs {
     hipStream_t stream;
 };
 
+// BatchedVariant::flags
+constexpr int kFlagNtStore = 1;  // non-temporal output stores
+constexpr int kFlagTwRegs = 2;   // twiddle base powers held in registers across transforms
+// bits 4..7: minimum waves per SIMD the register allocation must allow (0: compiler's choice)
+constexpr int flag_min_waves(int fl) { return (fl >> 4) & 15; }
+
 struct BatchedVariant {
     int log2n;
-    int E, G, nt, threads;
+    int E, G, flags, threads;
     int lds_bytes;
     const char* plan;  // radix split, e.g. "16x16x16"
     hipError_t (*launch)(Mode, const BatchedArgs&);
@@ -187,13 +193,17 @@ struct Io {
 };
 
 // ---------------------------------------------------------------------------
-template <int N, int E, int G, bool NT, int MODE, int... Rs>
-__global__ void __launch_bounds__(G*(N / E))
+template <int N, int E, int G, int FL, int MODE, int... Rs>
+__global__ void __launch_bounds__(G*(N / E)) __attribute__((amdgpu_waves_per_eu(flag_min_waves(FL) > 0 ? flag_min_waves(FL) : 1)))
 k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
           const float* __restrict__ win, uint64_t batch, float scale) {
     constexpr int T = N / E;
     constexpr bool INV = MODE == kInverse;
+    constexpr bool NT = (FL & kFlagNtStore) != 0;
+    constexpr bool HOIST = (FL & kFlagTwRegs) != 0;
     constexpr int LEN = LdsLayout<N, E, Rs...>::len;
+    using EN = Engine<N, E, T, INV>;
+    constexpr int NTW = HOIST ? EN::hoisted_count(Radices<Rs...>{}) : 0;
     __shared__ __align__(16) float2 lds_all[G * LEN];
 
     const uint32_t tid = threadIdx.x;
@@ -202,6 +212,8 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
     const uint64_t ngroups = (batch + G - 1) / G;
 
     using IO = Io<N, E, Rs...>;
+    float2 twr[NTW > 0 ? NTW : 1];
+    if constexpr (HOIST) EN::load_twiddles(twr, tw, t, Radices<Rs...>{});
     uint64_t grp = blockIdx.x;
     float2 nxt[E];
     if (grp < ngroups && grp * G + g < batch) IO::load(nxt, in + (grp * G + g) * N, t);
@@ -212,21 +224,26 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
         float2 d[E];
 #pragma unroll
         for (int i = 0; i < E; ++i) d[i] = nxt[i];
+        // Loads that this transform waits on are issued before the prefetch
+        // (vmcnt retires in order).
+        float w[E];
+        if constexpr (MODE == kWindowed) IO::load_real(w, win, t);
         const uint64_t ngrp = grp + gridDim.x;
         if (ngrp < ngroups && ngrp * G + g < batch) IO::load(nxt, in + (ngrp * G + g) * N, t);
         if constexpr (MODE == kWindowed) {
-            float w[E];
-            IO::load_real(w, win, t);
 #pragma unroll
             for (int i = 0; i < E; ++i) d[i] = cscale(d[i], w[i]);
         }
-        Engine<N, E, T, INV>::run(d, lds, tw, t, Radices<Rs...>{});
+        if constexpr (HOIST)
+            EN::run_hoisted(d, lds, twr, t, Radices<Rs...>{});
+        else
+            EN::run(d, lds, tw, t, Radices<Rs...>{});
         if (active) IO::template store<INV, NT>(out + tr * N, d, t, scale);
         WGFFT_LDS_SYNC();  // the next group's first pass overwrites the exchange buffer
     }
 }
 
-template <int N, int E, int G, bool NT, int... Rs>
+template <int N, int E, int G, int FL, int... Rs>
 struct BatchedKernel {
     static constexpr int T = N / E;
     static constexpr int THREADS = G * T;
@@ -238,7 +255,7 @@ struct BatchedKernel {
         uint64_t grid = a.grid > 0 ? (uint64_t)a.grid : ngroups;
         if (grid > ngroups) grid = ngroups;
         if (grid == 0) return hipSuccess;
-        hipLaunchKernelGGL((k_batched<N, E, G, NT, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0,
+        hipLaunchKernelGGL((k_batched<N, E, G, FL, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0,
                            a.stream, a.in, a.out, a.tw, a.win, a.batch, a.scale);
         return hipGetLastError();
     }
@@ -252,7 +269,7 @@ struct BatchedKernel {
     template <int MODE>
     static int occ() {
         int blocks = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_batched<N, E, G, NT, MODE, Rs...>,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_batched<N, E, G, FL, MODE, Rs...>,
                                                          THREADS, 0) != hipSuccess)
             return 1;
         return blocks > 0 ? blocks : 1;
@@ -266,13 +283,14 @@ struct BatchedKernel {
     }
 };
 
-#define WGFFT_REGISTER_BATCHED_NT(N, E, G, NT, PLAN, ...)                                               \
-    static ::wgfft::Registrar _wgfft_reg_##N##_##E##_##G##_##NT##_##PLAN(::wgfft::BatchedVariant{      \
-        ::wgfft::ilog2(N), E, G, NT, ::wgfft::BatchedKernel<N, E, G, NT, __VA_ARGS__>::THREADS,          \
-        ::wgfft::BatchedKernel<N, E, G, NT, __VA_ARGS__>::LDS_BYTES, #PLAN,                             \
-        &::wgfft::BatchedKernel<N, E, G, NT, __VA_ARGS__>::launch,                                      \
-        &::wgfft::BatchedKernel<N, E, G, NT, __VA_ARGS__>::blocks_per_cu})
-#define WGFFT_REGISTER_BATCHED(N, E, G, PLAN, ...) WGFFT_REGISTER_BATCHED_NT(N, E, G, 0, PLAN, __VA_ARGS__)
+// FL: kFlag* bits.
+#define WGFFT_REGISTER_BATCHED_F(N, E, G, FL, PLAN, ...)                                                \
+    static ::wgfft::Registrar _wgfft_reg_##N##_##E##_##G##_##FL##_##PLAN(::wgfft::BatchedVariant{      \
+        ::wgfft::ilog2(N), E, G, FL, ::wgfft::BatchedKernel<N, E, G, FL, __VA_ARGS__>::THREADS,          \
+        ::wgfft::BatchedKernel<N, E, G, FL, __VA_ARGS__>::LDS_BYTES, #PLAN,                             \
+        &::wgfft::BatchedKernel<N, E, G, FL, __VA_ARGS__>::launch,                                      \
+        &::wgfft::BatchedKernel<N, E, G, FL, __VA_ARGS__>::blocks_per_cu})
+#define WGFFT_REGISTER_BATCHED(N, E, G, PLAN, ...) WGFFT_REGISTER_BATCHED_F(N, E, G, 0, PLAN, __VA_ARGS__)
 
 // LDS slot (complex elements) per butterfly of a global pass: the larger of
 // the sub-FFT's exchange footprint and the staging layout (R + R/32), kept
@@ -285,18 +303,24 @@ struct GlobalPassSlot {
     static constexpr int len = m + ((4 - m % 32) + 32) % 32;
 };
 
-// Twiddle sources for W_N^j (forward sign).
+// Twiddle sources for W_N^j (forward sign).  raw(j) issues the table loads;
+// value() combines them, so the loads of the next group can be issued ahead
+// of its data prefetch and combined only when the group is transformed.
 struct TwTwoLevel {
     const float2* hi;
     const float2* lo;
     uint32_t h;
-    __device__ __forceinline__ float2 operator()(uint64_t j) const {
-        return cmul(hi[j >> h], lo[j & ((1ull << h) - 1)]);
-    }
+    struct raw_t {
+        float2 hi, lo;
+    };
+    __device__ __forceinline__ raw_t raw(uint64_t j) const { return {hi[j >> h], lo[j & ((1ull << h) - 1)]}; }
+    __device__ __forceinline__ static float2 value(const raw_t& r) { return cmul(r.hi, r.lo); }
 };
 struct TwOneLevel {
     const float2* t;
-    __device__ __forceinline__ float2 operator()(uint64_t j) const { return t[j]; }
+    using raw_t = float2;
+    __device__ __forceinline__ raw_t raw(uint64_t j) const { return t[j]; }
+    __device__ __forceinline__ static float2 value(const raw_t& r) { return r; }
 };
 
 // One group of one multi-pass Stockham pass over a length-N transform with
@@ -305,13 +329,42 @@ struct TwOneLevel {
 // elements); when L == 1 each butterfly's R outputs are contiguous and are
 // stored straight from registers.  src/dst point at the transform.
 //
-// Split in two so callers can software-pipeline: gp_load issues the 16-byte
-// loads of the R x G tile into E registers per thread (lane i of step q holds
-// rows m = (tid + THREADS q) / (G/2), butterfly pair (tid + THREADS q) mod
-// (G/2)); gp_compute applies the pass twiddles (or the window), stages the
-// tile through LDS into the sub-FFT's pair layout, runs it and stores.  The
-// caller provides G * SLOT complex elements of LDS and the LDS barrier before
-// the next gp_compute.
+// Split in three so callers can software-pipeline: gp_load_side issues the
+// loads the group's twiddles (or window) come from, gp_load issues the
+// 16-byte loads of the R x G tile into E registers per thread, gp_compute
+// applies the pass twiddles (or the window), stages the tile through LDS into
+// the sub-FFT's pair layout, runs it and stores.  Lane i of step q holds rows
+// m = m0 + 2T q (m0 = tid / (G/2)) of butterfly pair gp = tid mod (G/2), so
+// its twiddles W_N^{m k s} (k = k0, k0 + 1) are W^{m0 k s} (W^{2T k s})^q: four
+// table values per lane and group, advanced by one complex multiply per
+// step.  The caller provides G * SLOT complex elements of LDS and the LDS
+// barrier before the next gp_compute.
+template <class TW, int E, int MODE>
+struct GpSide {
+    typename TW::raw_t a, da, b, db;  // W^{m0 k0 s}, W^{2T k0 s}, W^{m0 (k0+1) s}, W^{2T (k0+1) s}
+    float2 w[MODE == kWindowed ? E / 2 : 1];  // window at the lane's element pairs (first pass)
+};
+
+template <int R, int E, int G, int MODE, class TW>
+__device__ __forceinline__ void gp_load_side(GpSide<TW, E, MODE>& sd, const TW& twN, const float* __restrict__ win,
+                                             uint64_t N, uint64_t L, uint64_t b0, uint32_t tid) {
+    constexpr int T = R / E;
+    const uint32_t gp = tid % (G / 2), m0 = tid / (G / 2);
+    if (L > 1) {
+        const uint64_t s = N / ((uint64_t)R * L);
+        const uint64_t k0 = (b0 + 2 * gp) & (L - 1);
+        sd.a = twN.raw((m0 * k0 * s) & (N - 1));
+        sd.da = twN.raw((2 * T * k0 * s) & (N - 1));
+        sd.b = twN.raw((m0 * (k0 + 1) * s) & (N - 1));
+        sd.db = twN.raw((2 * T * (k0 + 1) * s) & (N - 1));
+    } else if constexpr (MODE == kWindowed) {
+        const uint64_t NR = N / R;
+#pragma unroll
+        for (int q = 0; q < E / 2; ++q)
+            sd.w[q] = *reinterpret_cast<const float2*>(win + b0 + 2 * gp + (uint64_t)(m0 + 2 * T * q) * NR);
+    }
+}
+
 template <int R, int E, int G>
 __device__ __forceinline__ void gp_load(float2 (&v)[E], const float2* __restrict__ src, uint64_t NR, uint64_t b0,
                                         uint32_t tid) {
@@ -328,18 +381,23 @@ __device__ __forceinline__ void gp_load(float2 (&v)[E], const float2* __restrict
 }
 
 template <int R, int E, int G, int MODE, class TW, int... Rs>
-__device__ __forceinline__ void gp_compute(float2 (&v)[E], float2* __restrict__ dst, const float2* __restrict__ tw_r,
-                                           const TW& twN, const float* __restrict__ win, uint64_t N, uint64_t L,
-                                           uint64_t b0, float sc, float2* __restrict__ lds_all, uint32_t tid) {
+__device__ __forceinline__ void gp_compute(float2 (&v)[E], const GpSide<TW, E, MODE>& sd, float2* __restrict__ dst,
+                                           const float2* __restrict__ tw_r, uint64_t N, uint64_t L, uint64_t b0,
+                                           float sc, float2* __restrict__ lds_all, uint32_t tid) {
     constexpr int T = R / E;
     constexpr int THREADS = G * T;
     constexpr bool INV = MODE == kInverse;
     constexpr int SLOT = GlobalPassSlot<R, E, Rs...>::len;
     static_assert(IoKind<E, Rs...>::PAIR, "global-pass sub-FFTs use the pair layout");
-    const uint64_t NR = N / R;
-    const uint64_t s = N / ((uint64_t)R * L);  // W_{RL} = W_N^s
     const bool first = (L == 1);
 
+    float2 ta = make_float2(1.0f, 0.0f), tb = ta, da = ta, db = ta;
+    if (!first) {
+        ta = TW::value(sd.a);
+        da = TW::value(sd.da);
+        tb = TW::value(sd.b);
+        db = TW::value(sd.db);
+    }
 #pragma unroll
     for (int q = 0; q < E / 2; ++q) {
         const int it = tid + THREADS * q;
@@ -348,21 +406,21 @@ __device__ __forceinline__ void gp_compute(float2 (&v)[E], float2* __restrict__ 
         float2 a = v[2 * q], b = v[2 * q + 1];
         if constexpr (MODE == kWindowed) {
             if (first) {
-                const uint64_t e = b0 + 2 * gp + (uint64_t)m * NR;
-                a = cscale(a, win[e]);
-                b = cscale(b, win[e + 1]);
+                a = cscale(a, sd.w[q].x);
+                b = cscale(b, sd.w[q].y);
             }
         }
         if (!first) {
-            const uint64_t k0 = (b0 + 2 * gp) & (L - 1);
-            a = twmul<INV>(a, twN(((uint64_t)m * k0 * s) & (N - 1)));
-            b = twmul<INV>(b, twN(((uint64_t)m * (k0 + 1) * s) & (N - 1)));
+            a = twmul<INV>(a, ta);
+            b = twmul<INV>(b, tb);
+            if (q + 1 < E / 2) {
+                ta = cmul(ta, da);
+                tb = cmul(tb, db);
+            }
         }
         const uint32_t pm = pad_idx<5, 0>(m);
         lds_all[(2 * gp) * SLOT + pm] = a;
         lds_all[(2 * gp + 1) * SLOT + pm] = b;
-        // bound the twiddle-table loads in flight (16 registers per 4 steps)
-        if ((q & 3) == 3) __builtin_amdgcn_sched_barrier(0);
     }
     WGFFT_LDS_SYNC();
 
@@ -427,29 +485,37 @@ k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const flo
     const uint64_t lid0 = (pid & 7) * (nb >> 3) + (pid >> 3);
     const bool last = (L * R == N);
     const TwTwoLevel twN{tw_hi, tw_lo, h};
+    using Side = GpSide<TwTwoLevel, E, MODE>;
     float2 cur[E];
+    Side sd;
     if (PF && lid0 < total) {
         const uint64_t tr = lid0 / groups_per_tr, b0 = (lid0 % groups_per_tr) * G;
+        gp_load_side<R, E, G, MODE>(sd, twN, win, N, L, b0, threadIdx.x);
         gp_load<R, E, G>(cur, in + tr * N, NR, b0, threadIdx.x);
     }
     for (uint64_t lid = lid0; lid < total; lid += nb) {
         const uint64_t tr = lid / groups_per_tr, b0 = (lid % groups_per_tr) * G;
         const uint32_t tid = launder_lane(threadIdx.x);
         if constexpr (PF) {
-            // next group's tile in flight while this one is transformed
+            // next group's side values and tile in flight while this one is
+            // transformed; they are issued after everything this group waits on
             float2 nxt[E];
+            Side nsd;
             const uint64_t nl = lid + nb;
             if (nl < total) {
                 const uint64_t ntr = nl / groups_per_tr, nb0 = (nl % groups_per_tr) * G;
+                gp_load_side<R, E, G, MODE>(nsd, twN, win, N, L, nb0, tid);
                 gp_load<R, E, G>(nxt, in + ntr * N, NR, nb0, tid);
             }
-            gp_compute<R, E, G, MODE, TwTwoLevel, Rs...>(cur, out + tr * N, tw_r, twN, win, N, L, b0,
+            gp_compute<R, E, G, MODE, TwTwoLevel, Rs...>(cur, sd, out + tr * N, tw_r, N, L, b0,
                                                          last ? scale : 1.0f, lds_all, tid);
 #pragma unroll
             for (int i = 0; i < E; ++i) cur[i] = nxt[i];
+            sd = nsd;
         } else {
+            gp_load_side<R, E, G, MODE>(sd, twN, win, N, L, b0, tid);
             gp_load<R, E, G>(cur, in + tr * N, NR, b0, tid);
-            gp_compute<R, E, G, MODE, TwTwoLevel, Rs...>(cur, out + tr * N, tw_r, twN, win, N, L, b0,
+            gp_compute<R, E, G, MODE, TwTwoLevel, Rs...>(cur, sd, out + tr * N, tw_r, N, L, b0,
                                                          last ? scale : 1.0f, lds_all, tid);
         }
         WGFFT_LDS_SYNC();

@@ -93,43 +93,52 @@ struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV, PF
                                                float2* __restrict__ lds, uint32_t tid) {
         constexpr int M = INV ? kInverse : kForward;
         const TwOneLevel twN{tw_all + N};
+        using Side = GpSide<TwOneLevel, E1, M>;
         static_assert(E1 == E2, "both passes share the prefetch registers");
-        // pass 1 (L = 1): src -> scratch
+        // pass 1 (L = 1): src -> scratch (no twiddles, so no side loads)
         float2 cur[E1];
+        Side sd;
         if (PF) gp_load<R1, E1, G1>(cur, src, N / R1, 0, tid);
 #pragma unroll 1
         for (uint64_t b0 = 0; b0 < N / R1; b0 += G1) {
             if constexpr (PF) {
                 float2 nxt[E1];
                 if (b0 + G1 < N / R1) gp_load<R1, E1, G1>(nxt, src, N / R1, b0 + G1, tid);
-                gp_compute<R1, E1, G1, M, TwOneLevel, Rs1...>(cur, scratch, tw_all + R1, twN, nullptr, N, 1, b0,
-                                                              1.0f, lds, tid);
+                gp_compute<R1, E1, G1, M, TwOneLevel, Rs1...>(cur, sd, scratch, tw_all + R1, N, 1, b0, 1.0f, lds,
+                                                              tid);
 #pragma unroll
                 for (int i = 0; i < E1; ++i) cur[i] = nxt[i];
             } else {
                 gp_load<R1, E1, G1>(cur, src, N / R1, b0, tid);
-                gp_compute<R1, E1, G1, M, TwOneLevel, Rs1...>(cur, scratch, tw_all + R1, twN, nullptr, N, 1, b0,
-                                                              1.0f, lds, tid);
+                gp_compute<R1, E1, G1, M, TwOneLevel, Rs1...>(cur, sd, scratch, tw_all + R1, N, 1, b0, 1.0f, lds,
+                                                              tid);
             }
             WGFFT_LDS_SYNC();
         }
         __syncthreads();  // scratch stores of the whole workgroup visible to its loads
         // pass 2 (L = R1): scratch -> dst
         const float sc = INV ? 1.0f / (float)N : 1.0f;
-        if (PF) gp_load<R2, E2, G2>(cur, scratch, N / R2, 0, tid);
+        if (PF) {
+            gp_load_side<R2, E2, G2, M>(sd, twN, nullptr, N, R1, 0, tid);
+            gp_load<R2, E2, G2>(cur, scratch, N / R2, 0, tid);
+        }
 #pragma unroll 1
         for (uint64_t b0 = 0; b0 < N / R2; b0 += G2) {
             if constexpr (PF) {
                 float2 nxt[E2];
-                if (b0 + G2 < N / R2) gp_load<R2, E2, G2>(nxt, scratch, N / R2, b0 + G2, tid);
-                gp_compute<R2, E2, G2, M, TwOneLevel, Rs2...>(cur, dst, tw_all + R2, twN, nullptr, N, R1, b0, sc,
-                                                              lds, tid);
+                Side nsd;
+                if (b0 + G2 < N / R2) {
+                    gp_load_side<R2, E2, G2, M>(nsd, twN, nullptr, N, R1, b0 + G2, tid);
+                    gp_load<R2, E2, G2>(nxt, scratch, N / R2, b0 + G2, tid);
+                }
+                gp_compute<R2, E2, G2, M, TwOneLevel, Rs2...>(cur, sd, dst, tw_all + R2, N, R1, b0, sc, lds, tid);
 #pragma unroll
                 for (int i = 0; i < E2; ++i) cur[i] = nxt[i];
+                sd = nsd;
             } else {
+                gp_load_side<R2, E2, G2, M>(sd, twN, nullptr, N, R1, b0, tid);
                 gp_load<R2, E2, G2>(cur, scratch, N / R2, b0, tid);
-                gp_compute<R2, E2, G2, M, TwOneLevel, Rs2...>(cur, dst, tw_all + R2, twN, nullptr, N, R1, b0, sc,
-                                                              lds, tid);
+                gp_compute<R2, E2, G2, M, TwOneLevel, Rs2...>(cur, sd, dst, tw_all + R2, N, R1, b0, sc, lds, tid);
             }
             WGFFT_LDS_SYNC();
         }

@@ -131,9 +131,9 @@ const BatchedVariant* default_variant(int log2n) {
     return nullptr;
 }
 
-const BatchedVariant* find_variant(int log2n, const char* plan, int E, int G, int nt) {
+const BatchedVariant* find_variant(int log2n, const char* plan, int E, int G, int flags) {
     for (const auto& v : batched_registry())
-        if (v.log2n == log2n && v.E == E && v.G == G && v.nt == nt && std::strcmp(v.plan, plan) == 0) return &v;
+        if (v.log2n == log2n && v.E == E && v.G == G && v.flags == flags && std::strcmp(v.plan, plan) == 0) return &v;
     return nullptr;
 }
 

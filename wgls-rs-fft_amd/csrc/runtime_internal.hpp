@@ -54,7 +54,7 @@ int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x
 const char* adjoint_describe(const AdjointVariant* v, char* buf, uint64_t len);
 void synth_uniform(float* out, uint64_t nfloats, uint64_t seed);
 const BatchedVariant* default_variant(int log2n);
-const BatchedVariant* find_variant(int log2n, const char* plan, int E, int G, int nt);
+const BatchedVariant* find_variant(int log2n, const char* plan, int E, int G, int flags);
 std::vector<const GlobalPassVariant*> global_pass_candidates(int log2r, uint64_t L, uint64_t nr);
 
 struct Handle {

@@ -5,7 +5,7 @@
 // elements per thread, transforms per workgroup) on a resident scratch batch
 // and keeps the fastest for the handle.  With WGFFT_TUNE_FILE set, choices
 // are read at handle creation and appended when a tuning run finishes
-// (lines: "log2n plan E G nt").
+// (lines: "log2n plan E G flags").
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -59,10 +59,10 @@ void load_tune_file(Handle* h) {
     if (!path) return;
     FILE* f = std::fopen(path, "r");
     if (!f) return;
-    int log2n, E, G, nt;
+    int log2n, E, G, flags;
     char plan[64];
-    while (std::fscanf(f, "%d %63s %d %d %d", &log2n, plan, &E, &G, &nt) == 5) {
-        const BatchedVariant* v = find_variant(log2n, plan, E, G, nt);
+    while (std::fscanf(f, "%d %63s %d %d %d", &log2n, plan, &E, &G, &flags) == 5) {
+        const BatchedVariant* v = find_variant(log2n, plan, E, G, flags);
         if (v) h->tuned[log2n] = v;
     }
     std::fclose(f);
@@ -70,8 +70,11 @@ void load_tune_file(Handle* h) {
 
 static void describe(const BatchedVariant* v, char* desc, uint64_t len) {
     if (!desc || !len) return;
-    std::snprintf(desc, len, "lds N=%d plan=%s E=%d G=%d%s threads=%d lds=%dB", 1 << v->log2n, v->plan, v->E, v->G,
-                  v->nt ? " nt-store" : "", v->threads, v->lds_bytes);
+    char waves[24] = "";
+    if (flag_min_waves(v->flags)) std::snprintf(waves, sizeof waves, " waves>=%d", flag_min_waves(v->flags));
+    std::snprintf(desc, len, "lds N=%d plan=%s E=%d G=%d%s%s%s threads=%d lds=%dB", 1 << v->log2n, v->plan, v->E,
+                  v->G, (v->flags & kFlagNtStore) ? " nt-store" : "", (v->flags & kFlagTwRegs) ? " tw-regs" : "",
+                  waves, v->threads, v->lds_bytes);
 }
 
 static std::string describe_multipass(const Plan* p) {
@@ -271,7 +274,7 @@ extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc
         }
         if (const char* path = std::getenv("WGFFT_TUNE_FILE")) {
             if (FILE* f = std::fopen(path, "a")) {
-                std::fprintf(f, "%d %s %d %d %d\n", best->log2n, best->plan, best->E, best->G, best->nt);
+                std::fprintf(f, "%d %s %d %d %d\n", best->log2n, best->plan, best->E, best->G, best->flags);
                 std::fclose(f);
             }
         }

@@ -162,6 +162,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--headline-only", action="store_true", help="only the N=4096x4096 forward (profiling)")
+    ap.add_argument("--only", choices=("shapes", "mixed", "large"),
+                    help="profiling: skip the headline and run one secondary section")
     ap.add_argument("--no-tune", action="store_true", help="use the default kernel variants")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -174,6 +176,13 @@ def main() -> int:
     import wgfft
 
     gpu = wgfft.GpuFft.new()
+    if args.only:
+        section = {"shapes": lambda: bench_shapes(torch, gpu, args.steps, not args.no_tune),
+                   "mixed": lambda: bench_mixed(torch, gpu, wgfft, args.steps),
+                   "large": lambda: bench_large(torch, gpu, args.steps, not args.no_tune)}[args.only]
+        print(json.dumps({args.only: section()}))
+        gpu.close()
+        return 0
     tuned = None if args.no_tune else gpu.autotune(N, BATCH)
     x = torch.from_numpy(wgfft.synth_uniform((BATCH, N), 0x5EED0002)).cuda()
     y = torch.empty_like(x)
@@ -201,6 +210,8 @@ def main() -> int:
         "config": {"workload": "batched forward FFT, N=4096, batch 4096", "n": N, "batch": BATCH,
                    "samples": samples, "plan": gpu.describe(N), "autotuned": tuned},
     }
+    # HBM bytes per launch of the headline kernel from the committed
+    # FETCH_SIZE / WRITE_SIZE runs (scaled by a copy kernel of known size)
     traffic = None
     prof = os.path.join(ROOT, "profiles", "pmc_fft4096x4096.json")
     if os.path.exists(prof):

@@ -147,6 +147,16 @@ int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc, uint64_t 
 /* Description of the variant the handle currently uses for length n. */
 int wgfft_plan_describe(wgfft_t* h, uint64_t n, char* desc, uint64_t desc_len);
 
+/* Number of compiled single-kernel (LDS-resident) variants for length n;
+ * 0 for lengths served by the register kernel or by global passes. */
+int wgfft_variant_count(uint64_t n);
+
+/* Make compiled variant `index` (0 <= index < wgfft_variant_count(n)) the
+ * handle's choice for length n, as a tuning run would; writes its
+ * description into desc (if non-NULL).  Lets callers pin a variant and tests
+ * check every one. */
+int wgfft_select_variant(wgfft_t* h, uint64_t n, int index, char* desc, uint64_t desc_len);
+
 /* Fill nfloats floats with i.i.d. U[-1,1) = (u32 >> 8) * 2^-23 - 1 drawn
  * from PCG32 seeded through SplitMix64(seed) (input synthesis, host). */
 void wgfft_synth_uniform(float* out, uint64_t nfloats, uint64_t seed);

@@ -74,6 +74,17 @@ def test_validation_contract():
     assert e.value.code == wgfft.WGFFT_ERR_RAGGED_BATCH
 
 
+def test_variant_registry_counts():
+    # host-side registry queries need no device
+    for log2n in range(5, 15):
+        assert wgfft.GpuFft.variant_count(1 << log2n) >= 1
+    assert wgfft.GpuFft.variant_count(16) == 0  # register kernel
+    assert wgfft.GpuFft.variant_count(1 << 20) == 0  # global passes
+    assert wgfft.GpuFft.variant_count(12) == 0
+    L = _lib.load()
+    assert L.wgfft_select_variant(None, 4096, 0, None, 0) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
+
+
 def test_synth_uniform_matches_oracle_stream():
     for seed in (0, 1, 0x5EED0001):
         assert np.array_equal(wgfft.synth_uniform((5, 33), seed), oracle.synth_uniform((5, 33), seed))

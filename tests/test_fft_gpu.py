@@ -56,11 +56,53 @@ def test_all_lengths_vs_fp64_and_reference(gpu, log2n):
     assert np.array_equal(single[0], y[-1])  # batch output == single output
 
 
+@pytest.mark.parametrize("log2n", list(range(5, 15)))
+def test_every_compiled_variant(log2n):
+    """Each compiled single-kernel variant (radix split, E, G, store and
+    twiddle-source flags), pinned in turn, matches FP64 forward and inverse
+    on a batch that leaves a partial group."""
+    n = 1 << log2n
+    count = wgfft.GpuFft.variant_count(n)
+    assert count >= 1
+    x = oracle.synth_uniform((max(3, min(45, (1 << 18) // n)), n), 0x5EED0150 + log2n)
+    for i in range(count):
+        g = wgfft.GpuFft.new()
+        desc = g.select_variant(n, i)
+        assert g.describe(n) == desc
+        try:
+            check_fwd_inv(g, x)
+        except AssertionError as e:
+            raise AssertionError(f"variant {i}: {desc}: {e}") from None
+        finally:
+            g.close()
+
+
+def test_select_variant_errors(gpu):
+    with pytest.raises(wgfft.WgfftError):
+        gpu.select_variant(4096, wgfft.GpuFft.variant_count(4096))
+    with pytest.raises(wgfft.WgfftError):
+        gpu.select_variant(1 << 20, 0)  # served by global passes
+    assert wgfft.GpuFft.variant_count(3) == 0
+
+
 @pytest.mark.parametrize("log2n,batch", [(17, 3), (18, 2), (20, 2), (22, 2)])
 def test_multipass_lengths(gpu, log2n, batch):
     n = 1 << log2n
     x = oracle.synth_uniform((batch, n), 0x5EED0200 + log2n)
     check_fwd_inv(gpu, x)
+
+
+@pytest.mark.parametrize("log2n", [17, 20, 23])
+def test_multipass_autotuned(log2n):
+    """The global-pass combination the autotuner keeps matches FP64."""
+    n = 1 << log2n
+    g = wgfft.GpuFft.new()
+    try:
+        desc = g.autotune(n, 2)
+        assert desc.startswith("multipass"), desc
+        check_fwd_inv(g, oracle.synth_uniform((2, n), 0x5EED0250 + log2n))
+    finally:
+        g.close()
 
 
 @pytest.mark.parametrize("n,batch", [(4096, 1), (4096, 2), (4096, 3), (1024, 9), (256, 33), (64, 129),

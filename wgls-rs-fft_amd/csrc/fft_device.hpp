@@ -309,6 +309,61 @@ struct LdsLayout {
     static constexpr int len = (raw + 1) & ~1;
 };
 
+// Sources of the twiddle base powers w^(2^i), i < tw_bases(R), of butterfly
+// slot j of a pass (its twiddle index base = k N / (R L); TO = the pass's
+// first entry in a register-held set).
+//  TwTable:   read from the W_N table in global memory (tw[base << i]).
+//  TwHoisted: register copies loaded once per thread by load_twiddles.
+//  TwLds:     W_N^j = hi[j >> h] * lo[j & (2^h - 1)] from two short tables
+//             in LDS (2 sqrt(N) entries); LDS reads retire on lgkmcnt, so
+//             they never wait behind prefetched global loads.
+struct TwTable {
+    const float2* __restrict__ tw;
+    template <int R, int TO>
+    __device__ __forceinline__ void bases(float2* wb, int, uint32_t base) const {
+#pragma unroll
+        for (int i = 0; i < tw_bases(R); ++i) wb[i] = tw[base << i];
+    }
+};
+struct TwHoisted {
+    const float2* twr;
+    template <int R, int TO>
+    __device__ __forceinline__ void bases(float2* wb, int j, uint32_t) const {
+#pragma unroll
+        for (int i = 0; i < tw_bases(R); ++i) wb[i] = twr[TO + j * tw_bases(R) + i];
+    }
+};
+struct TwLds {
+    const float2* lo;
+    const float2* hi;
+    uint32_t h;
+    template <int R, int TO>
+    __device__ __forceinline__ void bases(float2* wb, int, uint32_t base) const {
+#pragma unroll
+        for (int i = 0; i < tw_bases(R); ++i) {
+            const uint32_t jj = base << i;
+            wb[i] = cmul(hi[jj >> h], lo[jj & ((1u << h) - 1)]);
+        }
+    }
+};
+
+// Split of a length-N W_N table into TwLds's two LDS tables: lo has 2^h
+// entries, hi has N >> h (only indices below N/2 are ever used).
+constexpr int twlds_h(int log2n) { return (log2n + 1) / 2; }
+constexpr int twlds_len(int log2n) { return (1 << twlds_h(log2n)) + ((1 << log2n) >> twlds_h(log2n)); }
+
+// Fill the TwLds tables at lds_tw (twlds_len(log2 N) entries) from the W_N
+// table; the caller synchronises before use.
+template <int N>
+__device__ __forceinline__ TwLds twlds_fill(float2* lds_tw, const float2* __restrict__ tw, uint32_t tid,
+                                            uint32_t nthreads) {
+    constexpr int H = twlds_h(ilog2(N));
+    constexpr int NLO = 1 << H, NHI = N >> H;
+    for (uint32_t i = tid; i < NLO + NHI; i += nthreads)
+        lds_tw[i] = i < NLO ? tw[i] : tw[(i - NLO) << H];
+    return TwLds{lds_tw, lds_tw + NLO, (uint32_t)H};
+}
+
 // Stockham pass engine.  d[] holds the thread's E elements; on entry (first
 // pass) and on exit (last pass) in the plan's global I/O layout (IoKind).
 template <int N, int E, int T, bool INV>
@@ -319,11 +374,11 @@ struct Engine {
     template <int R>
     static constexpr int pair_slots() { return E / (2 * R); }
 
-    // HOIST: the twiddle base powers come from twr[] (loaded once by
-    // load_twiddles, TO = this pass's first entry) instead of the table.
-    template <bool PAIR, bool HOIST, int L, int PA, int PB, int TO, bool FIRST, int R, int... Rest>
-    __device__ __forceinline__ static void passes(float2 (&d)[E], float2* __restrict__ lds,
-                                                  const float2* __restrict__ tw, const float2* twr, uint32_t t) {
+    // TS: twiddle base source (TwTable, TwHoisted, TwLds); TO = this pass's
+    // first entry in a register-held set.
+    template <bool PAIR, class TS, int L, int PA, int PB, int TO, bool FIRST, int R, int... Rest>
+    __device__ __forceinline__ static void passes(float2 (&d)[E], float2* __restrict__ lds, const TS& ts,
+                                                  uint32_t t) {
         constexpr bool LAST = sizeof...(Rest) == 0;
         constexpr bool IO = (FIRST || LAST) && PAIR;
         constexpr int NR = N / R;
@@ -366,13 +421,10 @@ struct Engine {
 #pragma unroll
                     for (int m = 0; m < R; ++m) v[m] = d[2 * (ip + m * pair_slots<R>()) + dl];
                     if constexpr (L > 1) {
-                        if constexpr (HOIST) {
-                            apply_twiddle_powers<R, INV>(v, twr + TO + (2 * ip + dl) * tw_bases(R));
-                        } else {
-                            uint32_t b = 2 * t + dl + 2 * T * ip;
-                            uint32_t k = b & (L - 1);
-                            apply_twiddles<R, INV>(v, tw, k * (N / (R * L)));
-                        }
+                        const uint32_t b = 2 * t + dl + 2 * T * ip;
+                        float2 wb[tw_bases(R)];
+                        ts.template bases<R, TO>(wb, 2 * ip + dl, (b & (L - 1)) * (N / (R * L)));
+                        apply_twiddle_powers<R, INV>(v, wb);
                     }
                     Dft<R, INV>::run(v);
 #pragma unroll
@@ -386,13 +438,10 @@ struct Engine {
 #pragma unroll
                 for (int m = 0; m < R; ++m) v[m] = d[i * R + m];
                 if constexpr (L > 1) {
-                    if constexpr (HOIST) {
-                        apply_twiddle_powers<R, INV>(v, twr + TO + i * tw_bases(R));
-                    } else {
-                        uint32_t b = t + T * i;
-                        uint32_t k = b & (L - 1);
-                        apply_twiddles<R, INV>(v, tw, k * (N / (R * L)));
-                    }
+                    const uint32_t b = t + T * i;
+                    float2 wb[tw_bases(R)];
+                    ts.template bases<R, TO>(wb, i, (b & (L - 1)) * (N / (R * L)));
+                    apply_twiddle_powers<R, INV>(v, wb);
                 }
                 Dft<R, INV>::run(v);
 #pragma unroll
@@ -425,8 +474,8 @@ struct Engine {
                 }
             }
             WGFFT_LDS_SYNC();
-            passes<PAIR, HOIST, L * R, P::A, P::B, FIRST ? TO : TO + (E / R) * tw_bases(R), false, Rest...>(
-                d, lds, tw, twr, t);
+            passes<PAIR, TS, L * R, P::A, P::B, FIRST ? TO : TO + (E / R) * tw_bases(R), false, Rest...>(d, lds,
+                                                                                                         ts, t);
         }
     }
 
@@ -453,7 +502,7 @@ struct Engine {
                                                const float2* __restrict__ tw, uint32_t t,
                                                Radices<Rs...>) {
         static_assert(RadixInfo<Rs...>::product == N, "radices must multiply to N");
-        passes<IoKind<E, Rs...>::PAIR, false, 1, 30, 0, 0, true, Rs...>(d, lds, tw, nullptr, t);
+        passes<IoKind<E, Rs...>::PAIR, TwTable, 1, 30, 0, 0, true, Rs...>(d, lds, TwTable{tw}, t);
     }
 
     // Base twiddle powers thread t needs for a plan: tw_bases(R) for each of
@@ -470,11 +519,12 @@ struct Engine {
                                                          Radices<Rs...>) {
         tw_load<IoKind<E, Rs...>::PAIR, 1, 0, true, Rs...>(twr, tw, t);
     }
-    template <int... Rs>
-    __device__ __forceinline__ static void run_hoisted(float2 (&d)[E], float2* __restrict__ lds, const float2* twr,
-                                                       uint32_t t, Radices<Rs...>) {
+    // Transform with any twiddle source (TwHoisted, TwLds, TwTable).
+    template <class TS, int... Rs>
+    __device__ __forceinline__ static void run_with(float2 (&d)[E], float2* __restrict__ lds, const TS& ts,
+                                                    uint32_t t, Radices<Rs...>) {
         static_assert(RadixInfo<Rs...>::product == N, "radices must multiply to N");
-        passes<IoKind<E, Rs...>::PAIR, true, 1, 30, 0, 0, true, Rs...>(d, lds, nullptr, twr, t);
+        passes<IoKind<E, Rs...>::PAIR, TS, 1, 30, 0, 0, true, Rs...>(d, lds, ts, t);
     }
 };
 

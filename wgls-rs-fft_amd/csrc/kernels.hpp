@@ -35,6 +35,7 @@ struct BatchedArgs {
 // BatchedVariant::flags
 constexpr int kFlagNtStore = 1;  // non-temporal output stores
 constexpr int kFlagTwRegs = 2;   // twiddle base powers held in registers across transforms
+constexpr int kFlagTwLds = 4;    // twiddle base powers from two short LDS tables
 // bits 4..7: minimum waves per SIMD the register allocation must allow (0: compiler's choice)
 constexpr int flag_min_waves(int fl) { return (fl >> 4) & 15; }
 
@@ -201,10 +202,13 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
     constexpr bool INV = MODE == kInverse;
     constexpr bool NT = (FL & kFlagNtStore) != 0;
     constexpr bool HOIST = (FL & kFlagTwRegs) != 0;
+    constexpr bool TWLDS = (FL & kFlagTwLds) != 0;
+    static_assert(!(HOIST && TWLDS), "one twiddle source per variant");
     constexpr int LEN = LdsLayout<N, E, Rs...>::len;
     using EN = Engine<N, E, T, INV>;
     constexpr int NTW = HOIST ? EN::hoisted_count(Radices<Rs...>{}) : 0;
     __shared__ __align__(16) float2 lds_all[G * LEN];
+    __shared__ __align__(16) float2 lds_tw[TWLDS ? twlds_len(ilog2(N)) : 1];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t g = tid / T, t = tid % T;
@@ -214,6 +218,11 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
     using IO = Io<N, E, Rs...>;
     float2 twr[NTW > 0 ? NTW : 1];
     if constexpr (HOIST) EN::load_twiddles(twr, tw, t, Radices<Rs...>{});
+    TwLds twl{};
+    if constexpr (TWLDS) {
+        twl = twlds_fill<N>(lds_tw, tw, tid, G * T);
+        __syncthreads();
+    }
     uint64_t grp = blockIdx.x;
     float2 nxt[E];
     if (grp < ngroups && grp * G + g < batch) IO::load(nxt, in + (grp * G + g) * N, t);
@@ -235,7 +244,9 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
             for (int i = 0; i < E; ++i) d[i] = cscale(d[i], w[i]);
         }
         if constexpr (HOIST)
-            EN::run_hoisted(d, lds, twr, t, Radices<Rs...>{});
+            EN::run_with(d, lds, TwHoisted{twr}, t, Radices<Rs...>{});
+        else if constexpr (TWLDS)
+            EN::run_with(d, lds, twl, t, Radices<Rs...>{});
         else
             EN::run(d, lds, tw, t, Radices<Rs...>{});
         if (active) IO::template store<INV, NT>(out + tr * N, d, t, scale);
@@ -247,7 +258,8 @@ template <int N, int E, int G, int FL, int... Rs>
 struct BatchedKernel {
     static constexpr int T = N / E;
     static constexpr int THREADS = G * T;
-    static constexpr int LDS_BYTES = G * LdsLayout<N, E, Rs...>::len * 8;
+    static constexpr int LDS_BYTES =
+        G * LdsLayout<N, E, Rs...>::len * 8 + ((FL & kFlagTwLds) ? twlds_len(ilog2(N)) * 8 : 0);
 
     template <int MODE>
     static hipError_t go(const BatchedArgs& a) {

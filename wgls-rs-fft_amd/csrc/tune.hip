@@ -54,6 +54,79 @@ void synth_uniform(float* out, uint64_t nfloats, uint64_t seed) {
     for (uint64_t i = 0; i < nfloats; ++i) out[i] = (float)((double)(r.next() >> 8) * (1.0 / 8388608.0) - 1.0);
 }
 
+// Uniform [-1, 1) noise for the tuning batch (an integer hash of the index;
+// zero-filled inputs draw less power and time optimistically).
+__global__ void k_fill_noise(float* __restrict__ x, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t v = (uint32_t)i * 0x9E3779B1u ^ (uint32_t)(i >> 32);
+        v ^= v >> 16;
+        v *= 0x85EBCA6Bu;
+        v ^= v >> 13;
+        v *= 0xC2B2AE35u;
+        v ^= v >> 16;
+        x[i] = (float)(v >> 8) * (1.0f / 8388608.0f) - 1.0f;
+    }
+}
+
+static int fill_noise(Handle* h, void* p, uint64_t bytes) {
+    const uint64_t n = bytes / sizeof(float);
+    hipLaunchKernelGGL(k_fill_noise, dim3(1024), dim3(256), 0, h->stream, (float*)p, n);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WGFFT_OK : hip_fail(e, "noise fill");
+}
+
+// Timer for candidate launches on the handle's stream.  The shader clock
+// ramps up over tens of milliseconds of continuous load, so settle() runs a
+// launch back to back before a sweep, and each candidate is timed as the best
+// of three bursts of at least ~2 ms.
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t s;
+    explicit Timer(hipStream_t st) : s(st) {
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+    }
+    ~Timer() {
+        if (a) hipEventDestroy(a);
+        if (b) hipEventDestroy(b);
+    }
+    template <class F>
+    int burst(F&& launch, int iters, float* ms) {
+        hipEventRecord(a, s);
+        for (int i = 0; i < iters; ++i) launch();
+        hipEventRecord(b, s);
+        const hipError_t e = hipEventSynchronize(b);
+        if (e != hipSuccess) return hip_fail(e, "autotune timing");
+        hipEventElapsedTime(ms, a, b);
+        return WGFFT_OK;
+    }
+    template <class F>
+    int settle(F&& launch, float target_ms = 40.f) {
+        float ms = 0.f, total = 0.f;
+        int iters = 4, rc;
+        while (total < target_ms) {
+            if ((rc = burst(launch, iters, &ms))) return rc;
+            total += ms;
+            if (ms < 2.f && iters < 4096) iters *= 2;
+        }
+        return WGFFT_OK;
+    }
+    // best per-launch time in ms
+    template <class F>
+    int time(F&& launch, float* best) {
+        float ms = 0.f;
+        int rc;
+        if ((rc = burst(launch, 2, &ms))) return rc;
+        const int iters = std::max(5, std::min(500, (int)(2.f / std::max(ms * 0.5f, 1e-3f))));
+        *best = 1e30f;
+        for (int round = 0; round < 3; ++round) {
+            if ((rc = burst(launch, iters, &ms))) return rc;
+            *best = std::min(*best, ms / iters);
+        }
+        return WGFFT_OK;
+    }
+};
+
 void load_tune_file(Handle* h) {
     const char* path = std::getenv("WGFFT_TUNE_FILE");
     if (!path) return;
@@ -73,7 +146,7 @@ static void describe(const BatchedVariant* v, char* desc, uint64_t len) {
     char waves[24] = "";
     if (flag_min_waves(v->flags)) std::snprintf(waves, sizeof waves, " waves>=%d", flag_min_waves(v->flags));
     std::snprintf(desc, len, "lds N=%d plan=%s E=%d G=%d%s%s%s threads=%d lds=%dB", 1 << v->log2n, v->plan, v->E,
-                  v->G, (v->flags & kFlagNtStore) ? " nt-store" : "", (v->flags & kFlagTwRegs) ? " tw-regs" : "",
+                  v->G, (v->flags & kFlagNtStore) ? " nt-store" : "", (v->flags & kFlagTwRegs) ? " tw-regs" : (v->flags & kFlagTwLds) ? " tw-lds" : "",
                   waves, v->threads, v->lds_bytes);
 }
 
@@ -94,7 +167,7 @@ static int tune_multipass(Handle* h, Plan* p, uint64_t batch, char* desc, uint64
     int rc;
     if ((rc = h->ensure(h->io_in, bytes)) || (rc = h->ensure(h->io_out, bytes))) return rc;
     hipError_t e;
-    if ((e = hipMemsetAsync(h->io_in.p, 0, bytes, h->stream)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+    if ((rc = fill_noise(h, h->io_in.p, bytes))) return rc;
     std::vector<std::vector<const GlobalPassVariant*>> cands;
     uint64_t L = 1;
     for (size_t i = 0; i < p->pass_log2r.size(); ++i) {
@@ -103,28 +176,16 @@ static int tune_multipass(Handle* h, Plan* p, uint64_t batch, char* desc, uint64
     }
     std::vector<size_t> idx(cands.size(), 0), best_idx = idx;
     float best_ms = 1e30f;
-    hipEvent_t a, b;
-    hipEventCreate(&a);
-    hipEventCreate(&b);
+    Timer tm(h->stream);
+    auto launch = [&] {
+        h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream);
+    };
+    for (size_t i = 0; i < cands.size(); ++i) p->pass_var[i] = cands[i][0];
+    if ((rc = tm.settle(launch))) return rc;
     for (;;) {
         for (size_t i = 0; i < cands.size(); ++i) p->pass_var[i] = cands[i][idx[i]];
-        h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream);
-        float ms = 1e30f;
-        for (int round = 0; round < 3; ++round) {
-            hipEventRecord(a, h->stream);
-            for (int it = 0; it < 3; ++it)
-                h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f,
-                       h->stream);
-            hipEventRecord(b, h->stream);
-            if ((e = hipEventSynchronize(b)) != hipSuccess) {
-                hipEventDestroy(a);
-                hipEventDestroy(b);
-                return hip_fail(e, "autotune timing");
-            }
-            float r = 0.f;
-            hipEventElapsedTime(&r, a, b);
-            ms = std::min(ms, r);
-        }
+        float ms;
+        if ((rc = tm.time(launch, &ms))) return rc;
         if (ms < best_ms) {
             best_ms = ms;
             best_idx = idx;
@@ -133,8 +194,6 @@ static int tune_multipass(Handle* h, Plan* p, uint64_t batch, char* desc, uint64
         while (k < idx.size() && ++idx[k] == cands[k].size()) idx[k++] = 0;
         if (k == idx.size()) break;
     }
-    hipEventDestroy(a);
-    hipEventDestroy(b);
     for (size_t i = 0; i < cands.size(); ++i) p->pass_var[i] = cands[i][best_idx[i]];
     if (desc && desc_len) std::snprintf(desc, desc_len, "%s", describe_multipass(p).c_str());
     return WGFFT_OK;
@@ -180,84 +239,51 @@ extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc
         return WGFFT_OK;
     }
     if ((rc = h->ensure(h->io_in, bytes)) || (rc = h->ensure(h->io_out, bytes))) return rc;
-    if ((e = hipMemsetAsync(h->io_in.p, 0, bytes, h->stream)) != hipSuccess) return hip_fail(e, "hipMemsetAsync");
-    hipEvent_t a, b;
-    hipEventCreate(&a);
-    hipEventCreate(&b);
+    if ((rc = fill_noise(h, h->io_in.p, bytes))) return rc;
+    Timer tm(h->stream);
     const BatchedVariant* best = nullptr;
     float best_ms = 1e30f;
+    const BatchedVariant* cur = p->var;
+    auto launch = [&] {
+        h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream, cur);
+    };
+    if ((rc = tm.settle(launch))) return rc;
     for (const auto& v : batched_registry()) {
         if (v.log2n != p->log2n) continue;
-        for (int i = 0; i < 2; ++i)
-            h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream, &v);
-        // best of three rounds of ten launches
-        const int iters = 10;
-        float ms = 1e30f;
-        for (int round = 0; round < 3; ++round) {
-            hipEventRecord(a, h->stream);
-            for (int i = 0; i < iters; ++i)
-                h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream,
-                       &v);
-            hipEventRecord(b, h->stream);
-            if ((e = hipEventSynchronize(b)) != hipSuccess) {
-                hipEventDestroy(a);
-                hipEventDestroy(b);
-                return hip_fail(e, "autotune timing");
-            }
-            float r = 0.f;
-            hipEventElapsedTime(&r, a, b);
-            ms = std::min(ms, r);
-        }
+        cur = &v;
+        float ms;
+        if ((rc = tm.time(launch, &ms))) return rc;
         if (ms < best_ms) {
             best_ms = ms;
             best = &v;
         }
     }
-    hipEventDestroy(a);
-    hipEventDestroy(b);
     // fused forward + adjoint variants for this length
     std::string adj_desc;
     const auto adjs = adjoint_variants(p->log2n);
     if (adjs.size() > 1) {
         if ((rc = h->ensure(h->io_aux, bytes)) || (rc = h->ensure(h->io_w, 2 * n * sizeof(float) + 64))) return rc;
-        if ((e = hipMemsetAsync(h->io_w.p, 0, 2 * n * sizeof(float) + 64, h->stream)) != hipSuccess)
-            return hip_fail(e, "hipMemsetAsync");
+        if ((rc = fill_noise(h, h->io_w.p, 2 * n * sizeof(float) + 64))) return rc;
         float* w = (float*)h->io_w.p;
         float* m = w + n;
         float* loss = m + n;
         float* gw = (float*)h->io_out.p;  // scratch: gx goes to io_aux
         const AdjointVariant* best_adj = nullptr;
         float best_adj_ms = 1e30f;
-        hipEventCreate(&a);
-        hipEventCreate(&b);
+        const AdjointVariant* av = adjs[0];
+        auto alaunch = [&] {
+            fused_adjoint(h, av, true, (const float*)h->io_in.p, w, m, nullptr, nullptr, (float*)h->io_aux.p, gw, loss,
+                          n, batch, p->tw, h->stream);
+        };
         for (const AdjointVariant* v : adjs) {
-            for (int i = 0; i < 2; ++i)
-                fused_adjoint(h, v, true, (const float*)h->io_in.p, w, m, nullptr, nullptr, (float*)h->io_aux.p, gw,
-                              loss, n, batch, p->tw, h->stream);
-            const int iters = 5;
-            float ms = 1e30f;
-            for (int round = 0; round < 3; ++round) {
-                hipEventRecord(a, h->stream);
-                for (int i = 0; i < iters; ++i)
-                    fused_adjoint(h, v, true, (const float*)h->io_in.p, w, m, nullptr, nullptr, (float*)h->io_aux.p,
-                                  gw, loss, n, batch, p->tw, h->stream);
-                hipEventRecord(b, h->stream);
-                if ((e = hipEventSynchronize(b)) != hipSuccess) {
-                    hipEventDestroy(a);
-                    hipEventDestroy(b);
-                    return hip_fail(e, "autotune timing");
-                }
-                float r = 0.f;
-                hipEventElapsedTime(&r, a, b);
-                ms = std::min(ms, r);
-            }
+            av = v;
+            float ms;
+            if ((rc = tm.time(alaunch, &ms))) return rc;
             if (ms < best_adj_ms) {
                 best_adj_ms = ms;
                 best_adj = v;
             }
         }
-        hipEventDestroy(a);
-        hipEventDestroy(b);
         if (best_adj) {
             h->tuned_adj[p->log2n] = best_adj;
             char buf[128];
@@ -280,4 +306,36 @@ extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc
         }
     }
     return WGFFT_OK;
+}
+
+extern "C" int wgfft_variant_count(uint64_t n) {
+    if (n == 0 || (n & (n - 1))) return 0;
+    int log2n = 0;
+    while ((1ull << log2n) < n) ++log2n;
+    int c = 0;
+    for (const auto& v : batched_registry()) c += v.log2n == log2n;
+    return c;
+}
+
+extern "C" int wgfft_select_variant(wgfft_t* h, uint64_t n, int index, char* desc, uint64_t desc_len) {
+    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (n == 0 || (n & (n - 1))) return fail(WGFFT_ERR_INVALID_LENGTH, kMsgLength);
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    Plan* p;
+    int rc;
+    if ((rc = h->get_plan(n, &p))) return rc;
+    if (p->kind != Plan::kBatched) return fail(WGFFT_ERR_INVALID_ARGUMENT, "length has no selectable variants");
+    int i = 0;
+    for (const auto& v : batched_registry()) {
+        if (v.log2n != p->log2n) continue;
+        if (i++ == index) {
+            h->tuned[p->log2n] = &v;
+            p->var = &v;
+            describe(&v, desc, desc_len);
+            return WGFFT_OK;
+        }
+    }
+    return fail(WGFFT_ERR_INVALID_ARGUMENT, "variant index out of range");
 }

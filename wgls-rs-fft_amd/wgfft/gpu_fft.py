@@ -194,6 +194,17 @@ class GpuFft(FftExecutor):
         check(self._L.wgfft_plan_describe(self._h, n, buf, 256))
         return buf.value.decode()
 
+    @staticmethod
+    def variant_count(n: int) -> int:
+        """Compiled single-kernel variants for length n (wgfft_variant_count)."""
+        return int(_lib.load().wgfft_variant_count(n))
+
+    def select_variant(self, n: int, index: int) -> str:
+        """Pin compiled variant `index` for length n (wgfft_select_variant)."""
+        buf = ctypes.create_string_buffer(256)
+        check(self._L.wgfft_select_variant(self._h, n, index, buf, 256))
+        return buf.value.decode()
+
 
 class RaggedPlan:
     """Mixed-length batch: transform i has length 2^log2n[i] at complex

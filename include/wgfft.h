@@ -157,6 +157,16 @@ int wgfft_variant_count(uint64_t n);
  * check every one. */
 int wgfft_select_variant(wgfft_t* h, uint64_t n, int index, char* desc, uint64_t desc_len);
 
+/* Global passes of the plan for length n (0 for single-kernel lengths), and
+ * the compiled candidates for pass `pass` (radix, E, G, prefetch or LDS-copy
+ * pipeline). */
+int wgfft_pass_count(uint64_t n);
+int wgfft_pass_variant_count(uint64_t n, int pass);
+
+/* Make candidate `index` the kernel of global pass `pass` of length n for
+ * this handle; writes the plan description into desc (if non-NULL). */
+int wgfft_select_pass_variant(wgfft_t* h, uint64_t n, int pass, int index, char* desc, uint64_t desc_len);
+
 /* Fill nfloats floats with i.i.d. U[-1,1) = (u32 >> 8) * 2^-23 - 1 drawn
  * from PCG32 seeded through SplitMix64(seed) (input synthesis, host). */
 void wgfft_synth_uniform(float* out, uint64_t nfloats, uint64_t seed);

@@ -81,8 +81,15 @@ def test_variant_registry_counts():
     assert wgfft.GpuFft.variant_count(16) == 0  # register kernel
     assert wgfft.GpuFft.variant_count(1 << 20) == 0  # global passes
     assert wgfft.GpuFft.variant_count(12) == 0
+    assert wgfft.GpuFft.pass_count(4096) == 0
+    assert wgfft.GpuFft.pass_count(1 << 22) == 2
+    assert wgfft.GpuFft.pass_count(1 << 27) == 3
+    for p in range(2):
+        assert wgfft.GpuFft.pass_variant_count(1 << 22, p) >= 2
+    assert wgfft.GpuFft.pass_variant_count(1 << 22, 2) == 0
     L = _lib.load()
     assert L.wgfft_select_variant(None, 4096, 0, None, 0) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
+    assert L.wgfft_select_pass_variant(None, 1 << 20, 0, 0, None, 0) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
 
 
 def test_synth_uniform_matches_oracle_stream():

@@ -92,6 +92,25 @@ def test_multipass_lengths(gpu, log2n, batch):
     check_fwd_inv(gpu, x)
 
 
+@pytest.mark.parametrize("log2n,batch", [(17, 3), (20, 2), (22, 2), (23, 1)])
+def test_every_global_pass_variant(log2n, batch):
+    """Every compiled candidate of every global pass (register-prefetch,
+    plain and LDS-copy pipelines), pinned in turn with the other passes on
+    their defaults, matches FP64 forward and inverse."""
+    n = 1 << log2n
+    x = oracle.synth_uniform((batch, n), 0x5EED0180 + log2n)
+    for p in range(wgfft.GpuFft.pass_count(n)):
+        for i in range(wgfft.GpuFft.pass_variant_count(n, p)):
+            g = wgfft.GpuFft.new()
+            try:
+                desc = g.select_pass_variant(n, p, i)
+                check_fwd_inv(g, x)
+            except AssertionError as e:
+                raise AssertionError(f"pass {p} variant {i}: {desc}: {e}") from None
+            finally:
+                g.close()
+
+
 @pytest.mark.parametrize("log2n", [17, 20, 23])
 def test_multipass_autotuned(log2n):
     """The global-pass combination the autotuner keeps matches FP64."""

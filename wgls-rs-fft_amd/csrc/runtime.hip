@@ -151,6 +151,65 @@ std::vector<const GlobalPassVariant*> global_pass_candidates(int log2r, uint64_t
     return out;
 }
 
+// Balanced radix split (log2 of each pass radix, larger first) of 2^log2n
+// into passes of at most 2^maxr.
+static std::vector<int> balanced_split(int log2n, int maxr) {
+    std::vector<int> r;
+    const int passes = (log2n + maxr - 1) / maxr;
+    int rem = log2n;
+    for (int i = 0; i < passes; ++i) {
+        const int ri = (rem + (passes - i) - 1) / (passes - i);
+        r.push_back(ri);
+        rem -= ri;
+    }
+    return r;
+}
+
+// Default global-pass split of 2^log2n: radices up to 2^13 (fewest passes).
+std::vector<int> multipass_radices(int log2n) { return balanced_split(log2n, 13); }
+
+// Splits the autotuner times: the default, and the ones with radices up to
+// 2^9 and 2^8 (one or two more passes over memory, but tiles small enough
+// for several workgroups per CU and 128-byte column runs).
+std::vector<std::vector<int>> multipass_splits(int log2n) {
+    std::vector<std::vector<int>> out;
+    for (int maxr : {13, 9, 8}) {
+        std::vector<int> r = balanced_split(log2n, maxr);
+        bool dup = false;
+        for (const auto& o : out) dup |= (o == r);
+        bool ok = true;
+        uint64_t L = 1;
+        for (int ri : r) {
+            ok &= !global_pass_candidates(ri, L, (1ull << log2n) >> ri).empty();
+            L <<= ri;
+        }
+        if (!dup && ok) out.push_back(r);
+    }
+    return out;
+}
+
+// Give a multipass plan the radix split r (default variant per pass) and
+// upload the sub-FFT tables it needs.
+int Handle::set_split(Plan* p, const std::vector<int>& r) {
+    int rc;
+    std::vector<const GlobalPassVariant*> vars;
+    uint64_t L = 1;
+    for (int ri : r) {
+        auto cands = global_pass_candidates(ri, L, p->n >> ri);
+        if (cands.empty()) return fail(WGFFT_ERR_INVALID_LENGTH, "no global-pass variant for radix 2^" + std::to_string(ri));
+        vars.push_back(cands.front());
+        if (!p->tw_r.count(ri)) {
+            float2* t = nullptr;
+            if ((rc = upload_table(twiddle_table(1ull << ri, 1ull << ri, 1), &t))) return rc;
+            p->tw_r[ri] = t;
+        }
+        L <<= ri;
+    }
+    p->pass_log2r = r;
+    p->pass_var = vars;
+    return WGFFT_OK;
+}
+
 int Handle::get_plan(uint64_t n, Plan** out) {
     auto it = plans.find(n);
     if (it != plans.end()) {
@@ -171,24 +230,7 @@ int Handle::get_plan(uint64_t n, Plan** out) {
     } else {
         if (p.log2n > 39) return fail(WGFFT_ERR_INVALID_LENGTH, "transform length above 2^39 is not supported");
         p.kind = Plan::kMultiPass;
-        int passes = (p.log2n + 12) / 13;
-        int rem = p.log2n;
-        uint64_t L = 1;
-        for (int i = 0; i < passes; ++i) {
-            int r = (rem + (passes - i) - 1) / (passes - i);  // larger radices first
-            rem -= r;
-            auto cands = global_pass_candidates(r, L, n >> r);
-            const GlobalPassVariant* v = cands.empty() ? nullptr : cands.front();
-            if (!v) return fail(WGFFT_ERR_INVALID_LENGTH, "no global-pass variant for radix 2^" + std::to_string(r));
-            p.pass_log2r.push_back(r);
-            p.pass_var.push_back(v);
-            if (!p.tw_r.count(r)) {
-                float2* t = nullptr;
-                if ((rc = upload_table(twiddle_table(1ull << r, 1ull << r, 1), &t))) return rc;
-                p.tw_r[r] = t;
-            }
-            L <<= r;
-        }
+        if ((rc = set_split(&p, multipass_radices(p.log2n)))) return rc;
         p.h = (uint32_t)((p.log2n + 1) / 2);
         uint64_t lo_n = 1ull << p.h, hi_n = 1ull << (p.log2n - p.h);
         if ((rc = upload_table(twiddle_table(n, lo_n, 1), &p.tw_lo))) return rc;

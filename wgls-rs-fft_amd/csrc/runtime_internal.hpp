@@ -54,6 +54,8 @@ int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x
 const char* adjoint_describe(const AdjointVariant* v, char* buf, uint64_t len);
 void synth_uniform(float* out, uint64_t nfloats, uint64_t seed);
 const BatchedVariant* default_variant(int log2n);
+std::vector<int> multipass_radices(int log2n);
+std::vector<std::vector<int>> multipass_splits(int log2n);
 const BatchedVariant* find_variant(int log2n, const char* plan, int E, int G, int flags);
 std::vector<const GlobalPassVariant*> global_pass_candidates(int log2r, uint64_t L, uint64_t nr);
 
@@ -75,6 +77,7 @@ struct Handle {
     void release(DevBuf& b);
     int upload_table(const std::vector<float2>& host, float2** dev);
     int get_plan(uint64_t n, Plan** out);
+    int set_split(Plan* p, const std::vector<int>& log2r);
     int blocks_per_cu(const BatchedVariant* v, Mode m);
     int run(Plan* p, Mode mode, const float2* in, float2* out, const float* win, uint64_t batch, float scale,
             hipStream_t s, const BatchedVariant* force = nullptr);

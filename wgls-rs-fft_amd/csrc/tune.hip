@@ -155,46 +155,52 @@ static std::string describe_multipass(const Plan* p) {
     for (size_t i = 0; i < p->pass_var.size(); ++i) {
         const GlobalPassVariant* v = p->pass_var[i];
         s += (i ? "," : "") + std::string("R") + std::to_string(1 << v->log2r) + "(" + v->plan + ",G" +
-             std::to_string(v->G) + (v->prefetch ? ",prefetch" : "") + ")";
+             std::to_string(v->G) + (v->prefetch == 2 ? ",lds-copy" : v->prefetch ? ",prefetch" : "") + ")";
     }
     return s;
 }
 
-// Time every combination of global-pass variants for a multipass plan (a
-// few per pass) on a scratch batch and keep the fastest.
+// Time every radix split the planner offers (multipass_splits) and, for
+// each, every combination of global-pass variants (a few per pass) on a
+// scratch batch; keep the fastest split and variants.
 static int tune_multipass(Handle* h, Plan* p, uint64_t batch, char* desc, uint64_t desc_len) {
     const uint64_t bytes = p->n * batch * sizeof(float2);
     int rc;
     if ((rc = h->ensure(h->io_in, bytes)) || (rc = h->ensure(h->io_out, bytes))) return rc;
-    hipError_t e;
     if ((rc = fill_noise(h, h->io_in.p, bytes))) return rc;
-    std::vector<std::vector<const GlobalPassVariant*>> cands;
-    uint64_t L = 1;
-    for (size_t i = 0; i < p->pass_log2r.size(); ++i) {
-        cands.push_back(global_pass_candidates(p->pass_log2r[i], L, p->n >> p->pass_log2r[i]));
-        L <<= p->pass_log2r[i];
-    }
-    std::vector<size_t> idx(cands.size(), 0), best_idx = idx;
-    float best_ms = 1e30f;
     Timer tm(h->stream);
     auto launch = [&] {
         h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream);
     };
-    for (size_t i = 0; i < cands.size(); ++i) p->pass_var[i] = cands[i][0];
     if ((rc = tm.settle(launch))) return rc;
-    for (;;) {
-        for (size_t i = 0; i < cands.size(); ++i) p->pass_var[i] = cands[i][idx[i]];
-        float ms;
-        if ((rc = tm.time(launch, &ms))) return rc;
-        if (ms < best_ms) {
-            best_ms = ms;
-            best_idx = idx;
+    std::vector<int> best_split = p->pass_log2r;
+    std::vector<const GlobalPassVariant*> best_vars = p->pass_var;
+    float best_ms = 1e30f;
+    for (const std::vector<int>& split : multipass_splits(p->log2n)) {
+        if ((rc = h->set_split(p, split))) return rc;
+        std::vector<std::vector<const GlobalPassVariant*>> cands;
+        uint64_t L = 1;
+        for (size_t i = 0; i < split.size(); ++i) {
+            cands.push_back(global_pass_candidates(split[i], L, p->n >> split[i]));
+            L <<= split[i];
         }
-        size_t k = 0;
-        while (k < idx.size() && ++idx[k] == cands[k].size()) idx[k++] = 0;
-        if (k == idx.size()) break;
+        std::vector<size_t> idx(cands.size(), 0);
+        for (;;) {
+            for (size_t i = 0; i < cands.size(); ++i) p->pass_var[i] = cands[i][idx[i]];
+            float ms;
+            if ((rc = tm.time(launch, &ms))) return rc;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_split = split;
+                best_vars = p->pass_var;
+            }
+            size_t k = 0;
+            while (k < idx.size() && ++idx[k] == cands[k].size()) idx[k++] = 0;
+            if (k == idx.size()) break;
+        }
     }
-    for (size_t i = 0; i < cands.size(); ++i) p->pass_var[i] = cands[i][best_idx[i]];
+    if ((rc = h->set_split(p, best_split))) return rc;
+    p->pass_var = best_vars;
     if (desc && desc_len) std::snprintf(desc, desc_len, "%s", describe_multipass(p).c_str());
     return WGFFT_OK;
 }
@@ -338,4 +344,48 @@ extern "C" int wgfft_select_variant(wgfft_t* h, uint64_t n, int index, char* des
         }
     }
     return fail(WGFFT_ERR_INVALID_ARGUMENT, "variant index out of range");
+}
+
+static int log2_exact(uint64_t n) {
+    if (n == 0 || (n & (n - 1))) return -1;
+    int l = 0;
+    while ((1ull << l) < n) ++l;
+    return l;
+}
+
+extern "C" int wgfft_pass_count(uint64_t n) {
+    const int l = log2_exact(n);
+    if (l <= 14 || l > 39) return 0;
+    return (int)multipass_radices(l).size();
+}
+
+extern "C" int wgfft_pass_variant_count(uint64_t n, int pass) {
+    const int l = log2_exact(n);
+    if (l <= 14 || l > 39) return 0;
+    const std::vector<int> r = multipass_radices(l);
+    if (pass < 0 || pass >= (int)r.size()) return 0;
+    uint64_t L = 1;
+    for (int i = 0; i < pass; ++i) L <<= r[i];
+    return (int)global_pass_candidates(r[pass], L, n >> r[pass]).size();
+}
+
+extern "C" int wgfft_select_pass_variant(wgfft_t* h, uint64_t n, int pass, int index, char* desc,
+                                         uint64_t desc_len) {
+    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (log2_exact(n) < 0) return fail(WGFFT_ERR_INVALID_LENGTH, kMsgLength);
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    Plan* p;
+    int rc;
+    if ((rc = h->get_plan(n, &p))) return rc;
+    if (p->kind != Plan::kMultiPass) return fail(WGFFT_ERR_INVALID_ARGUMENT, "length has no global passes");
+    if (pass < 0 || pass >= (int)p->pass_log2r.size()) return fail(WGFFT_ERR_INVALID_ARGUMENT, "pass out of range");
+    uint64_t L = 1;
+    for (int i = 0; i < pass; ++i) L <<= p->pass_log2r[i];
+    const auto c = global_pass_candidates(p->pass_log2r[pass], L, n >> p->pass_log2r[pass]);
+    if (index < 0 || index >= (int)c.size()) return fail(WGFFT_ERR_INVALID_ARGUMENT, "variant index out of range");
+    p->pass_var[pass] = c[index];
+    if (desc && desc_len) std::snprintf(desc, desc_len, "%s", describe_multipass(p).c_str());
+    return WGFFT_OK;
 }

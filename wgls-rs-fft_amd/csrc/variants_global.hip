@@ -23,3 +23,15 @@ WGFFT_REGISTER_GLOBAL_PASS(4096, 32, 4, 0, 16x16x16, 16, 16, 16);
 WGFFT_REGISTER_GLOBAL_PASS(8192, 16, 2, 0, 8x16x8x8, 8, 16, 8, 8);
 WGFFT_REGISTER_GLOBAL_PASS(8192, 32, 2, 0, 16x16x16x2, 16, 16, 16, 2);
 WGFFT_REGISTER_GLOBAL_PASS(8192, 32, 2, 1, 16x16x16x2, 16, 16, 16, 2);
+
+// Tiles streamed into two LDS buffers by the asynchronous global -> LDS copy
+// (one workgroup per CU, no prefetch registers).
+WGFFT_REGISTER_GLOBAL_PASS_DMA(1024, 16, 8, 8x16x8, 8, 16, 8);
+WGFFT_REGISTER_GLOBAL_PASS_DMA(2048, 16, 4, 8x16x2x8, 8, 16, 2, 8);
+WGFFT_REGISTER_GLOBAL_PASS_DMA(2048, 16, 4, 8x8x4x8, 8, 8, 4, 8);
+WGFFT_REGISTER_GLOBAL_PASS_DMA(2048, 32, 4, 16x16x8, 16, 16, 8);
+WGFFT_REGISTER_GLOBAL_PASS_DMA(4096, 16, 2, 8x8x8x8, 8, 8, 8, 8);
+
+// Wider column groups for the small-radix splits (256-byte runs).
+WGFFT_REGISTER_GLOBAL_PASS(256, 16, 32, 0, 8x4x8, 8, 4, 8);
+WGFFT_REGISTER_GLOBAL_PASS(512, 16, 16, 0, 8x8x8, 8, 8, 8);

@@ -29,6 +29,7 @@ EXPORTS = (
     "wgfft_ragged_create", "wgfft_ragged_destroy", "wgfft_ragged_exec_device", "wgfft_ragged_samples",
     "wgfft_bench_device",
     "wgfft_autotune", "wgfft_plan_describe", "wgfft_variant_count", "wgfft_select_variant",
+    "wgfft_pass_count", "wgfft_pass_variant_count", "wgfft_select_pass_variant",
     "wgfft_synth_uniform", "wgfft_set_alloc_cap",
 )
 
@@ -90,6 +91,9 @@ def load() -> ctypes.CDLL:
     L.wgfft_plan_describe.argtypes = [_vp, _u64, ctypes.c_char_p, _u64]
     L.wgfft_variant_count.argtypes = [_u64]
     L.wgfft_select_variant.argtypes = [_vp, _u64, _i, ctypes.c_char_p, _u64]
+    L.wgfft_pass_count.argtypes = [_u64]
+    L.wgfft_pass_variant_count.argtypes = [_u64, _i]
+    L.wgfft_select_pass_variant.argtypes = [_vp, _u64, _i, _i, ctypes.c_char_p, _u64]
     L.wgfft_synth_uniform.argtypes = [_vp, _u64, _u64]
     L.wgfft_synth_uniform.restype = None
     L.wgfft_set_alloc_cap.argtypes = [_u64]

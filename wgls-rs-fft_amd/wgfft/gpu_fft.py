@@ -199,6 +199,22 @@ class GpuFft(FftExecutor):
         """Compiled single-kernel variants for length n (wgfft_variant_count)."""
         return int(_lib.load().wgfft_variant_count(n))
 
+    @staticmethod
+    def pass_count(n: int) -> int:
+        """Global passes of the plan for length n (wgfft_pass_count)."""
+        return int(_lib.load().wgfft_pass_count(n))
+
+    @staticmethod
+    def pass_variant_count(n: int, p: int) -> int:
+        """Compiled candidates for global pass p of length n."""
+        return int(_lib.load().wgfft_pass_variant_count(n, p))
+
+    def select_pass_variant(self, n: int, p: int, index: int) -> str:
+        """Pin candidate `index` for global pass p of length n."""
+        buf = ctypes.create_string_buffer(256)
+        check(self._L.wgfft_select_pass_variant(self._h, n, p, index, buf, 256))
+        return buf.value.decode()
+
     def select_variant(self, n: int, index: int) -> str:
         """Pin compiled variant `index` for length n (wgfft_select_variant)."""
         buf = ctypes.create_string_buffer(256)

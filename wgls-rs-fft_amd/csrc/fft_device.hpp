@@ -314,9 +314,10 @@ struct LdsLayout {
 // first entry in a register-held set).
 //  TwTable:   read from the W_N table in global memory (tw[base << i]).
 //  TwHoisted: register copies loaded once per thread by load_twiddles.
-//  TwLds:     W_N^j = hi[j >> h] * lo[j & (2^h - 1)] from two short tables
-//             in LDS (2 sqrt(N) entries); LDS reads retire on lgkmcnt, so
-//             they never wait behind prefetched global loads.
+//  TwLds:     W_N^j = hi[J >> h] * lo[J & (2^h - 1)], J = j << sh, from two
+//             short tables of a length-(N << sh) table in LDS (2 sqrt entries);
+//             LDS reads retire on lgkmcnt, so they never wait behind
+//             prefetched global loads.
 struct TwTable {
     const float2* __restrict__ tw;
     template <int R, int TO>
@@ -337,14 +338,17 @@ struct TwLds {
     const float2* lo;
     const float2* hi;
     uint32_t h;
+    uint32_t sh;
     template <int R, int TO>
     __device__ __forceinline__ void bases(float2* wb, int, uint32_t base) const {
 #pragma unroll
         for (int i = 0; i < tw_bases(R); ++i) {
-            const uint32_t jj = base << i;
+            const uint32_t jj = base << (i + sh);
             wb[i] = cmul(hi[jj >> h], lo[jj & ((1u << h) - 1)]);
         }
     }
+    // the same tables serving a length N >> s transform
+    __device__ __forceinline__ TwLds shifted(uint32_t s) const { return TwLds{lo, hi, h, sh + s}; }
 };
 
 // Split of a length-N W_N table into TwLds's two LDS tables: lo has 2^h
@@ -361,7 +365,7 @@ __device__ __forceinline__ TwLds twlds_fill(float2* lds_tw, const float2* __rest
     constexpr int NLO = 1 << H, NHI = N >> H;
     for (uint32_t i = tid; i < NLO + NHI; i += nthreads)
         lds_tw[i] = i < NLO ? tw[i] : tw[(i - NLO) << H];
-    return TwLds{lds_tw, lds_tw + NLO, (uint32_t)H};
+    return TwLds{lds_tw, lds_tw + NLO, (uint32_t)H, 0u};
 }
 
 // Stockham pass engine.  d[] holds the thread's E elements; on entry (first

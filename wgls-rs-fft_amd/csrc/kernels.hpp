@@ -392,10 +392,10 @@ __device__ __forceinline__ void gp_load(float2 (&v)[E], const float2* __restrict
     }
 }
 
-template <int R, int E, int G, int MODE, class TW, int... Rs>
+template <int R, int E, int G, int MODE, class TW, class TS, int... Rs>
 __device__ __forceinline__ void gp_compute(float2 (&v)[E], const GpSide<TW, E, MODE>& sd, float2* __restrict__ dst,
-                                           const float2* __restrict__ tw_r, uint64_t N, uint64_t L, uint64_t b0,
-                                           float sc, float2* __restrict__ lds_all, uint32_t tid) {
+                                           const TS& ts, uint64_t N, uint64_t L, uint64_t b0, float sc,
+                                           float2* __restrict__ lds_all, uint32_t tid) {
     constexpr int T = R / E;
     constexpr int THREADS = G * T;
     constexpr bool INV = MODE == kInverse;
@@ -445,7 +445,7 @@ __device__ __forceinline__ void gp_compute(float2 (&v)[E], const GpSide<TW, E, M
         d[2 * q + 1] = lds[pad_idx<5, 0>(2 * t + 1 + 2 * T * q)];
     }
     WGFFT_LDS_SYNC();
-    Engine<R, E, T, INV>::run(d, lds, tw_r, t, Radices<Rs...>{});
+    Engine<R, E, T, INV>::run_with(d, lds, ts, t, Radices<Rs...>{});
 
     if (first) {
         // output b R + m: R contiguous elements per butterfly
@@ -487,6 +487,7 @@ k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const flo
               float scale) {
     constexpr int SLOT = GlobalPassSlot<R, E, Rs...>::len;
     __shared__ __align__(16) float2 lds_all[G * SLOT];
+    __shared__ __align__(16) float2 lds_tw[twlds_len(ilog2(R))];
 
     const uint64_t N = 1ull << log2n;
     const uint64_t NR = N / R;
@@ -497,6 +498,9 @@ k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const flo
     const uint64_t lid0 = (pid & 7) * (nb >> 3) + (pid >> 3);
     const bool last = (L * R == N);
     const TwTwoLevel twN{tw_hi, tw_lo, h};
+    // the sub-FFT's twiddles come from LDS, so no wait on them stalls on the
+    // prefetched tile (the tables are synchronised by gp_compute's first barrier)
+    const TwLds twr = twlds_fill<R>(lds_tw, tw_r, threadIdx.x, G * (R / E));
     using Side = GpSide<TwTwoLevel, E, MODE>;
     float2 cur[E];
     Side sd;
@@ -519,16 +523,16 @@ k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const flo
                 gp_load_side<R, E, G, MODE>(nsd, twN, win, N, L, nb0, tid);
                 gp_load<R, E, G>(nxt, in + ntr * N, NR, nb0, tid);
             }
-            gp_compute<R, E, G, MODE, TwTwoLevel, Rs...>(cur, sd, out + tr * N, tw_r, N, L, b0,
-                                                         last ? scale : 1.0f, lds_all, tid);
+            gp_compute<R, E, G, MODE, TwTwoLevel, TwLds, Rs...>(cur, sd, out + tr * N, twr, N, L, b0,
+                                                                last ? scale : 1.0f, lds_all, tid);
 #pragma unroll
             for (int i = 0; i < E; ++i) cur[i] = nxt[i];
             sd = nsd;
         } else {
             gp_load_side<R, E, G, MODE>(sd, twN, win, N, L, b0, tid);
             gp_load<R, E, G>(cur, in + tr * N, NR, b0, tid);
-            gp_compute<R, E, G, MODE, TwTwoLevel, Rs...>(cur, sd, out + tr * N, tw_r, N, L, b0,
-                                                         last ? scale : 1.0f, lds_all, tid);
+            gp_compute<R, E, G, MODE, TwTwoLevel, TwLds, Rs...>(cur, sd, out + tr * N, twr, N, L, b0,
+                                                                last ? scale : 1.0f, lds_all, tid);
         }
         WGFFT_LDS_SYNC();
     }
@@ -538,7 +542,7 @@ template <int R, int E, int G, bool PF, int... Rs>
 struct GlobalPassKernel {
     static constexpr int T = R / E;
     static constexpr int THREADS = G * T;
-    static constexpr int LDS_BYTES = G * GlobalPassSlot<R, E, Rs...>::len * 8;
+    static constexpr int LDS_BYTES = G * GlobalPassSlot<R, E, Rs...>::len * 8 + twlds_len(ilog2(R)) * 8;
 
     template <int MODE>
     static int occ() {

@@ -34,11 +34,12 @@ struct RaggedUnit {
 };
 
 constexpr int kRaggedThreads = 256;
-constexpr int kMinLog2 = 6, kMaxLog2 = 16, kDirectMaxLog2 = 12;
-constexpr int kRaggedE = 16;  // elements per thread in every unit
+constexpr int kMinLog2 = 6, kMaxLog2 = 16, kDirectMaxLog2 = 13;
 
+// elements per thread of the direct (LDS-resident) unit for length 2^k
+constexpr int direct_e(int log2n) { return log2n == 13 ? 32 : 16; }
 // transforms per direct unit for length 2^k
-constexpr int direct_g(int log2n) { return kRaggedThreads / ((1 << log2n) / kRaggedE); }
+constexpr int direct_g(int log2n) { return kRaggedThreads / ((1 << log2n) / direct_e(log2n)); }
 
 // LDS footprint (complex elements) of every unit kind
 template <int N, int E, int... Rs>
@@ -51,7 +52,7 @@ constexpr int kRaggedLds = cmax(
     cmax(cmax(direct_lds<64, 16, 8, 8>(), direct_lds<128, 16, 4, 4, 8>()),
          cmax(direct_lds<256, 16, 8, 4, 8>(), direct_lds<512, 16, 8, 8, 8>())),
     cmax(cmax(direct_lds<1024, 16, 8, 16, 8>(), direct_lds<2048, 16, 8, 4, 8, 8>()),
-         cmax(direct_lds<4096, 16, 8, 8, 8, 8>(),
+         cmax(cmax(direct_lds<4096, 16, 8, 8, 8, 8>(), direct_lds<8192, 32, 16, 16, 16, 2>()),
               cmax(cmax(pass_lds<64, 16, 8, 8>(), pass_lds<128, 16, 4, 4, 8>()), pass_lds<256, 16, 8, 4, 8>()))));
 
 // Unit handlers take the thread index through launder_lane() once per unit,
@@ -60,8 +61,7 @@ constexpr int kRaggedLds = cmax(
 template <int N, int E, bool INV, int... Rs>
 __device__ __forceinline__ void ragged_direct(const float2* __restrict__ in, float2* __restrict__ out,
                                               const uint64_t* __restrict__ offs, uint32_t first, uint32_t count,
-                                              const float2* __restrict__ tw_all, float2* __restrict__ lds_raw,
-                                              uint32_t tid) {
+                                              const TwLds& tl, float2* __restrict__ lds_raw, uint32_t tid) {
     constexpr int T = N / E;
     constexpr int LEN = LdsLayout<N, E, Rs...>::len;
     static_assert(T * direct_g(ilog2(N)) == kRaggedThreads, "direct unit must fill the workgroup");
@@ -70,7 +70,7 @@ __device__ __forceinline__ void ragged_direct(const float2* __restrict__ in, flo
     const uint64_t off = active ? offs[first + g] : 0;
     float2 d[E];
     if (active) Io<N, E, Rs...>::load(d, in + off, t);
-    Engine<N, E, T, INV>::run(d, lds_raw + g * LEN, tw_all + N, t, Radices<Rs...>{});
+    Engine<N, E, T, INV>::run_with(d, lds_raw + g * LEN, tl.shifted(kMaxLog2 - ilog2(N)), t, Radices<Rs...>{});
     if (active) Io<N, E, Rs...>::template store<INV>(out + off, d, t, 1.0f / (float)N);
 }
 
@@ -90,7 +90,8 @@ struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV, PF
     static_assert(G2 <= R1, "second pass needs G <= L");
     __device__ __forceinline__ static void run(const float2* __restrict__ src, float2* __restrict__ dst,
                                                float2* __restrict__ scratch, const float2* __restrict__ tw_all,
-                                               float2* __restrict__ lds, uint32_t tid) {
+                                               const TwLds& tl, float2* __restrict__ lds, uint32_t tid) {
+        const TwLds t1 = tl.shifted(kMaxLog2 - ilog2(R1)), t2 = tl.shifted(kMaxLog2 - ilog2(R2));
         constexpr int M = INV ? kInverse : kForward;
         const TwOneLevel twN{tw_all + N};
         using Side = GpSide<TwOneLevel, E1, M>;
@@ -104,14 +105,14 @@ struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV, PF
             if constexpr (PF) {
                 float2 nxt[E1];
                 if (b0 + G1 < N / R1) gp_load<R1, E1, G1>(nxt, src, N / R1, b0 + G1, tid);
-                gp_compute<R1, E1, G1, M, TwOneLevel, Rs1...>(cur, sd, scratch, tw_all + R1, N, 1, b0, 1.0f, lds,
-                                                              tid);
+                gp_compute<R1, E1, G1, M, TwOneLevel, TwLds, Rs1...>(cur, sd, scratch, t1, N, 1, b0, 1.0f, lds,
+                                                                     tid);
 #pragma unroll
                 for (int i = 0; i < E1; ++i) cur[i] = nxt[i];
             } else {
                 gp_load<R1, E1, G1>(cur, src, N / R1, b0, tid);
-                gp_compute<R1, E1, G1, M, TwOneLevel, Rs1...>(cur, sd, scratch, tw_all + R1, N, 1, b0, 1.0f, lds,
-                                                              tid);
+                gp_compute<R1, E1, G1, M, TwOneLevel, TwLds, Rs1...>(cur, sd, scratch, t1, N, 1, b0, 1.0f, lds,
+                                                                     tid);
             }
             WGFFT_LDS_SYNC();
         }
@@ -131,14 +132,14 @@ struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV, PF
                     gp_load_side<R2, E2, G2, M>(nsd, twN, nullptr, N, R1, b0 + G2, tid);
                     gp_load<R2, E2, G2>(nxt, scratch, N / R2, b0 + G2, tid);
                 }
-                gp_compute<R2, E2, G2, M, TwOneLevel, Rs2...>(cur, sd, dst, tw_all + R2, N, R1, b0, sc, lds, tid);
+                gp_compute<R2, E2, G2, M, TwOneLevel, TwLds, Rs2...>(cur, sd, dst, t2, N, R1, b0, sc, lds, tid);
 #pragma unroll
                 for (int i = 0; i < E2; ++i) cur[i] = nxt[i];
                 sd = nsd;
             } else {
                 gp_load_side<R2, E2, G2, M>(sd, twN, nullptr, N, R1, b0, tid);
                 gp_load<R2, E2, G2>(cur, scratch, N / R2, b0, tid);
-                gp_compute<R2, E2, G2, M, TwOneLevel, Rs2...>(cur, sd, dst, tw_all + R2, N, R1, b0, sc, lds, tid);
+                gp_compute<R2, E2, G2, M, TwOneLevel, TwLds, Rs2...>(cur, sd, dst, t2, N, R1, b0, sc, lds, tid);
             }
             WGFFT_LDS_SYNC();
         }
@@ -152,8 +153,13 @@ k_ragged(const float2* __restrict__ in, float2* __restrict__ out, const RaggedUn
          uint32_t u_begin, uint32_t u_end, const uint64_t* __restrict__ offs, const float2* __restrict__ tw_all,
          float2* __restrict__ scratch, uint32_t* __restrict__ counters) {
     __shared__ __align__(16) float2 lds[kRaggedLds];
+    __shared__ __align__(16) float2 lds_tw[twlds_len(kMaxLog2)];
     __shared__ uint32_t s_unit;
     float2* my_scratch = scratch + (uint64_t)blockIdx.x * (1ull << kMaxLog2);
+    // W_65536 as two LDS tables; every sub-FFT of length R reads it with
+    // stride 65536 / R (TwLds::shifted), so no twiddle load of a unit waits
+    // behind another unit's global loads
+    const TwLds tl = twlds_fill<1 << kMaxLog2>(lds_tw, tw_all + (1u << kMaxLog2), threadIdx.x, kRaggedThreads);
 
     for (;;) {
         if (threadIdx.x == 0) {
@@ -173,35 +179,30 @@ k_ragged(const float2* __restrict__ in, float2* __restrict__ out, const RaggedUn
         const RaggedUnit un = units[u];
         const uint32_t tid = launder_lane(threadIdx.x);
         switch (un.log2n) {
-            case 6: ragged_direct<64, 16, INV, 8, 8>(in, out, offs, un.first, un.count, tw_all, lds, tid); break;
-            case 7: ragged_direct<128, 16, INV, 4, 4, 8>(in, out, offs, un.first, un.count, tw_all, lds, tid); break;
-            case 8: ragged_direct<256, 16, INV, 8, 4, 8>(in, out, offs, un.first, un.count, tw_all, lds, tid); break;
-            case 9: ragged_direct<512, 16, INV, 8, 8, 8>(in, out, offs, un.first, un.count, tw_all, lds, tid); break;
-            case 10: ragged_direct<1024, 16, INV, 8, 16, 8>(in, out, offs, un.first, un.count, tw_all, lds, tid); break;
-            case 11: ragged_direct<2048, 16, INV, 8, 4, 8, 8>(in, out, offs, un.first, un.count, tw_all, lds, tid); break;
-            case 12: ragged_direct<4096, 16, INV, 8, 8, 8, 8>(in, out, offs, un.first, un.count, tw_all, lds, tid); break;
-            case 13: {
-                const uint64_t o = offs[un.first];
-                RaggedBig<13, 128, 16, Radices<4, 4, 8>, 64, 16, Radices<8, 8>, INV, PF>::run(in + o, out + o,
-                                                                                          my_scratch, tw_all, lds, tid);
-                break;
-            }
+            case 6: ragged_direct<64, 16, INV, 8, 8>(in, out, offs, un.first, un.count, tl, lds, tid); break;
+            case 7: ragged_direct<128, 16, INV, 4, 4, 8>(in, out, offs, un.first, un.count, tl, lds, tid); break;
+            case 8: ragged_direct<256, 16, INV, 8, 4, 8>(in, out, offs, un.first, un.count, tl, lds, tid); break;
+            case 9: ragged_direct<512, 16, INV, 8, 8, 8>(in, out, offs, un.first, un.count, tl, lds, tid); break;
+            case 10: ragged_direct<1024, 16, INV, 8, 16, 8>(in, out, offs, un.first, un.count, tl, lds, tid); break;
+            case 11: ragged_direct<2048, 16, INV, 8, 4, 8, 8>(in, out, offs, un.first, un.count, tl, lds, tid); break;
+            case 12: ragged_direct<4096, 16, INV, 8, 8, 8, 8>(in, out, offs, un.first, un.count, tl, lds, tid); break;
+            case 13: ragged_direct<8192, 32, INV, 16, 16, 16, 2>(in, out, offs, un.first, un.count, tl, lds, tid); break;
             case 14: {
                 const uint64_t o = offs[un.first];
                 RaggedBig<14, 128, 16, Radices<4, 4, 8>, 128, 16, Radices<4, 4, 8>, INV, PF>::run(in + o, out + o,
-                                                                                              my_scratch, tw_all, lds, tid);
+                                                                                              my_scratch, tw_all, tl, lds, tid);
                 break;
             }
             case 15: {
                 const uint64_t o = offs[un.first];
                 RaggedBig<15, 256, 16, Radices<8, 4, 8>, 128, 16, Radices<4, 4, 8>, INV, PF>::run(in + o, out + o,
-                                                                                              my_scratch, tw_all, lds, tid);
+                                                                                              my_scratch, tw_all, tl, lds, tid);
                 break;
             }
             default: {
                 const uint64_t o = offs[un.first];
                 RaggedBig<16, 256, 16, Radices<8, 4, 8>, 256, 16, Radices<8, 4, 8>, INV, PF>::run(in + o, out + o,
-                                                                                              my_scratch, tw_all, lds, tid);
+                                                                                              my_scratch, tw_all, tl, lds, tid);
                 break;
             }
         }

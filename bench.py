@@ -98,6 +98,7 @@ def bench_mixed(torch, gpu, wgfft, steps: int) -> dict:
         gbs = 16.0 * total / (ms * 1e-3) / 1e9
         out[name] = {"ms": round(ms, 4), "gsamples_s": round(total / (ms * 1e-3) / 1e9, 2),
                      "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    out["kernel"] = plan.describe()
     plan.close()
     del x, y
     return out

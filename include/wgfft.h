@@ -130,6 +130,10 @@ int wgfft_ragged_exec_device(wgfft_ragged_t* p, const float* d_in, float* d_out,
 /* Total complex samples covered by a mixed-length plan (0 for NULL). */
 uint64_t wgfft_ragged_samples(const wgfft_ragged_t* p);
 
+/* The plan's kernel choice (tile prefetch, workgroups), made by timing on
+ * its first out-of-place call. */
+int wgfft_ragged_describe(const wgfft_ragged_t* p, char* desc, uint64_t desc_len);
+
 /* ---- measurement and tuning ------------------------------------------- */
 
 /* Device-only timing of wgfft_exec_device on resident buffers: `warmup`

@@ -382,10 +382,13 @@ static hipError_t ragged_launch_pf(wgfft_ragged_t* p, const float* in, float* ou
     return hipGetLastError();
 }
 
-// First out-of-place use: time both kernel variants over the whole queue
-// (each launch rewrites the same output from the untouched input) and keep
-// the faster; the grid follows the chosen variant's occupancy.  An in-place
-// first call keeps the default variant.
+// First out-of-place use: time the kernel variants (with and without the
+// big-unit tile prefetch) at each workgroups-per-CU count up to their
+// occupancy over the whole queue (each launch rewrites the same output from
+// the untouched input; best of three after a warm launch) and keep the
+// fastest.  Fewer workgroups keep less of the big units' scratch in flight,
+// which can then stay in the memory-side cache.  An in-place first call
+// keeps the default variant and full occupancy.
 static hipError_t ragged_choose(wgfft_ragged_t* p, const float* in, float* out, bool inv, hipStream_t s) {
     if (p->pf >= 0) return hipSuccess;
     if ((const void*)in == (const void*)out) {
@@ -393,7 +396,6 @@ static hipError_t ragged_choose(wgfft_ragged_t* p, const float* in, float* out, 
         p->grid = std::max(ragged_blocks_per_cu(false), 1) * p->h->num_cus;
         return hipSuccess;
     }
-    float ms[2] = {0.f, 0.f};
     hipEvent_t a, b;
     hipError_t e = hipEventCreate(&a);
     if (e != hipSuccess) return e;
@@ -401,20 +403,29 @@ static hipError_t ragged_choose(wgfft_ragged_t* p, const float* in, float* out, 
         hipEventDestroy(a);
         return e;
     }
+    float best = 1e30f;
     for (int pf = 0; pf < 2 && e == hipSuccess; ++pf) {
-        const int grid = std::max(ragged_blocks_per_cu(pf != 0), 1) * p->h->num_cus;
-        ragged_launch_pf(p, in, out, inv, pf != 0, 0, p->nunits, grid, s);  // warm
-        hipEventRecord(a, s);
-        ragged_launch_pf(p, in, out, inv, pf != 0, 0, p->nunits, grid, s);
-        hipEventRecord(b, s);
-        if ((e = hipEventSynchronize(b)) == hipSuccess) hipEventElapsedTime(&ms[pf], a, b);
+        const int occ = std::max(ragged_blocks_per_cu(pf != 0), 1);
+        for (int per_cu = occ; per_cu >= 1 && e == hipSuccess; --per_cu) {
+            const int grid = per_cu * p->h->num_cus;
+            ragged_launch_pf(p, in, out, inv, pf != 0, 0, p->nunits, grid, s);  // warm
+            for (int r = 0; r < 3 && e == hipSuccess; ++r) {
+                hipEventRecord(a, s);
+                ragged_launch_pf(p, in, out, inv, pf != 0, 0, p->nunits, grid, s);
+                hipEventRecord(b, s);
+                float ms = 0.f;
+                if ((e = hipEventSynchronize(b)) == hipSuccess) hipEventElapsedTime(&ms, a, b);
+                if (e == hipSuccess && ms < best) {
+                    best = ms;
+                    p->pf = pf;
+                    p->grid = grid;
+                }
+            }
+        }
     }
     hipEventDestroy(a);
     hipEventDestroy(b);
-    if (e != hipSuccess) return e;
-    p->pf = ms[1] < ms[0] ? 1 : 0;
-    p->grid = std::max(ragged_blocks_per_cu(p->pf != 0), 1) * p->h->num_cus;
-    return hipSuccess;
+    return e;
 }
 
 static hipError_t ragged_launch(wgfft_ragged_t* p, const float* in, float* out, bool inv, uint32_t u0, uint32_t u1,
@@ -479,3 +490,12 @@ extern "C" int wgfft_ragged_exec_device(wgfft_ragged_t* p, const float* d_in, fl
 }
 
 extern "C" uint64_t wgfft_ragged_samples(const wgfft_ragged_t* p) { return p ? p->total_samples : 0; }
+
+extern "C" int wgfft_ragged_describe(const wgfft_ragged_t* p, char* desc, uint64_t desc_len) {
+    if (!p) return fail(WGFFT_ERR_INVALID_ARGUMENT, "plan is NULL");
+    if (desc && desc_len)
+        std::snprintf(desc, desc_len, "mixed N=2^%d..2^%d units=%u tile-prefetch=%s workgroups=%d (%d per CU)",
+                      kMinLog2, kMaxLog2, p->nunits, p->pf < 0 ? "undecided" : p->pf ? "yes" : "no", p->grid,
+                      p->grid / std::max(p->h->num_cus, 1));
+    return WGFFT_OK;
+}

@@ -26,7 +26,7 @@ EXPORTS = (
     "wgfft_device_available", "wgfft_create", "wgfft_destroy", "wgfft_last_error", "wgfft_name",
     "wgfft_validate_lengths", "wgfft_exec_host", "wgfft_exec_device", "wgfft_exec_windowed_host",
     "wgfft_exec_windowed_device", "wgfft_vjp_windowed_device", "wgfft_fwd_adjoint_masked_device",
-    "wgfft_ragged_create", "wgfft_ragged_destroy", "wgfft_ragged_exec_device", "wgfft_ragged_samples",
+    "wgfft_ragged_create", "wgfft_ragged_destroy", "wgfft_ragged_exec_device", "wgfft_ragged_samples", "wgfft_ragged_describe",
     "wgfft_bench_device",
     "wgfft_autotune", "wgfft_plan_describe", "wgfft_variant_count", "wgfft_select_variant",
     "wgfft_pass_count", "wgfft_pass_variant_count", "wgfft_select_pass_variant",
@@ -86,6 +86,7 @@ def load() -> ctypes.CDLL:
     L.wgfft_ragged_exec_device.argtypes = [_vp, _vp, _vp, _i, _i, _vp]
     L.wgfft_ragged_samples.argtypes = [_vp]
     L.wgfft_ragged_samples.restype = _u64
+    L.wgfft_ragged_describe.argtypes = [_vp, ctypes.c_char_p, _u64]
     L.wgfft_bench_device.argtypes = [_vp, _vp, _vp, _u64, _u64, _i, _i, _i, ctypes.POINTER(ctypes.c_double)]
     L.wgfft_autotune.argtypes = [_vp, _u64, _u64, ctypes.c_char_p, _u64]
     L.wgfft_plan_describe.argtypes = [_vp, _u64, ctypes.c_char_p, _u64]

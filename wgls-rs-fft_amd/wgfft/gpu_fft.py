@@ -250,6 +250,12 @@ class RaggedPlan:
                                                     WGFFT_INVERSE if inverse else WGFFT_FORWARD, int(mode),
                                                     GpuFft._stream(stream)))
 
+    def describe(self) -> str:
+        """Kernel choice of the plan (wgfft_ragged_describe)."""
+        buf = ctypes.create_string_buffer(256)
+        check(self._gpu._L.wgfft_ragged_describe(self._p, buf, 256))
+        return buf.value.decode()
+
     def close(self) -> None:
         if getattr(self, "_p", None):
             self._gpu._L.wgfft_ragged_destroy(self._p)

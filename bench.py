@@ -15,6 +15,8 @@ The same JSON line also carries, measured in the same run:
   * mixed_batch: 20,000 transforms of lengths 2^k, k ~ U{6..16}, as one
     persistent launch, its hipGraph replay, and one launch per length;
   * large_n: N = 2^22 x 16 and N = 2^26 x 1 forward transforms;
+  * c1_and_host_api: N=1024 x 8 device latency, and the host-buffer API
+    (H2D + transform + D2H, PCIe-inclusive) at 1024 x 8 and 4096 x 4096;
   * cpu_baseline: the reference algorithm (the oracle's C restatement of the
     wgls-rs-fft f32 Stockham R4/R2 path) on the host cores, bounded sample.
 """
@@ -136,6 +138,27 @@ def bench_shapes(torch, gpu, steps: int, tune: bool) -> dict:
     return out
 
 
+def bench_c1_and_host(torch, gpu, wgfft) -> dict:
+    """C1 (N=1024 x 8, launch/latency-bound: device us per forward and
+    inverse call) and the host-buffer API (wgfft_exec_host: H2D + transform
+    + D2H, synchronous, PCIe-inclusive) at C1 and at 4096 x 4096."""
+    out = {}
+    x = torch.from_numpy(wgfft.synth_uniform((8, 1024), 0x5EED0001)).cuda()
+    y = torch.empty_like(x)
+    out["c1_device_us"] = {"fwd": round(1000 * timed(torch, lambda: gpu.exec_device(x, y), 2000, 100), 2),
+                           "inv": round(1000 * timed(torch, lambda: gpu.exec_device(y, x, inverse=True), 2000, 100), 2)}
+    for (n, batch, reps) in ((1024, 8, 200), (4096, 4096, 5)):
+        xh = wgfft.synth_uniform((batch, n), 0x5EED0002)
+        gpu.fft_array(xh)  # warm (plan, staging)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            gpu.fft_array(xh)
+        dt = (time.perf_counter() - t0) / reps
+        out[f"host_api_{n}x{batch}"] = {"ms": round(dt * 1e3, 4),
+                                       "gsamples_s": round(n * batch / dt / 1e9, 3)}
+    return out
+
+
 def bench_large(torch, gpu, steps: int, tune: bool) -> dict:
     out = {}
     for log2n, batch in ((22, 16), (26, 1)):
@@ -163,7 +186,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--headline-only", action="store_true", help="only the N=4096x4096 forward (profiling)")
-    ap.add_argument("--only", choices=("shapes", "mixed", "large"),
+    ap.add_argument("--only", choices=("shapes", "mixed", "large", "c1"),
                     help="profiling: skip the headline and run one secondary section")
     ap.add_argument("--no-tune", action="store_true", help="use the default kernel variants")
     args = ap.parse_args()
@@ -180,7 +203,8 @@ def main() -> int:
     if args.only:
         section = {"shapes": lambda: bench_shapes(torch, gpu, args.steps, not args.no_tune),
                    "mixed": lambda: bench_mixed(torch, gpu, wgfft, args.steps),
-                   "large": lambda: bench_large(torch, gpu, args.steps, not args.no_tune)}[args.only]
+                   "large": lambda: bench_large(torch, gpu, args.steps, not args.no_tune),
+                   "c1": lambda: bench_c1_and_host(torch, gpu, wgfft)}[args.only]
         print(json.dumps({args.only: section()}))
         gpu.close()
         return 0
@@ -239,6 +263,7 @@ def main() -> int:
         out["other_shapes"] = bench_shapes(torch, gpu, args.steps, not args.no_tune)
         out["mixed_batch"] = bench_mixed(torch, gpu, wgfft, args.steps)
         out["large_n"] = bench_large(torch, gpu, args.steps, not args.no_tune)
+        out["c1_and_host_api"] = bench_c1_and_host(torch, gpu, wgfft)
         del x_keep, y_keep
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

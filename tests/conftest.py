@@ -29,3 +29,22 @@ def torch_cuda():
 
     assert torch.cuda.is_available(), "GPU tests need a HIP device"
     return torch
+
+
+_PARITY = {}
+
+
+@pytest.fixture(scope="session")
+def parity():
+    """Collects measured errors per configuration; written as JSON to the
+    path in WGFFT_PARITY_REPORT (if set) when the session ends."""
+    return _PARITY
+
+
+def pytest_sessionfinish(session, exitstatus):
+    path = os.environ.get("WGFFT_PARITY_REPORT")
+    if path and _PARITY:
+        import json
+
+        with open(path, "w") as f:
+            json.dump(_PARITY, f, indent=1, sort_keys=True, default=float)

@@ -254,26 +254,34 @@ def test_alloc_cap_out_of_memory(gpu):
 
 # ------------------------------------------------ full-size configurations
 @pytest.mark.parametrize("n,batch", [(4096, 4096), (256, 65536), (16384, 1024)])
-def test_headline_shapes_vs_fp64(gpu, n, batch):
+def test_headline_shapes_vs_fp64(gpu, parity, n, batch):
     x = wgfft.synth_uniform((batch, n), 0x5EED0002)
     y = gpu.fft_array(x)
     err = oracle.rel_l2(y, oracle.fft64(x))
     print(f"N={n} batch={batch} fwd rel-L2 vs FP64 = {err:.3e}")
     assert err <= 5e-6
     rt = gpu.fft_array(y, inverse=True)
-    assert oracle.rel_l2(rt, x) <= 5e-6
+    e_rt = oracle.rel_l2(rt, x)
+    parity[f"C2 {n}x{batch}"] = {"fwd_rel_l2_vs_fp64": err, "round_trip_rel_l2": e_rt, "bar": 5e-6}
+    assert e_rt <= 5e-6
 
 
-def test_c1_forward_then_inverse(gpu):
+def test_c1_forward_then_inverse(gpu, parity):
     x = wgfft.synth_uniform((8, 1024), 0x5EED0001)
     y = gpu.fft_array(x)
-    assert oracle.rel_l2(y, oracle.fft64(x)) <= 5e-6
+    e_f = oracle.rel_l2(y, oracle.fft64(x))
     r = gpu.fft_array(y, inverse=True)
-    assert oracle.rel_l2(r, oracle.ifft64(y.astype(np.complex128))) <= 5e-6
+    e_i = oracle.rel_l2(r, oracle.ifft64(y.astype(np.complex128)))
+    ref_diff = float(np.abs(y - oracle.ref_fft_f32(x)).max())
+    parity["C1 1024x8"] = {"fwd_rel_l2_vs_fp64": e_f, "inv_rel_l2_vs_fp64": e_i,
+                           "round_trip_max_abs": float(np.abs(r - x).max()),
+                           "fwd_max_abs_vs_reference_restatement": ref_diff, "bar": 5e-6}
+    assert e_f <= 5e-6 and e_i <= 5e-6
+    assert ref_diff < 1e-3
     assert np.abs(r - x).max() < 1e-4
 
 
-def test_c3_large_n_2pow22(gpu):
+def test_c3_large_n_2pow22(gpu, parity):
     n, batch = 1 << 22, 16
     x = oracle.synth_sinusoids(n, batch, 0x5EED0003)
     y = gpu.fft_array(x)
@@ -281,22 +289,30 @@ def test_c3_large_n_2pow22(gpu):
     print(f"N=2^22 batch=16 fwd rel-L2 vs FP64 = {err:.3e}")
     assert err <= 2e-5
     r = gpu.fft_array(y, inverse=True)
-    assert oracle.rel_l2(r, x) <= 2e-5
+    e_rt = oracle.rel_l2(r, x)
+    parity["C3 2^22x16"] = {"fwd_rel_l2_vs_fp64": err, "round_trip_rel_l2": e_rt, "bar": 2e-5,
+                            "plan": gpu.describe(n)}
+    assert e_rt <= 2e-5
 
 
-def test_c3_large_n_2pow26(gpu):
+def test_c3_large_n_2pow26(gpu, parity):
     n = 1 << 26
     x = wgfft.synth_uniform((1, n), 0x5EED0013)
     y = gpu.fft_array(x)
     bins = np.array([0, 1, 12345, n // 2, n - 1, 33554431 - 777], dtype=np.uint64)
     want = oracle.dft_bins64(x[0], bins)
     scale = np.sqrt(np.sum(np.abs(x.astype(np.complex128)) ** 2))  # |Y_k| ~ ||x||
-    assert np.abs(y[0, bins.astype(np.int64)] - want).max() / scale < 2e-5
+    e_bins = float(np.abs(y[0, bins.astype(np.int64)] - want).max() / scale)
+    assert e_bins < 2e-5
     # Parseval and round trip over the whole transform
     py = np.sum(np.abs(y.astype(np.complex128)) ** 2)
-    assert abs(py / (n * scale**2) - 1) < 1e-5
+    e_pars = float(abs(py / (n * scale**2) - 1))
+    assert e_pars < 1e-5
     r = gpu.fft_array(y, inverse=True)
-    assert oracle.rel_l2(r, x) <= 2e-5
+    e_rt = oracle.rel_l2(r, x)
+    parity["C3 2^26x1"] = {"bins_max_err_over_norm_vs_fp64_dft": e_bins, "parseval_rel_err": e_pars,
+                           "round_trip_rel_l2": e_rt, "bar": 2e-5, "plan": gpu.describe(n)}
+    assert e_rt <= 2e-5
 
 
 # ------------------------------------------------------- device entry points
@@ -320,7 +336,7 @@ def test_device_api_inplace_and_stream(gpu, torch_cuda):
     assert oracle.rel_l2(z.cpu().numpy(), x) <= 5e-6
 
 
-def test_windowed_device_c5_shape(gpu, torch_cuda):
+def test_windowed_device_c5_shape(gpu, torch_cuda, parity):
     torch = torch_cuda
     n, batch = 8192, 1024
     x = wgfft.synth_uniform((batch, n), 0x5EED0004)
@@ -329,4 +345,6 @@ def test_windowed_device_c5_shape(gpu, torch_cuda):
     ty = torch.empty_like(tx)
     gpu.exec_windowed_device(tx, tw, ty)
     torch.cuda.synchronize()
-    assert oracle.rel_l2(ty.cpu().numpy(), oracle.windowed_fwd64(x, w)) <= 5e-6
+    err = oracle.rel_l2(ty.cpu().numpy(), oracle.windowed_fwd64(x, w))
+    parity["C5 8192x1024 windowed forward"] = {"rel_l2_vs_fp64": err, "bar": 5e-6}
+    assert err <= 5e-6

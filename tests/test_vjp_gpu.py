@@ -23,7 +23,7 @@ def c5(torch_cuda):
     return dict(x=x, w=w, m=m, dev=dev, loss64=loss64, gy64=gy64, gx64=gx64, gw64=gw64)
 
 
-def test_vjp_entry_vs_fp64_adjoint(gpu, torch_cuda, c5):
+def test_vjp_entry_vs_fp64_adjoint(gpu, torch_cuda, c5, parity):
     torch = torch_cuda
     gy = c5["gy64"].astype(np.complex64)
     tgy = torch.from_numpy(gy).cuda()
@@ -34,10 +34,11 @@ def test_vjp_entry_vs_fp64_adjoint(gpu, torch_cuda, c5):
     e_gx = oracle.rel_l2(gx.cpu().numpy(), c5["gx64"])
     e_gw = np.linalg.norm(gw.cpu().numpy() - c5["gw64"]) / np.linalg.norm(c5["gw64"])
     print(f"VJP rel-L2: gx {e_gx:.3e} gw {e_gw:.3e}")
+    parity["C5 VJP entry vs FP64 adjoint"] = {"gx_rel_l2": e_gx, "gw_rel_l2": e_gw, "bars": [5e-6, 1e-5]}
     assert e_gx <= 5e-6 and e_gw <= 1e-5
 
 
-def test_fused_fwd_adjoint_vs_fp64(gpu, torch_cuda, c5):
+def test_fused_fwd_adjoint_vs_fp64(gpu, torch_cuda, c5, parity):
     torch = torch_cuda
     from wgfft.autograd import masked_power_loss
 
@@ -50,10 +51,12 @@ def test_fused_fwd_adjoint_vs_fp64(gpu, torch_cuda, c5):
     e_gx = oracle.rel_l2(gx.cpu().numpy(), gx64)
     e_gw = np.linalg.norm(gw.cpu().numpy() - gw64) / np.linalg.norm(gw64)
     print(f"fused: loss {e_l:.3e} Y {e_y:.3e} gx {e_gx:.3e} gw {e_gw:.3e}")
+    parity["C5 fused forward+adjoint vs FP64"] = {"loss_rel": e_l, "y_rel_l2": e_y, "gx_rel_l2": e_gx,
+                                                 "gw_rel_l2": e_gw, "bars": [1e-5, 5e-6, 1e-5, 1e-5]}
     assert e_l <= 1e-5 and e_y <= 5e-6 and e_gx <= 1e-5 and e_gw <= 1e-5
 
 
-def test_fused_vs_finite_differences(gpu, torch_cuda, c5):
+def test_fused_vs_finite_differences(gpu, torch_cuda, c5, parity):
     """64 coordinates: 48 of x (re/im) and 16 of w; FP64 central differences
     of the FP64 loss (numpy.fft) against the GPU's f32 gradients.  L is
     exactly quadratic in x and in w, so the central difference has no
@@ -77,6 +80,7 @@ def test_fused_vs_finite_differences(gpu, torch_cuda, c5):
 
     rng = np.random.default_rng(0x5EED)
     h = 1e-2
+    worst_x = worst_w = 0.0  # max |fd - analytic| / gradient RMS
     for _ in range(24):
         b, n = int(rng.integers(B)), int(rng.integers(N))
         scale = float(np.sqrt(np.mean(np.abs(gx[b]) ** 2)))
@@ -86,6 +90,7 @@ def test_fused_vs_finite_differences(gpu, torch_cuda, c5):
             xm[n] -= h * part
             fd = (row_loss(xp, w64) - row_loss(xm, w64)) / (2 * h)
             an = gx[b, n].real if part == 1.0 else gx[b, n].imag
+            worst_x = max(worst_x, abs(fd - an) / scale)
             assert abs(fd - an) <= 1e-5 * scale + 1e-4 * abs(fd), (b, n, part, fd, an, scale)
     scale_w = float(np.sqrt(np.mean(gw**2)))
     for _ in range(16):
@@ -97,7 +102,10 @@ def test_fused_vs_finite_differences(gpu, torch_cuda, c5):
             y = np.fft.fft(wv[None] * x64, axis=-1)
             losses.append(float(np.sum(m64[None] * np.abs(y) ** 2)))
         fd = (losses[0] - losses[1]) / (2 * h)
+        worst_w = max(worst_w, abs(fd - gw[n]) / scale_w)
         assert abs(fd - gw[n]) <= 1e-5 * scale_w + 1e-4 * abs(fd), (n, fd, gw[n], scale_w)
+    parity["C5 fused gradients vs FP64 central differences (64 coordinates)"] = {
+        "gx_max_abs_err_over_rms": worst_x, "gw_max_abs_err_over_rms": worst_w}
 
 
 def test_autograd_harness(gpu, torch_cuda):

@@ -36,6 +36,7 @@ struct BatchedArgs {
 constexpr int kFlagNtStore = 1;  // non-temporal output stores
 constexpr int kFlagTwRegs = 2;   // twiddle base powers held in registers across transforms
 constexpr int kFlagTwLds = 4;    // twiddle base powers from two short LDS tables
+constexpr int kFlagNoPrefetch = 8;  // no register prefetch of the next transform (fewer registers)
 // bits 4..7: minimum waves per SIMD the register allocation must allow (0: compiler's choice)
 constexpr int flag_min_waves(int fl) { return (fl >> 4) & 15; }
 
@@ -223,22 +224,29 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
         twl = twlds_fill<N>(lds_tw, tw, tid, G * T);
         __syncthreads();
     }
+    constexpr bool PREFETCH = (FL & kFlagNoPrefetch) == 0;
     uint64_t grp = blockIdx.x;
-    float2 nxt[E];
-    if (grp < ngroups && grp * G + g < batch) IO::load(nxt, in + (grp * G + g) * N, t);
+    float2 nxt[PREFETCH ? E : 1];
+    if constexpr (PREFETCH) {
+        if (grp < ngroups && grp * G + g < batch) IO::load(nxt, in + (grp * G + g) * N, t);
+    }
 
     for (; grp < ngroups; grp += gridDim.x) {
         const uint64_t tr = grp * G + g;
         const bool active = tr < batch;
         float2 d[E];
-#pragma unroll
-        for (int i = 0; i < E; ++i) d[i] = nxt[i];
         // Loads that this transform waits on are issued before the prefetch
         // (vmcnt retires in order).
         float w[E];
         if constexpr (MODE == kWindowed) IO::load_real(w, win, t);
-        const uint64_t ngrp = grp + gridDim.x;
-        if (ngrp < ngroups && ngrp * G + g < batch) IO::load(nxt, in + (ngrp * G + g) * N, t);
+        if constexpr (PREFETCH) {
+#pragma unroll
+            for (int i = 0; i < E; ++i) d[i] = nxt[i];
+            const uint64_t ngrp = grp + gridDim.x;
+            if (ngrp < ngroups && ngrp * G + g < batch) IO::load(nxt, in + (ngrp * G + g) * N, t);
+        } else {
+            if (active) IO::load(d, in + tr * N, t);
+        }
         if constexpr (MODE == kWindowed) {
 #pragma unroll
             for (int i = 0; i < E; ++i) d[i] = cscale(d[i], w[i]);

@@ -18,3 +18,6 @@ WGFFT_REGISTER_BATCHED_F(4096, 16, 1, 4, 16x16x16, 16, 16, 16);
 WGFFT_REGISTER_BATCHED_F(4096, 16, 1, 5, 16x16x16, 16, 16, 16);
 WGFFT_REGISTER_BATCHED_F(4096, 32, 1, 5, 16x16x16, 16, 16, 16);
 WGFFT_REGISTER_BATCHED_F(4096, 16, 1, 5, 8x8x8x8, 8, 8, 8, 8);
+WGFFT_REGISTER_BATCHED_F(4096, 16, 1, 9, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_BATCHED_F(4096, 32, 1, 9, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_BATCHED_F(4096, 32, 2, 9, 16x16x16, 16, 16, 16);

@@ -13,3 +13,10 @@ WGFFT_REGISTER_BATCHED(256, 16, 16, 8x4x8, 8, 4, 8);
 WGFFT_REGISTER_BATCHED(256, 16, 16, 16x16, 16, 16);
 WGFFT_REGISTER_BATCHED(512, 32, 16, 16x16x2, 16, 16, 2);
 WGFFT_REGISTER_BATCHED(512, 16, 8, 8x8x8, 8, 8, 8);
+
+// Without the register prefetch (flag 8; 9 adds non-temporal stores): fewer
+// registers, more resident waves to hide the loads instead.
+WGFFT_REGISTER_BATCHED_F(256, 16, 16, 9, 16x16, 16, 16);
+WGFFT_REGISTER_BATCHED_F(256, 32, 32, 9, 16x16, 16, 16);
+WGFFT_REGISTER_BATCHED_F(512, 32, 16, 9, 16x16x2, 16, 16, 2);
+WGFFT_REGISTER_BATCHED_F(512, 16, 8, 9, 8x8x8, 8, 8, 8);

@@ -85,12 +85,35 @@ hipError_t launch_small(Mode mode, int log2n, const BatchedArgs& a);
 
 // ---------------------------------------------------------------------------
 // Global I/O in the pair layout: d[2q + delta] = X[2t + delta + 2T q].
-template <int N, int E, int T>
+// 16-byte load, optionally non-temporal (streamed data read once).
+template <bool NT>
+__device__ __forceinline__ float4 ld16(const float4* p) {
+    if constexpr (NT) {
+        typedef float f4v __attribute__((ext_vector_type(4)));
+        const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
+        return make_float4(v.x, v.y, v.z, v.w);
+    } else {
+        return *p;
+    }
+}
+// 16-byte store, optionally non-temporal (streamed data not re-read soon).
+template <bool NT>
+__device__ __forceinline__ void st16(float4* p, float4 v) {
+    if constexpr (NT) {
+        typedef float f4v __attribute__((ext_vector_type(4)));
+        const f4v nv = {v.x, v.y, v.z, v.w};
+        __builtin_nontemporal_store(nv, reinterpret_cast<f4v*>(p));
+    } else {
+        *p = v;
+    }
+}
+
+template <int N, int E, int T, bool NT = false>
 __device__ __forceinline__ void load_pairs(float2 (&d)[E], const float2* __restrict__ src, uint32_t t) {
     const float4* s4 = reinterpret_cast<const float4*>(src);
 #pragma unroll
     for (int q = 0; q < E / 2; ++q) {
-        float4 v = s4[t + T * q];
+        float4 v = ld16<NT>(s4 + t + T * q);
         d[2 * q] = make_float2(v.x, v.y);
         d[2 * q + 1] = make_float2(v.z, v.w);
     }
@@ -141,9 +164,10 @@ struct Io {
         if constexpr (PAIR) return 2 * t + (s & 1) + 2 * T * (s >> 1);
         else return t + T * (s / R) + (s % R) * (N / R);
     }
+    template <bool NT = false>
     __device__ __forceinline__ static void load(float2 (&d)[E], const float2* __restrict__ src, uint32_t t) {
         if constexpr (PAIR) {
-            load_pairs<N, E, T>(d, src, t);
+            load_pairs<N, E, T, NT>(d, src, t);
         } else {
 #pragma unroll
             for (int s = 0; s < E; ++s) d[s] = src[elem(s, t)];
@@ -385,7 +409,7 @@ __device__ __forceinline__ void gp_load_side(GpSide<TW, E, MODE>& sd, const TW& 
     }
 }
 
-template <int R, int E, int G>
+template <int R, int E, int G, bool NT = false>
 __device__ __forceinline__ void gp_load(float2 (&v)[E], const float2* __restrict__ src, uint64_t NR, uint64_t b0,
                                         uint32_t tid) {
     constexpr int THREADS = G * (R / E);
@@ -394,13 +418,14 @@ __device__ __forceinline__ void gp_load(float2 (&v)[E], const float2* __restrict
         const int it = tid + THREADS * q;
         const int gp = it % (G / 2);
         const int m = it / (G / 2);
-        float4 x = *reinterpret_cast<const float4*>(src + b0 + 2 * gp + (uint64_t)m * NR);
+        float4 x = ld16<NT>(reinterpret_cast<const float4*>(src + b0 + 2 * gp + (uint64_t)m * NR));
         v[2 * q] = make_float2(x.x, x.y);
         v[2 * q + 1] = make_float2(x.z, x.w);
     }
 }
 
-template <int R, int E, int G, int MODE, class TW, class TS, int... Rs>
+// NTS: the results are stored non-temporally.
+template <int R, int E, int G, int MODE, class TW, class TS, bool NTS, int... Rs>
 __device__ __forceinline__ void gp_compute(float2 (&v)[E], const GpSide<TW, E, MODE>& sd, float2* __restrict__ dst,
                                            const TS& ts, uint64_t N, uint64_t L, uint64_t b0, float sc,
                                            float2* __restrict__ lds_all, uint32_t tid) {
@@ -457,7 +482,7 @@ __device__ __forceinline__ void gp_compute(float2 (&v)[E], const GpSide<TW, E, M
 
     if (first) {
         // output b R + m: R contiguous elements per butterfly
-        store_pairs<R, E, T, true>(dst + (b0 + g) * R, d, t, sc);
+        store_pairs<R, E, T, true, NTS>(dst + (b0 + g) * R, d, t, sc);
         return;
     }
     WGFFT_LDS_SYNC();
@@ -477,7 +502,7 @@ __device__ __forceinline__ void gp_compute(float2 (&v)[E], const GpSide<TW, E, M
         float2 a = cscale(lds_all[(2 * gp) * SLOT + pm], sc);
         float2 b = cscale(lds_all[(2 * gp + 1) * SLOT + pm], sc);
         const uint64_t o = bj * R * L + k0base + 2 * gp + (uint64_t)m * L;
-        *reinterpret_cast<float4*>(dst + o) = make_float4(a.x, a.y, b.x, b.y);
+        st16<NTS>(reinterpret_cast<float4*>(dst + o), make_float4(a.x, a.y, b.x, b.y));
     }
 }
 
@@ -531,7 +556,7 @@ k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const flo
                 gp_load_side<R, E, G, MODE>(nsd, twN, win, N, L, nb0, tid);
                 gp_load<R, E, G>(nxt, in + ntr * N, NR, nb0, tid);
             }
-            gp_compute<R, E, G, MODE, TwTwoLevel, TwLds, Rs...>(cur, sd, out + tr * N, twr, N, L, b0,
+            gp_compute<R, E, G, MODE, TwTwoLevel, TwLds, false, Rs...>(cur, sd, out + tr * N, twr, N, L, b0,
                                                                 last ? scale : 1.0f, lds_all, tid);
 #pragma unroll
             for (int i = 0; i < E; ++i) cur[i] = nxt[i];
@@ -539,7 +564,7 @@ k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const flo
         } else {
             gp_load_side<R, E, G, MODE>(sd, twN, win, N, L, b0, tid);
             gp_load<R, E, G>(cur, in + tr * N, NR, b0, tid);
-            gp_compute<R, E, G, MODE, TwTwoLevel, TwLds, Rs...>(cur, sd, out + tr * N, twr, N, L, b0,
+            gp_compute<R, E, G, MODE, TwTwoLevel, TwLds, false, Rs...>(cur, sd, out + tr * N, twr, N, L, b0,
                                                                 last ? scale : 1.0f, lds_all, tid);
         }
         WGFFT_LDS_SYNC();

@@ -34,6 +34,9 @@ struct RaggedUnit {
 };
 
 constexpr int kRaggedThreads = 256;
+// Inputs and outputs are streamed non-temporally (read and written once), so
+// they displace less of the cached big-unit scratch.
+constexpr bool NTIO = true;
 constexpr int kMinLog2 = 6, kMaxLog2 = 16, kDirectMaxLog2 = 13;
 
 // elements per thread of the direct (LDS-resident) unit for length 2^k
@@ -69,9 +72,9 @@ __device__ __forceinline__ void ragged_direct(const float2* __restrict__ in, flo
     const bool active = g < count;
     const uint64_t off = active ? offs[first + g] : 0;
     float2 d[E];
-    if (active) Io<N, E, Rs...>::load(d, in + off, t);
+    if (active) Io<N, E, Rs...>::template load<NTIO>(d, in + off, t);
     Engine<N, E, T, INV>::run_with(d, lds_raw + g * LEN, tl.shifted(kMaxLog2 - ilog2(N)), t, Radices<Rs...>{});
-    if (active) Io<N, E, Rs...>::template store<INV>(out + off, d, t, 1.0f / (float)N);
+    if (active) Io<N, E, Rs...>::template store<INV, NTIO>(out + off, d, t, 1.0f / (float)N);
 }
 
 // Two-pass transform of one length-N signal (N = R1 * R2) by one workgroup:
@@ -99,19 +102,19 @@ struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV, PF
         // pass 1 (L = 1): src -> scratch (no twiddles, so no side loads)
         float2 cur[E1];
         Side sd;
-        if (PF) gp_load<R1, E1, G1>(cur, src, N / R1, 0, tid);
+        if (PF) gp_load<R1, E1, G1, NTIO>(cur, src, N / R1, 0, tid);
 #pragma unroll 1
         for (uint64_t b0 = 0; b0 < N / R1; b0 += G1) {
             if constexpr (PF) {
                 float2 nxt[E1];
-                if (b0 + G1 < N / R1) gp_load<R1, E1, G1>(nxt, src, N / R1, b0 + G1, tid);
-                gp_compute<R1, E1, G1, M, TwOneLevel, TwLds, Rs1...>(cur, sd, scratch, t1, N, 1, b0, 1.0f, lds,
+                if (b0 + G1 < N / R1) gp_load<R1, E1, G1, NTIO>(nxt, src, N / R1, b0 + G1, tid);
+                gp_compute<R1, E1, G1, M, TwOneLevel, TwLds, false, Rs1...>(cur, sd, scratch, t1, N, 1, b0, 1.0f, lds,
                                                                      tid);
 #pragma unroll
                 for (int i = 0; i < E1; ++i) cur[i] = nxt[i];
             } else {
-                gp_load<R1, E1, G1>(cur, src, N / R1, b0, tid);
-                gp_compute<R1, E1, G1, M, TwOneLevel, TwLds, Rs1...>(cur, sd, scratch, t1, N, 1, b0, 1.0f, lds,
+                gp_load<R1, E1, G1, NTIO>(cur, src, N / R1, b0, tid);
+                gp_compute<R1, E1, G1, M, TwOneLevel, TwLds, false, Rs1...>(cur, sd, scratch, t1, N, 1, b0, 1.0f, lds,
                                                                      tid);
             }
             WGFFT_LDS_SYNC();
@@ -132,14 +135,14 @@ struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV, PF
                     gp_load_side<R2, E2, G2, M>(nsd, twN, nullptr, N, R1, b0 + G2, tid);
                     gp_load<R2, E2, G2>(nxt, scratch, N / R2, b0 + G2, tid);
                 }
-                gp_compute<R2, E2, G2, M, TwOneLevel, TwLds, Rs2...>(cur, sd, dst, t2, N, R1, b0, sc, lds, tid);
+                gp_compute<R2, E2, G2, M, TwOneLevel, TwLds, NTIO, Rs2...>(cur, sd, dst, t2, N, R1, b0, sc, lds, tid);
 #pragma unroll
                 for (int i = 0; i < E2; ++i) cur[i] = nxt[i];
                 sd = nsd;
             } else {
                 gp_load_side<R2, E2, G2, M>(sd, twN, nullptr, N, R1, b0, tid);
                 gp_load<R2, E2, G2>(cur, scratch, N / R2, b0, tid);
-                gp_compute<R2, E2, G2, M, TwOneLevel, TwLds, Rs2...>(cur, sd, dst, t2, N, R1, b0, sc, lds, tid);
+                gp_compute<R2, E2, G2, M, TwOneLevel, TwLds, NTIO, Rs2...>(cur, sd, dst, t2, N, R1, b0, sc, lds, tid);
             }
             WGFFT_LDS_SYNC();
         }

@@ -506,7 +506,8 @@ __device__ __forceinline__ void gp_compute(float2 (&v)[E], const GpSide<TW, E, M
     }
 }
 
-// Persistent multi-pass kernel: workgroups loop over the groups of all
+// Persistent multi-pass kernel (tiles read and written non-temporally: each
+// element is touched once per pass): workgroups loop over the groups of all
 // transforms in the batch with the next group's tile prefetched into
 // registers while the current one is transformed.  Consecutive workgroup ids
 // are dealt round-robin over the 8 XCDs; the logical numbering makes the
@@ -540,7 +541,7 @@ k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const flo
     if (PF && lid0 < total) {
         const uint64_t tr = lid0 / groups_per_tr, b0 = (lid0 % groups_per_tr) * G;
         gp_load_side<R, E, G, MODE>(sd, twN, win, N, L, b0, threadIdx.x);
-        gp_load<R, E, G>(cur, in + tr * N, NR, b0, threadIdx.x);
+        gp_load<R, E, G, true>(cur, in + tr * N, NR, b0, threadIdx.x);
     }
     for (uint64_t lid = lid0; lid < total; lid += nb) {
         const uint64_t tr = lid / groups_per_tr, b0 = (lid % groups_per_tr) * G;
@@ -554,18 +555,18 @@ k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const flo
             if (nl < total) {
                 const uint64_t ntr = nl / groups_per_tr, nb0 = (nl % groups_per_tr) * G;
                 gp_load_side<R, E, G, MODE>(nsd, twN, win, N, L, nb0, tid);
-                gp_load<R, E, G>(nxt, in + ntr * N, NR, nb0, tid);
+                gp_load<R, E, G, true>(nxt, in + ntr * N, NR, nb0, tid);
             }
-            gp_compute<R, E, G, MODE, TwTwoLevel, TwLds, false, Rs...>(cur, sd, out + tr * N, twr, N, L, b0,
-                                                                last ? scale : 1.0f, lds_all, tid);
+            gp_compute<R, E, G, MODE, TwTwoLevel, TwLds, true, Rs...>(cur, sd, out + tr * N, twr, N, L, b0,
+                                                               last ? scale : 1.0f, lds_all, tid);
 #pragma unroll
             for (int i = 0; i < E; ++i) cur[i] = nxt[i];
             sd = nsd;
         } else {
             gp_load_side<R, E, G, MODE>(sd, twN, win, N, L, b0, tid);
-            gp_load<R, E, G>(cur, in + tr * N, NR, b0, tid);
-            gp_compute<R, E, G, MODE, TwTwoLevel, TwLds, false, Rs...>(cur, sd, out + tr * N, twr, N, L, b0,
-                                                                last ? scale : 1.0f, lds_all, tid);
+            gp_load<R, E, G, true>(cur, in + tr * N, NR, b0, tid);
+            gp_compute<R, E, G, MODE, TwTwoLevel, TwLds, true, Rs...>(cur, sd, out + tr * N, twr, N, L, b0,
+                                                               last ? scale : 1.0f, lds_all, tid);
         }
         WGFFT_LDS_SYNC();
     }

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <map>
 #include <numeric>
@@ -407,6 +408,7 @@ static hipError_t ragged_choose(wgfft_ragged_t* p, const float* in, float* out, 
         return e;
     }
     float best = 1e30f;
+    const bool trace = std::getenv("WGFFT_RAGGED_TRACE") != nullptr;  // print the timings to stderr
     for (int pf = 0; pf < 2 && e == hipSuccess; ++pf) {
         const int occ = std::max(ragged_blocks_per_cu(pf != 0), 1);
         for (int per_cu = occ; per_cu >= 1 && e == hipSuccess; --per_cu) {
@@ -418,6 +420,8 @@ static hipError_t ragged_choose(wgfft_ragged_t* p, const float* in, float* out, 
                 hipEventRecord(b, s);
                 float ms = 0.f;
                 if ((e = hipEventSynchronize(b)) == hipSuccess) hipEventElapsedTime(&ms, a, b);
+                if (trace && e == hipSuccess)
+                    std::fprintf(stderr, "wgfft ragged: prefetch=%d workgroups/CU=%d %.4f ms\n", pf, per_cu, ms);
                 if (e == hipSuccess && ms < best) {
                     best = ms;
                     p->pf = pf;

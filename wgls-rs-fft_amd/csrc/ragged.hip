@@ -409,6 +409,19 @@ static hipError_t ragged_choose(wgfft_ragged_t* p, const float* in, float* out, 
     }
     float best = 1e30f;
     const bool trace = std::getenv("WGFFT_RAGGED_TRACE") != nullptr;  // print the timings to stderr
+    // settle the shader clock (it ramps up over tens of milliseconds of load)
+    // so the candidates timed first are not penalised
+    {
+        const int grid = std::max(ragged_blocks_per_cu(false), 1) * p->h->num_cus;
+        float ms = 0.f, total = 0.f;
+        for (int r = 0; r < 64 && total < 30.f && e == hipSuccess; ++r) {
+            hipEventRecord(a, s);
+            ragged_launch_pf(p, in, out, inv, false, 0, p->nunits, grid, s);
+            hipEventRecord(b, s);
+            if ((e = hipEventSynchronize(b)) == hipSuccess) hipEventElapsedTime(&ms, a, b);
+            total += ms;
+        }
+    }
     for (int pf = 0; pf < 2 && e == hipSuccess; ++pf) {
         const int occ = std::max(ragged_blocks_per_cu(pf != 0), 1);
         for (int per_cu = occ; per_cu >= 1 && e == hipSuccess; --per_cu) {

@@ -530,6 +530,15 @@ struct Engine {
         static_assert(RadixInfo<Rs...>::product == N, "radices must multiply to N");
         passes<IoKind<E, Rs...>::PAIR, TS, 1, 30, 0, 0, true, Rs...>(d, lds, ts, t);
     }
+    // The same in the single layout whatever E is: slot i R + m holds element
+    // t + T i + m N/R on entry and on exit (R = first = last radix).
+    template <class TS, int... Rs>
+    __device__ __forceinline__ static void run_single(float2 (&d)[E], float2* __restrict__ lds, const TS& ts,
+                                                      uint32_t t, Radices<Rs...>) {
+        static_assert(RadixInfo<Rs...>::product == N, "radices must multiply to N");
+        static_assert(RadixInfo<Rs...>::first == RadixInfo<Rs...>::last, "single layout needs first = last radix");
+        passes<false, TS, 1, 30, 0, 0, true, Rs...>(d, lds, ts, t);
+    }
 };
 
 }  // namespace wgfft

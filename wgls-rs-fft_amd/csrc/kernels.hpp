@@ -506,6 +506,250 @@ __device__ __forceinline__ void gp_compute(float2 (&v)[E], const GpSide<TW, E, M
     }
 }
 
+// ---------------------------------------------------------------------------
+// Global pass with direct tile I/O: no LDS staging.  Thread (g, t) of the
+// workgroup, g = tid mod G (column fastest), t = tid / G, holds slot
+// i RF + m = row t + T i + m R/RF of column g in the single layout of its
+// radix-R sub-FFT (RF its first and last radix).  A wavefront's 8-byte loads
+// of one slot are 64/G rows x G adjacent columns, i.e. whole G*8-byte column
+// runs; for L > 1 the results are stored the same way (rows m L apart), so a
+// tile costs the sub-FFT's own exchanges only.  For L = 1 each column's R
+// outputs are contiguous: they go through one LDS transposition and leave as
+// contiguous runs.
+template <int R, int E, int G, int... Rs>
+struct GdMap {
+    static constexpr int T = R / E;
+    static constexpr int RF = RadixInfo<Rs...>::first;
+    static constexpr int NG = E / RF;  // sub-FFT butterflies per thread in the first and last pass
+    static_assert(RF == RadixInfo<Rs...>::last, "direct global passes need first radix = last radix");
+    static_assert(E % RF == 0, "radix must divide E");
+    __device__ __forceinline__ static uint32_t row(int s, uint32_t t) {
+        return t + T * (s / RF) + (s % RF) * (R / RF);
+    }
+};
+
+// LDS slot (complex elements) per column of a direct global pass: the
+// sub-FFT's exchange footprint or the transposition (R + R/32), kept 2 mod 32
+// so the G columns of a wavefront start 16 bytes apart (all 64 banks).
+template <int R, int E, int... Rs>
+struct GdSlot {
+    static constexpr int a = LdsLayout<R, E, Rs...>::len;
+    static constexpr int b = R + (R >> 5);
+    static constexpr int m = a > b ? a : b;
+    static constexpr int len = m + ((2 - m % 32) + 32) % 32;
+};
+
+// Twiddle bases of one tile: W_N^{row k s} for row = t + T i + (R/RF) m is
+// u (W^{T k s})^i (W^{(R/RF) k s})^m with u = W^{t k s}; the windowed first
+// pass needs the window at the thread's E elements.
+template <class TW, int E, int MODE>
+struct GdSide {
+    typename TW::raw_t u, di, dm;
+    float w[MODE == kWindowed ? E : 1];
+};
+
+template <int R, int E, int G, int MODE, class TW, int... Rs>
+__device__ __forceinline__ void gd_load_side(GdSide<TW, E, MODE>& sd, const TW& twN, const float* __restrict__ win,
+                                             uint64_t N, uint64_t L, uint64_t b0, uint32_t tid) {
+    using MP = GdMap<R, E, G, Rs...>;
+    const uint32_t g = tid % G, t = tid / G;
+    if (L > 1) {
+        const uint64_t s = N / ((uint64_t)R * L);
+        const uint64_t k = (b0 + g) & (L - 1);
+        sd.u = twN.raw(((uint64_t)t * k * s) & (N - 1));
+        sd.di = twN.raw(((uint64_t)MP::T * k * s) & (N - 1));
+        sd.dm = twN.raw(((uint64_t)(R / MP::RF) * k * s) & (N - 1));
+    } else if constexpr (MODE == kWindowed) {
+        const uint64_t NR = N / R;
+#pragma unroll
+        for (int e = 0; e < E; ++e) sd.w[e] = win[b0 + g + (uint64_t)MP::row(e, t) * NR];
+    }
+}
+
+template <int R, int E, int G, bool NT, int... Rs>
+__device__ __forceinline__ void gd_load(float2 (&v)[E], const float2* __restrict__ src, uint64_t NR, uint64_t b0,
+                                        uint32_t tid) {
+    using MP = GdMap<R, E, G, Rs...>;
+    const uint32_t g = tid % G, t = tid / G;
+    const float2* base = src + b0 + g;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const float2* p = base + (uint64_t)MP::row(e, t) * NR;
+        if constexpr (NT) {
+            typedef float f2v __attribute__((ext_vector_type(2)));
+            const f2v x = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+            v[e] = make_float2(x.x, x.y);
+        } else {
+            v[e] = *p;
+        }
+    }
+}
+
+template <int R, int E, int G, int MODE, class TW, class TS, bool NTS, int... Rs>
+__device__ __forceinline__ void gd_compute(float2 (&d)[E], const GdSide<TW, E, MODE>& sd, float2* __restrict__ dst,
+                                           const TS& ts, uint64_t N, uint64_t L, uint64_t b0, float sc,
+                                           float2* __restrict__ lds_all, uint32_t tid) {
+    using MP = GdMap<R, E, G, Rs...>;
+    constexpr int T = MP::T;
+    constexpr int THREADS = G * T;
+    constexpr int RF = MP::RF;
+    constexpr bool INV = MODE == kInverse;
+    constexpr int SLOT = GdSlot<R, E, Rs...>::len;
+    const uint32_t g = tid % G, t = tid / G;
+    const bool first = (L == 1);
+    if (!first) {
+        const float2 di = TW::value(sd.di), dm = TW::value(sd.dm);
+        float2 ui = TW::value(sd.u);
+#pragma unroll
+        for (int i = 0; i < MP::NG; ++i) {
+            float2 w = ui;
+#pragma unroll
+            for (int m = 0; m < RF; ++m) {
+                d[i * RF + m] = twmul<INV>(d[i * RF + m], w);
+                if (m + 1 < RF) w = cmul(w, dm);
+            }
+            if (i + 1 < MP::NG) ui = cmul(ui, di);
+        }
+    } else if constexpr (MODE == kWindowed) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) d[e] = cscale(d[e], sd.w[e]);
+    }
+    float2* lds = lds_all + g * SLOT;
+    Engine<R, E, T, INV>::run_single(d, lds, ts, t, Radices<Rs...>{});
+    if (!first) {
+        // output b + m L (b = b0 + g, rows of G contiguous butterflies)
+        const uint64_t bj = b0 / L, k = (b0 & (L - 1)) + g;
+        float2* o = dst + bj * R * L + k;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const float2 v = cscale(d[e], sc);
+            float2* p = o + (uint64_t)MP::row(e, t) * L;
+            if constexpr (NTS) {
+                typedef float f2v __attribute__((ext_vector_type(2)));
+                const f2v nv = {v.x, v.y};
+                __builtin_nontemporal_store(nv, reinterpret_cast<f2v*>(p));
+            } else {
+                *p = v;
+            }
+        }
+        return;
+    }
+    // L = 1: output (b0 + g) R + q, contiguous per column: transpose through LDS
+    WGFFT_LDS_SYNC();  // the sub-FFT's last exchange has been read
+#pragma unroll
+    for (int e = 0; e < E; ++e) lds[pad_idx<5, 0>(MP::row(e, t))] = d[e];
+    WGFFT_LDS_SYNC();
+#pragma unroll
+    for (int e = 0; e < E / 2; ++e) {
+        const uint32_t f = 2 * (tid + THREADS * e);  // element pair f, f + 1 of the tile's G x R outputs
+        const uint32_t gg = f / R, q = f % R;
+        const float2 a = cscale(lds_all[gg * SLOT + pad_idx<5, 0>(q)], sc);
+        const float2 b = cscale(lds_all[gg * SLOT + pad_idx<5, 0>(q + 1)], sc);
+        st16<NTS>(reinterpret_cast<float4*>(dst + (b0 + gg) * R + q), make_float4(a.x, a.y, b.x, b.y));
+    }
+}
+
+// Persistent direct global pass (see gd_compute), tiles read and written
+// non-temporally, next tile optionally prefetched into registers.
+template <int R, int E, int G, bool PF, int MODE, int... Rs>
+__global__ void __launch_bounds__(G*(R / E))
+k_global_pass_direct(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw_r,
+                     const float2* __restrict__ tw_hi, const float2* __restrict__ tw_lo,
+                     const float* __restrict__ win, uint32_t log2n, uint32_t h, uint64_t L, uint64_t batch,
+                     float scale) {
+    constexpr int SLOT = GdSlot<R, E, Rs...>::len;
+    __shared__ __align__(16) float2 lds_all[G * SLOT];
+    __shared__ __align__(16) float2 lds_tw[twlds_len(ilog2(R))];
+
+    const uint64_t N = 1ull << log2n;
+    const uint64_t NR = N / R;
+    const uint64_t groups_per_tr = NR / G;
+    const uint64_t total = groups_per_tr * batch;
+    const uint64_t nb = gridDim.x;  // a multiple of 8
+    const uint64_t pid = blockIdx.x;
+    const uint64_t lid0 = (pid & 7) * (nb >> 3) + (pid >> 3);
+    const bool last = (L * R == N);
+    const TwTwoLevel twN{tw_hi, tw_lo, h};
+    const TwLds twr = twlds_fill<R>(lds_tw, tw_r, threadIdx.x, G * (R / E));
+    __syncthreads();
+    using Side = GdSide<TwTwoLevel, E, MODE>;
+    float2 cur[E];
+    Side sd;
+    if (PF && lid0 < total) {
+        const uint64_t tr = lid0 / groups_per_tr, b0 = (lid0 % groups_per_tr) * G;
+        gd_load_side<R, E, G, MODE, TwTwoLevel, Rs...>(sd, twN, win, N, L, b0, threadIdx.x);
+        gd_load<R, E, G, true, Rs...>(cur, in + tr * N, NR, b0, threadIdx.x);
+    }
+    for (uint64_t lid = lid0; lid < total; lid += nb) {
+        const uint64_t tr = lid / groups_per_tr, b0 = (lid % groups_per_tr) * G;
+        const uint32_t tid = launder_lane(threadIdx.x);
+        if constexpr (PF) {
+            float2 nxt[E];
+            Side nsd;
+            const uint64_t nl = lid + nb;
+            if (nl < total) {
+                const uint64_t ntr = nl / groups_per_tr, nb0 = (nl % groups_per_tr) * G;
+                gd_load_side<R, E, G, MODE, TwTwoLevel, Rs...>(nsd, twN, win, N, L, nb0, tid);
+                gd_load<R, E, G, true, Rs...>(nxt, in + ntr * N, NR, nb0, tid);
+            }
+            gd_compute<R, E, G, MODE, TwTwoLevel, TwLds, true, Rs...>(cur, sd, out + tr * N, twr, N, L, b0,
+                                                                      last ? scale : 1.0f, lds_all, tid);
+#pragma unroll
+            for (int i = 0; i < E; ++i) cur[i] = nxt[i];
+            sd = nsd;
+        } else {
+            gd_load_side<R, E, G, MODE, TwTwoLevel, Rs...>(sd, twN, win, N, L, b0, tid);
+            gd_load<R, E, G, true, Rs...>(cur, in + tr * N, NR, b0, tid);
+            gd_compute<R, E, G, MODE, TwTwoLevel, TwLds, true, Rs...>(cur, sd, out + tr * N, twr, N, L, b0,
+                                                                      last ? scale : 1.0f, lds_all, tid);
+        }
+        WGFFT_LDS_SYNC();
+    }
+}
+
+template <int R, int E, int G, bool PF, int... Rs>
+struct GlobalPassDirectKernel {
+    static constexpr int T = R / E;
+    static constexpr int THREADS = G * T;
+    static constexpr int LDS_BYTES = G * GdSlot<R, E, Rs...>::len * 8 + twlds_len(ilog2(R)) * 8;
+
+    template <int MODE>
+    static int occ() {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &blocks, (const void*)k_global_pass_direct<R, E, G, PF, MODE, Rs...>, THREADS, 0) != hipSuccess)
+            return 1;
+        return blocks > 0 ? blocks : 1;
+    }
+    template <int MODE>
+    static hipError_t go(const GlobalPassArgs& a) {
+        const uint64_t N = 1ull << a.log2n;
+        const uint64_t groups = (N / R) / G * a.batch;
+        if (groups == 0) return hipSuccess;
+        static const int per_cu = occ<MODE>();
+        uint64_t grid = std::min<uint64_t>(groups, (uint64_t)per_cu * (uint64_t)a.num_cus);
+        if (grid % 8) grid += 8 - grid % 8;  // keep the XCD numbering a bijection; extra blocks exit
+        hipLaunchKernelGGL((k_global_pass_direct<R, E, G, PF, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0,
+                           a.stream, a.in, a.out, a.tw_r, a.tw_hi, a.tw_lo, a.win, a.log2n, a.h, a.L, a.batch,
+                           a.scale);
+        return hipGetLastError();
+    }
+    static hipError_t launch(Mode m, const GlobalPassArgs& a) {
+        switch (m) {
+            case kForward: return go<kForward>(a);
+            case kInverse: return go<kInverse>(a);
+            default: return go<kWindowed>(a);
+        }
+    }
+};
+
+// prefetch = 3 (+ PF) in the registry marks the direct-I/O pipeline.
+#define WGFFT_REGISTER_GLOBAL_PASS_DIRECT(R, E, G, PF, PLAN, ...)                                          \
+    static ::wgfft::Registrar _wgfft_gpx_##R##_##E##_##G##_##PF##_##PLAN(::wgfft::GlobalPassVariant{    \
+        ::wgfft::ilog2(R), E, G, 3 + PF, ::wgfft::GlobalPassDirectKernel<R, E, G, PF, __VA_ARGS__>::THREADS, \
+        ::wgfft::GlobalPassDirectKernel<R, E, G, PF, __VA_ARGS__>::LDS_BYTES, #PLAN,                     \
+        &::wgfft::GlobalPassDirectKernel<R, E, G, PF, __VA_ARGS__>::launch})
+
 // Persistent multi-pass kernel (tiles read and written non-temporally: each
 // element is touched once per pass): workgroups loop over the groups of all
 // transforms in the batch with the next group's tile prefetched into
@@ -613,268 +857,5 @@ struct GlobalPassKernel {
         ::wgfft::ilog2(R), E, G, PF, ::wgfft::GlobalPassKernel<R, E, G, PF, __VA_ARGS__>::THREADS,       \
         ::wgfft::GlobalPassKernel<R, E, G, PF, __VA_ARGS__>::LDS_BYTES, #PLAN,                          \
         &::wgfft::GlobalPassKernel<R, E, G, PF, __VA_ARGS__>::launch})
-
-// ---------------------------------------------------------------------------
-// Global pass with the tile streamed into LDS by the asynchronous global ->
-// LDS copy (global_load_lds_dwordx4) and double-buffered: tile i+1 is in
-// flight into one LDS buffer while tile i is transformed in the other, with
-// no registers held for the prefetch.
-//
-// Tile layout in a buffer, in 16-byte granules (columns 2cp, 2cp + 1 of one
-// row): the tile's R rows are cut into blocks of RB = 128 / G rows; block b
-// occupies granules [65 b, 65 b + 64), one wavefront copy instruction each,
-// row-major with the pairs of a row adjacent (so adjacent lanes fetch
-// adjacent 16 bytes of global memory).  Inside a block the even rows come
-// first, and the one-granule gap between blocks staggers the banks: a
-// half-wavefront reading every other row of one column then touches each
-// 8-byte bank slot at most twice.
-// Asynchronous 16-byte global -> LDS copy of one wavefront: lane l's 16
-// bytes at src land at dst_wave + 16 l (dst_wave must be wave-uniform).  It
-// is issued as inline assembly so that the compiler does not add its own
-// (conservative, all-copies) waits before LDS reads of the other buffer;
-// callers wait for completion with s_waitcnt vmcnt(0) and a barrier before
-// the destination is read.  No register is written by the instruction.
-__device__ __forceinline__ void lds_copy16(float4* dst_wave, const float2* src) {
-    const uint32_t lds = __builtin_amdgcn_readfirstlane(
-        (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void*)dst_wave);
-    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds), "v"(src) : "memory", "m0");
-}
-
-template <int R, int E, int G>
-struct DmaTile {
-    static constexpr int T = R / E;
-    static constexpr int THREADS = G * T;
-    static constexpr int WAVES = THREADS / 64;
-    static constexpr int RB = 128 / G;              // rows per 64-granule block
-    static constexpr int BLOCKS = R / RB;
-    static constexpr int BSTRIDE = 65;              // granules per block incl. the gap
-    static constexpr int GRANULES = BLOCKS * BSTRIDE;
-    static_assert(T % 64 == 0, "a wavefront must lie inside one column's sub-FFT");
-    static_assert(G % 2 == 0 && 128 % G == 0 && R % RB == 0, "tile shape");
-    static_assert(BLOCKS == WAVES * (E / 2), "E / 2 copies per thread fill the tile");
-
-    // issue the copy of columns b0 .. b0+G-1 (rows m at stride NR) into buf
-    __device__ __forceinline__ static void load(float2* buf, const float2* __restrict__ src, uint64_t NR, uint64_t b0,
-                                                uint32_t tid) {
-        const uint32_t wave = tid >> 6, lane = tid & 63;
-        const uint32_t rho = lane / (G / 2), cp = lane % (G / 2);
-        const uint32_t row = rho < RB / 2 ? 2 * rho : 2 * (rho - RB / 2) + 1;
-#pragma unroll
-        for (int q = 0; q < E / 2; ++q) {
-            const uint32_t blk = wave + WAVES * q;
-            const float2* g = src + (uint64_t)(blk * RB + row) * NR + b0 + 2 * cp;
-            float4* l = reinterpret_cast<float4*>(buf) + blk * BSTRIDE;
-            lds_copy16(l, g);
-        }
-    }
-    // element (row 2t + delta + 2T q, column g) of the tile
-    __device__ __forceinline__ static uint32_t index(uint32_t g, uint32_t t, int delta, int q) {
-        const uint32_t m = 2 * t + delta + 2 * T * q;
-        const uint32_t rho = (delta ? RB / 2 : 0) + (t % (RB / 2));
-        return 2 * ((m / RB) * BSTRIDE + rho * (G / 2) + (g >> 1)) + (g & 1);
-    }
-};
-
-// Side values of one tile: W_N^{m k s} for rows m = 2t + delta + 2T q of the
-// thread's column (k its butterfly) are W^{(2t+delta) k s} (W^{2T k s})^q;
-// the windowed first pass needs the window at the thread's E elements.
-template <class TW, int E, int MODE>
-struct DmaSide {
-    typename TW::raw_t a0, a1, st;
-    float w[MODE == kWindowed ? E : 1];
-};
-
-template <int R, int E, int G, int MODE, class TW>
-__device__ __forceinline__ void dma_load_side(DmaSide<TW, E, MODE>& sd, const TW& twN, const float* __restrict__ win,
-                                              uint64_t N, uint64_t L, uint64_t b0, uint32_t tid) {
-    constexpr int T = R / E;
-    const uint32_t g = tid / T, t = tid % T;
-    if (L > 1) {
-        const uint64_t s = N / ((uint64_t)R * L);
-        const uint64_t k = (b0 + g) & (L - 1);
-        sd.a0 = twN.raw((2 * t * k * s) & (N - 1));
-        sd.a1 = twN.raw(((2 * t + 1) * k * s) & (N - 1));
-        sd.st = twN.raw((2 * T * k * s) & (N - 1));
-    } else if constexpr (MODE == kWindowed) {
-        const uint64_t NR = N / R;
-#pragma unroll
-        for (int q = 0; q < E / 2; ++q) {
-            sd.w[2 * q] = win[b0 + g + (uint64_t)(2 * t + 2 * T * q) * NR];
-            sd.w[2 * q + 1] = win[b0 + g + (uint64_t)(2 * t + 1 + 2 * T * q) * NR];
-        }
-    }
-}
-
-// Transform the tile in buf (columns b0 .. b0+G-1) and store it; buf is then
-// reused for the sub-FFT's exchanges and the output transposition.
-template <int R, int E, int G, int MODE, class TW, int... Rs>
-__device__ __forceinline__ void dma_compute(float2* buf, const DmaSide<TW, E, MODE>& sd, float2* __restrict__ dst,
-                                            const TwLds& twr, uint64_t N, uint64_t L, uint64_t b0, float sc,
-                                            uint32_t tid) {
-    using TL = DmaTile<R, E, G>;
-    constexpr int T = R / E;
-    constexpr int THREADS = G * T;
-    constexpr bool INV = MODE == kInverse;
-    constexpr int SLOT = GlobalPassSlot<R, E, Rs...>::len;
-    static_assert(IoKind<E, Rs...>::PAIR, "global-pass sub-FFTs use the pair layout");
-    static_assert(2 * TL::GRANULES <= G * SLOT, "the copied tile fits the buffer");
-    const uint32_t g = tid / T, t = tid % T;
-    const bool first = (L == 1);
-
-    float2 d[E];
-#pragma unroll
-    for (int q = 0; q < E / 2; ++q) {
-        d[2 * q] = buf[TL::index(g, t, 0, q)];
-        d[2 * q + 1] = buf[TL::index(g, t, 1, q)];
-    }
-    if (!first) {
-        float2 u0 = TW::value(sd.a0), u1 = TW::value(sd.a1);
-        const float2 st = TW::value(sd.st);
-#pragma unroll
-        for (int q = 0; q < E / 2; ++q) {
-            d[2 * q] = twmul<INV>(d[2 * q], u0);
-            d[2 * q + 1] = twmul<INV>(d[2 * q + 1], u1);
-            if (q + 1 < E / 2) {
-                u0 = cmul(u0, st);
-                u1 = cmul(u1, st);
-            }
-        }
-    } else if constexpr (MODE == kWindowed) {
-#pragma unroll
-        for (int i = 0; i < E; ++i) d[i] = cscale(d[i], sd.w[i]);
-    }
-    WGFFT_LDS_SYNC();  // the whole tile is in registers before the exchanges overwrite it
-    float2* lds = buf + g * SLOT;
-    Engine<R, E, T, INV>::run_with(d, lds, twr, t, Radices<Rs...>{});
-
-    if (first) {
-        store_pairs<R, E, T, true>(dst + (b0 + g) * R, d, t, sc);
-        return;
-    }
-    WGFFT_LDS_SYNC();
-#pragma unroll
-    for (int q = 0; q < E / 2; ++q) {
-        lds[pad_idx<5, 0>(2 * t + 2 * T * q)] = d[2 * q];
-        lds[pad_idx<5, 0>(2 * t + 1 + 2 * T * q)] = d[2 * q + 1];
-    }
-    WGFFT_LDS_SYNC();
-    const uint64_t bj = b0 / L, k0base = b0 & (L - 1);
-#pragma unroll
-    for (int q = 0; q < E / 2; ++q) {
-        const int it = tid + THREADS * q;
-        const int gp = it % (G / 2);
-        const int m = it / (G / 2);
-        const uint32_t pm = pad_idx<5, 0>(m);
-        float2 a = cscale(buf[(2 * gp) * SLOT + pm], sc);
-        float2 b = cscale(buf[(2 * gp + 1) * SLOT + pm], sc);
-        const uint64_t o = bj * R * L + k0base + 2 * gp + (uint64_t)m * L;
-        *reinterpret_cast<float4*>(dst + o) = make_float4(a.x, a.y, b.x, b.y);
-    }
-}
-
-// Persistent global pass over two LDS tile buffers (one workgroup per CU).
-// Each step waits for its tile's copy and the workgroup barrier, issues the
-// next tile's side loads and copy into the other buffer, then transforms its
-// tile.  Nothing in the transform loads from global memory (the sub-FFT's
-// twiddles come from LDS tables), so no wait in it can stall on the copy in
-// flight (vmcnt retires in order).
-template <int R, int E, int G, int MODE, int... Rs>
-__global__ void __launch_bounds__(G*(R / E))
-k_global_pass_dma(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw_r,
-                  const float2* __restrict__ tw_hi, const float2* __restrict__ tw_lo,
-                  const float* __restrict__ win, uint32_t log2n, uint32_t h, uint64_t L, uint64_t batch,
-                  float scale) {
-    using TL = DmaTile<R, E, G>;
-    constexpr int SLOT = GlobalPassSlot<R, E, Rs...>::len;
-    __shared__ __align__(16) float2 buf_a[G * SLOT];
-    __shared__ __align__(16) float2 buf_b[G * SLOT];
-    __shared__ __align__(16) float2 lds_tw[twlds_len(ilog2(R))];
-    const TwLds twr = twlds_fill<R>(lds_tw, tw_r, threadIdx.x, G * (R / E));
-
-    const uint64_t N = 1ull << log2n;
-    const uint64_t NR = N / R;
-    const uint64_t groups_per_tr = NR / G;
-    const uint64_t total = groups_per_tr * batch;
-    const uint64_t nb = gridDim.x;  // a multiple of 8
-    const uint64_t pid = blockIdx.x;
-    const uint64_t lid0 = (pid & 7) * (nb >> 3) + (pid >> 3);
-    const float sc = (L * R == N) ? scale : 1.0f;
-    const TwTwoLevel twN{tw_hi, tw_lo, h};
-    using Side = DmaSide<TwTwoLevel, E, MODE>;
-    Side sa, sb;
-
-    // the thread index is laundered per call so per-thread address arithmetic
-    // is recomputed per step rather than hoisted and held in registers
-    auto issue = [&](float2* buf, Side& sd, uint64_t lid) {
-        const uint32_t tid = launder_lane(threadIdx.x);
-        const uint64_t tr = lid / groups_per_tr, b0 = (lid % groups_per_tr) * G;
-        dma_load_side<R, E, G, MODE>(sd, twN, win, N, L, b0, tid);
-        TL::load(buf, in + tr * N, NR, b0, tid);
-    };
-    auto compute = [&](float2* buf, const Side& sd, uint64_t lid) {
-        const uint32_t tid = launder_lane(threadIdx.x);
-        const uint64_t tr = lid / groups_per_tr, b0 = (lid % groups_per_tr) * G;
-        dma_compute<R, E, G, MODE, TwTwoLevel, Rs...>(buf, sd, out + tr * N, twr, N, L, b0, sc, tid);
-    };
-
-    // s_waitcnt vmcnt(0) as a builtin (vmcnt 0, expcnt 7, lgkmcnt 15), so the
-    // compiler knows every earlier global load has retired
-    constexpr int kWaitVm0 = 0x0F70;
-    if (lid0 < total) issue(buf_a, sa, lid0);  // the LDS twiddle tables are synchronised by the first step
-    for (uint64_t lid = lid0; lid < total; lid += 2 * nb) {
-        __builtin_amdgcn_s_waitcnt(kWaitVm0);
-        WGFFT_LDS_SYNC();  // tile lid is in buf_a; everyone is done with buf_b
-        if (lid + nb < total) issue(buf_b, sb, lid + nb);
-        compute(buf_a, sa, lid);
-        if (lid + nb >= total) break;
-        __builtin_amdgcn_s_waitcnt(kWaitVm0);
-        WGFFT_LDS_SYNC();
-        if (lid + 2 * nb < total) issue(buf_a, sa, lid + 2 * nb);
-        compute(buf_b, sb, lid + nb);
-    }
-}
-
-template <int R, int E, int G, int... Rs>
-struct GlobalPassDmaKernel {
-    static constexpr int T = R / E;
-    static constexpr int THREADS = G * T;
-    static constexpr int LDS_BYTES = 2 * G * GlobalPassSlot<R, E, Rs...>::len * 8 + twlds_len(ilog2(R)) * 8;
-
-    template <int MODE>
-    static int occ() {
-        int blocks = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_global_pass_dma<R, E, G, MODE, Rs...>,
-                                                         THREADS, 0) != hipSuccess)
-            return 1;
-        return blocks > 0 ? blocks : 1;
-    }
-    template <int MODE>
-    static hipError_t go(const GlobalPassArgs& a) {
-        const uint64_t N = 1ull << a.log2n;
-        const uint64_t groups = (N / R) / G * a.batch;
-        if (groups == 0) return hipSuccess;
-        static const int per_cu = occ<MODE>();
-        uint64_t grid = std::min<uint64_t>(groups, (uint64_t)per_cu * (uint64_t)a.num_cus);
-        if (grid % 8) grid += 8 - grid % 8;  // keep the XCD numbering a bijection; extra blocks exit
-        hipLaunchKernelGGL((k_global_pass_dma<R, E, G, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0,
-                           a.stream, a.in, a.out, a.tw_r, a.tw_hi, a.tw_lo, a.win, a.log2n, a.h, a.L, a.batch,
-                           a.scale);
-        return hipGetLastError();
-    }
-    static hipError_t launch(Mode m, const GlobalPassArgs& a) {
-        switch (m) {
-            case kForward: return go<kForward>(a);
-            case kInverse: return go<kInverse>(a);
-            default: return go<kWindowed>(a);
-        }
-    }
-};
-
-// prefetch = 2 in the registry marks the LDS-copy pipeline.
-#define WGFFT_REGISTER_GLOBAL_PASS_DMA(R, E, G, PLAN, ...)                                              \
-    static ::wgfft::Registrar _wgfft_gpd_##R##_##E##_##G##_##PLAN(::wgfft::GlobalPassVariant{        \
-        ::wgfft::ilog2(R), E, G, 2, ::wgfft::GlobalPassDmaKernel<R, E, G, __VA_ARGS__>::THREADS,       \
-        ::wgfft::GlobalPassDmaKernel<R, E, G, __VA_ARGS__>::LDS_BYTES, #PLAN,                          \
-        &::wgfft::GlobalPassDmaKernel<R, E, G, __VA_ARGS__>::launch})
 
 }  // namespace wgfft

@@ -156,7 +156,10 @@ static std::string describe_multipass(const Plan* p) {
     for (size_t i = 0; i < p->pass_var.size(); ++i) {
         const GlobalPassVariant* v = p->pass_var[i];
         s += (i ? "," : "") + std::string("R") + std::to_string(1 << v->log2r) + "(" + v->plan + ",G" +
-             std::to_string(v->G) + (v->prefetch == 2 ? ",lds-copy" : v->prefetch ? ",prefetch" : "") + ")";
+             std::to_string(v->G) + (v->prefetch == 3 ? ",direct"
+                                      : v->prefetch == 4 ? ",direct,prefetch"
+                                      : v->prefetch      ? ",prefetch"
+                                                         : "") + ")";
     }
     return s;
 }

@@ -49,7 +49,7 @@ constexpr int direct_g(int log2n) { return kRaggedThreads / ((1 << log2n) / dire
 template <int N, int E, int... Rs>
 constexpr int direct_lds() { return direct_g(ilog2(N)) * LdsLayout<N, E, Rs...>::len; }
 template <int R, int E, int... Rs>
-constexpr int pass_lds() { return (kRaggedThreads / (R / E)) * GlobalPassSlot<R, E, Rs...>::len; }
+constexpr int pass_lds() { return (kRaggedThreads / (R / E)) * GdSlot<R, E, Rs...>::len; }
 
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int kRaggedLds = cmax(
@@ -57,7 +57,7 @@ constexpr int kRaggedLds = cmax(
          cmax(direct_lds<256, 16, 8, 4, 8>(), direct_lds<512, 16, 8, 8, 8>())),
     cmax(cmax(direct_lds<1024, 16, 8, 16, 8>(), direct_lds<2048, 16, 8, 4, 8, 8>()),
          cmax(cmax(direct_lds<4096, 16, 8, 8, 8, 8>(), direct_lds<8192, 32, 16, 16, 16, 2>()),
-              cmax(cmax(pass_lds<64, 16, 8, 8>(), pass_lds<128, 16, 4, 4, 8>()), pass_lds<256, 16, 8, 4, 8>()))));
+              cmax(pass_lds<128, 16, 8, 2, 8>(), pass_lds<256, 16, 16, 16>()))));
 
 // Unit handlers take the thread index through launder_lane() once per unit,
 // so no handler's index arithmetic is hoisted out of the persistent loop and
@@ -98,25 +98,25 @@ struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV, PF
         const TwLds t1 = tl.shifted(kMaxLog2 - ilog2(R1)), t2 = tl.shifted(kMaxLog2 - ilog2(R2));
         constexpr int M = INV ? kInverse : kForward;
         const TwOneLevel twN{tw_all + N};
-        using Side = GpSide<TwOneLevel, E1, M>;
+        using Side = GdSide<TwOneLevel, E1, M>;
         static_assert(E1 == E2, "both passes share the prefetch registers");
         // pass 1 (L = 1): src -> scratch (no twiddles, so no side loads)
         float2 cur[E1];
         Side sd;
-        if (PF) gp_load<R1, E1, G1, NTIO>(cur, src, N / R1, 0, tid);
+        if (PF) gd_load<R1, E1, G1, NTIO, Rs1...>(cur, src, N / R1, 0, tid);
 #pragma unroll 1
         for (uint64_t b0 = 0; b0 < N / R1; b0 += G1) {
             if constexpr (PF) {
                 float2 nxt[E1];
-                if (b0 + G1 < N / R1) gp_load<R1, E1, G1, NTIO>(nxt, src, N / R1, b0 + G1, tid);
-                gp_compute<R1, E1, G1, M, TwOneLevel, TwLds, false, Rs1...>(cur, sd, scratch, t1, N, 1, b0, 1.0f, lds,
-                                                                     tid);
+                if (b0 + G1 < N / R1) gd_load<R1, E1, G1, NTIO, Rs1...>(nxt, src, N / R1, b0 + G1, tid);
+                gd_compute<R1, E1, G1, M, TwOneLevel, TwLds, false, Rs1...>(cur, sd, scratch, t1, N, 1, b0, 1.0f, lds,
+                                                                            tid);
 #pragma unroll
                 for (int i = 0; i < E1; ++i) cur[i] = nxt[i];
             } else {
-                gp_load<R1, E1, G1, NTIO>(cur, src, N / R1, b0, tid);
-                gp_compute<R1, E1, G1, M, TwOneLevel, TwLds, false, Rs1...>(cur, sd, scratch, t1, N, 1, b0, 1.0f, lds,
-                                                                     tid);
+                gd_load<R1, E1, G1, NTIO, Rs1...>(cur, src, N / R1, b0, tid);
+                gd_compute<R1, E1, G1, M, TwOneLevel, TwLds, false, Rs1...>(cur, sd, scratch, t1, N, 1, b0, 1.0f, lds,
+                                                                            tid);
             }
             WGFFT_LDS_SYNC();
         }
@@ -124,8 +124,8 @@ struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV, PF
         // pass 2 (L = R1): scratch -> dst
         const float sc = INV ? 1.0f / (float)N : 1.0f;
         if (PF) {
-            gp_load_side<R2, E2, G2, M>(sd, twN, nullptr, N, R1, 0, tid);
-            gp_load<R2, E2, G2>(cur, scratch, N / R2, 0, tid);
+            gd_load_side<R2, E2, G2, M, TwOneLevel, Rs2...>(sd, twN, nullptr, N, R1, 0, tid);
+            gd_load<R2, E2, G2, false, Rs2...>(cur, scratch, N / R2, 0, tid);
         }
 #pragma unroll 1
         for (uint64_t b0 = 0; b0 < N / R2; b0 += G2) {
@@ -133,17 +133,17 @@ struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV, PF
                 float2 nxt[E2];
                 Side nsd;
                 if (b0 + G2 < N / R2) {
-                    gp_load_side<R2, E2, G2, M>(nsd, twN, nullptr, N, R1, b0 + G2, tid);
-                    gp_load<R2, E2, G2>(nxt, scratch, N / R2, b0 + G2, tid);
+                    gd_load_side<R2, E2, G2, M, TwOneLevel, Rs2...>(nsd, twN, nullptr, N, R1, b0 + G2, tid);
+                    gd_load<R2, E2, G2, false, Rs2...>(nxt, scratch, N / R2, b0 + G2, tid);
                 }
-                gp_compute<R2, E2, G2, M, TwOneLevel, TwLds, NTIO, Rs2...>(cur, sd, dst, t2, N, R1, b0, sc, lds, tid);
+                gd_compute<R2, E2, G2, M, TwOneLevel, TwLds, NTIO, Rs2...>(cur, sd, dst, t2, N, R1, b0, sc, lds, tid);
 #pragma unroll
                 for (int i = 0; i < E2; ++i) cur[i] = nxt[i];
                 sd = nsd;
             } else {
-                gp_load_side<R2, E2, G2, M>(sd, twN, nullptr, N, R1, b0, tid);
-                gp_load<R2, E2, G2>(cur, scratch, N / R2, b0, tid);
-                gp_compute<R2, E2, G2, M, TwOneLevel, TwLds, NTIO, Rs2...>(cur, sd, dst, t2, N, R1, b0, sc, lds, tid);
+                gd_load_side<R2, E2, G2, M, TwOneLevel, Rs2...>(sd, twN, nullptr, N, R1, b0, tid);
+                gd_load<R2, E2, G2, false, Rs2...>(cur, scratch, N / R2, b0, tid);
+                gd_compute<R2, E2, G2, M, TwOneLevel, TwLds, NTIO, Rs2...>(cur, sd, dst, t2, N, R1, b0, sc, lds, tid);
             }
             WGFFT_LDS_SYNC();
         }
@@ -193,19 +193,19 @@ k_ragged(const float2* __restrict__ in, float2* __restrict__ out, const RaggedUn
             case 13: ragged_direct<8192, 32, INV, 16, 16, 16, 2>(in, out, offs, un.first, un.count, tl, lds, tid); break;
             case 14: {
                 const uint64_t o = offs[un.first];
-                RaggedBig<14, 128, 16, Radices<4, 4, 8>, 128, 16, Radices<4, 4, 8>, INV, PF>::run(in + o, out + o,
+                RaggedBig<14, 128, 16, Radices<8, 2, 8>, 128, 16, Radices<8, 2, 8>, INV, PF>::run(in + o, out + o,
                                                                                               my_scratch, tw_all, tl, lds, tid);
                 break;
             }
             case 15: {
                 const uint64_t o = offs[un.first];
-                RaggedBig<15, 256, 16, Radices<8, 4, 8>, 128, 16, Radices<4, 4, 8>, INV, PF>::run(in + o, out + o,
+                RaggedBig<15, 256, 16, Radices<16, 16>, 128, 16, Radices<8, 2, 8>, INV, PF>::run(in + o, out + o,
                                                                                               my_scratch, tw_all, tl, lds, tid);
                 break;
             }
             default: {
                 const uint64_t o = offs[un.first];
-                RaggedBig<16, 256, 16, Radices<8, 4, 8>, 256, 16, Radices<8, 4, 8>, INV, PF>::run(in + o, out + o,
+                RaggedBig<16, 256, 16, Radices<16, 16>, 256, 16, Radices<16, 16>, INV, PF>::run(in + o, out + o,
                                                                                               my_scratch, tw_all, tl, lds, tid);
                 break;
             }

@@ -309,6 +309,12 @@ struct LdsLayout {
     static constexpr int len = (raw + 1) & ~1;
 };
 
+// The footprint of the same plan run in the single layout (Engine::run_single).
+template <int N, int E, int... Rs>
+struct LdsLayoutSingle {
+    static constexpr int len = LdsLayoutAcc<N, E, false>::template max_len<1, true, Rs...>();
+};
+
 // Sources of the twiddle base powers w^(2^i), i < tw_bases(R), of butterfly
 // slot j of a pass (its twiddle index base = k N / (R L); TO = the pass's
 // first entry in a register-held set).

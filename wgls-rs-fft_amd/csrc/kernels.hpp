@@ -529,14 +529,15 @@ struct GdMap {
 };
 
 // LDS slot (complex elements) per column of a direct global pass: the
-// sub-FFT's exchange footprint or the transposition (R + R/32), kept 2 mod 32
-// so the G columns of a wavefront start 16 bytes apart (all 64 banks).
+// sub-FFT's exchange footprint or the transposition (R + R/32), made odd so
+// the columns of a wavefront (column index fastest) start on different
+// 8-byte bank slots.  Every access to it is 8 bytes wide.
 template <int R, int E, int... Rs>
 struct GdSlot {
-    static constexpr int a = LdsLayout<R, E, Rs...>::len;
+    static constexpr int a = LdsLayoutSingle<R, E, Rs...>::len;
     static constexpr int b = R + (R >> 5);
     static constexpr int m = a > b ? a : b;
-    static constexpr int len = m + ((2 - m % 32) + 32) % 32;
+    static constexpr int len = m | 1;
 };
 
 // Twiddle bases of one tile: W_N^{row k s} for row = t + T i + (R/RF) m is

@@ -57,7 +57,7 @@ constexpr int kRaggedLds = cmax(
          cmax(direct_lds<256, 16, 8, 4, 8>(), direct_lds<512, 16, 8, 8, 8>())),
     cmax(cmax(direct_lds<1024, 16, 8, 16, 8>(), direct_lds<2048, 16, 8, 4, 8, 8>()),
          cmax(cmax(direct_lds<4096, 16, 8, 8, 8, 8>(), direct_lds<8192, 32, 16, 16, 16, 2>()),
-              cmax(pass_lds<128, 16, 8, 2, 8>(), pass_lds<256, 16, 16, 16>()))));
+              cmax(pass_lds<128, 16, 4, 8, 4>(), pass_lds<256, 16, 16, 16>()))));
 
 // Unit handlers take the thread index through launder_lane() once per unit,
 // so no handler's index arithmetic is hoisted out of the persistent loop and
@@ -193,13 +193,13 @@ k_ragged(const float2* __restrict__ in, float2* __restrict__ out, const RaggedUn
             case 13: ragged_direct<8192, 32, INV, 16, 16, 16, 2>(in, out, offs, un.first, un.count, tl, lds, tid); break;
             case 14: {
                 const uint64_t o = offs[un.first];
-                RaggedBig<14, 128, 16, Radices<8, 2, 8>, 128, 16, Radices<8, 2, 8>, INV, PF>::run(in + o, out + o,
+                RaggedBig<14, 128, 16, Radices<4, 8, 4>, 128, 16, Radices<4, 8, 4>, INV, PF>::run(in + o, out + o,
                                                                                               my_scratch, tw_all, tl, lds, tid);
                 break;
             }
             case 15: {
                 const uint64_t o = offs[un.first];
-                RaggedBig<15, 256, 16, Radices<16, 16>, 128, 16, Radices<8, 2, 8>, INV, PF>::run(in + o, out + o,
+                RaggedBig<15, 256, 16, Radices<16, 16>, 128, 16, Radices<4, 8, 4>, INV, PF>::run(in + o, out + o,
                                                                                               my_scratch, tw_all, tl, lds, tid);
                 break;
             }

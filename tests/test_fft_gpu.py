@@ -293,6 +293,18 @@ def test_c3_large_n_2pow22(gpu, parity):
     parity["C3 2^22x16"] = {"fwd_rel_l2_vs_fp64": err, "round_trip_rel_l2": e_rt, "bar": 2e-5,
                             "plan": gpu.describe(n)}
     assert e_rt <= 2e-5
+    # the plan the bench times (autotuned radix split and pass variants)
+    g = wgfft.GpuFft.new()
+    try:
+        plan = g.autotune(n, batch)
+        y2 = g.fft_array(x)
+        e2 = oracle.rel_l2(y2, oracle.fft64(x))
+        e2_rt = oracle.rel_l2(g.fft_array(y2, inverse=True), x)
+        parity["C3 2^22x16 autotuned"] = {"fwd_rel_l2_vs_fp64": e2, "round_trip_rel_l2": e2_rt, "bar": 2e-5,
+                                          "plan": plan}
+        assert e2 <= 2e-5 and e2_rt <= 2e-5
+    finally:
+        g.close()
 
 
 def test_c3_large_n_2pow26(gpu, parity):
@@ -313,6 +325,19 @@ def test_c3_large_n_2pow26(gpu, parity):
     parity["C3 2^26x1"] = {"bins_max_err_over_norm_vs_fp64_dft": e_bins, "parseval_rel_err": e_pars,
                            "round_trip_rel_l2": e_rt, "bar": 2e-5, "plan": gpu.describe(n)}
     assert e_rt <= 2e-5
+    # the plan the bench times (autotuned radix split and pass variants)
+    g = wgfft.GpuFft.new()
+    try:
+        plan = g.autotune(n, 1)
+        y2 = g.fft_array(x)
+        e2_bins = float(np.abs(y2[0, bins.astype(np.int64)] - want).max() / scale)
+        e2_rt = oracle.rel_l2(g.fft_array(y2, inverse=True), x)
+        e2_vs_default = oracle.rel_l2(y2, y)
+        parity["C3 2^26x1 autotuned"] = {"bins_max_err_over_norm_vs_fp64_dft": e2_bins, "round_trip_rel_l2": e2_rt,
+                                         "rel_l2_vs_default_plan": e2_vs_default, "bar": 2e-5, "plan": plan}
+        assert e2_bins < 2e-5 and e2_rt <= 2e-5 and e2_vs_default < 2e-5
+    finally:
+        g.close()
 
 
 # ------------------------------------------------------- device entry points

@@ -44,6 +44,27 @@ def test_ragged_small_batches_all_modes(gpu, torch_cuda, count, seed):
     plan.close()
 
 
+@pytest.mark.parametrize("kmax", [13, 14, 15])
+def test_ragged_scratch_sized_by_largest_length(gpu, torch_cuda, kmax):
+    """Batches whose largest length is 2^13 (no two-pass unit, no scratch),
+    2^14 or 2^15 (scratch slots of that size) in every launch mode."""
+    torch = torch_cuda
+    k = oracle.synth_ragged_log2(700, 6, kmax, 0x5EED0503 + kmax)
+    offsets, total = wgfft.ragged_layout(k)
+    x = oracle.synth_uniform((total,), 0x5EED0504 + kmax)
+    plan = wgfft.RaggedPlan(gpu, offsets, k)
+    tx = torch.from_numpy(x).cuda()
+    outs = []
+    for mode in (0, 1, 2):
+        ty = torch.zeros_like(tx)
+        plan.exec_device(tx, ty, mode=mode)
+        torch.cuda.synchronize()
+        outs.append(ty.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    _check(outs[0], x, offsets, k, range(0, 700, 3))
+    plan.close()
+
+
 def test_ragged_graph_replays(gpu, torch_cuda):
     torch = torch_cuda
     k = oracle.synth_ragged_log2(500, 6, 16, 0x5EED0303)

@@ -155,11 +155,11 @@ template <bool INV, bool PF>
 __global__ void __launch_bounds__(kRaggedThreads)
 k_ragged(const float2* __restrict__ in, float2* __restrict__ out, const RaggedUnit* __restrict__ units,
          uint32_t u_begin, uint32_t u_end, const uint64_t* __restrict__ offs, const float2* __restrict__ tw_all,
-         float2* __restrict__ scratch, uint32_t* __restrict__ counters) {
+         float2* __restrict__ scratch, uint32_t slot_log2, uint32_t* __restrict__ counters) {
     __shared__ __align__(16) float2 lds[kRaggedLds];
     __shared__ __align__(16) float2 lds_tw[twlds_len(kMaxLog2)];
     __shared__ uint32_t s_unit;
-    float2* my_scratch = scratch + (uint64_t)blockIdx.x * (1ull << kMaxLog2);
+    float2* my_scratch = scratch + ((uint64_t)blockIdx.x << slot_log2);  // 2^slot_log2: the largest two-pass length
     // W_65536 as two LDS tables; every sub-FFT of length R reads it with
     // stride 65536 / R (TwLds::shifted), so no twiddle load of a unit waits
     // behind another unit's global loads
@@ -238,6 +238,7 @@ struct wgfft_ragged_plan {
     void* d_scratch = nullptr;
     int grid = 0;      // workgroups of the chosen kernel variant
     int grid_max = 0;  // scratch slots allocated (largest grid of the variants)
+    uint32_t slot_log2 = 0;  // scratch slot of 2^slot_log2 complex per workgroup (0: no two-pass length)
     int pf = -1;       // big-unit tile prefetch: -1 undecided (timed on first use)
     struct GraphKey {
         const void* in;
@@ -330,7 +331,9 @@ extern "C" int wgfft_ragged_create(wgfft_t* h, const uint64_t* offsets, const ui
         }
     }
     tw[0] = make_float2(1.f, 0.f);
-    const uint64_t scratch_bytes = (uint64_t)p->grid_max * (1ull << kMaxLog2) * sizeof(float2);
+    // per-workgroup scratch only for the two-pass lengths, sized for the largest present
+    for (const auto& u : units) p->slot_log2 = std::max(p->slot_log2, u.log2n > (uint32_t)kDirectMaxLog2 ? u.log2n : 0u);
+    const uint64_t scratch_bytes = p->slot_log2 ? ((uint64_t)p->grid_max << p->slot_log2) * sizeof(float2) : 0;
     struct {
         void** dst;
         const void* src;
@@ -371,7 +374,7 @@ static void ragged_launch_t(wgfft_ragged_t* p, const float* in, float* out, uint
                             hipStream_t s) {
     hipLaunchKernelGGL((k_ragged<INV, PF>), dim3(grid), dim3(kRaggedThreads), 0, s, (const float2*)in, (float2*)out,
                        (const RaggedUnit*)p->d_units, u0, u1, (const uint64_t*)p->d_offs, (const float2*)p->d_tw,
-                       (float2*)p->d_scratch, (uint32_t*)p->d_counters);
+                       (float2*)p->d_scratch, p->slot_log2, (uint32_t*)p->d_counters);
 }
 
 static hipError_t ragged_launch_pf(wgfft_ragged_t* p, const float* in, float* out, bool inv, bool pf, uint32_t u0,

@@ -162,8 +162,8 @@ int wgfft_variant_count(uint64_t n);
 int wgfft_select_variant(wgfft_t* h, uint64_t n, int index, char* desc, uint64_t desc_len);
 
 /* Global passes of the plan for length n (0 for single-kernel lengths), and
- * the compiled candidates for pass `pass` (radix, E, G, prefetch or LDS-copy
- * pipeline). */
+ * the compiled candidates for pass `pass` (radix, E, G, LDS-staged or direct
+ * I/O, with or without the register prefetch of the next tile). */
 int wgfft_pass_count(uint64_t n);
 int wgfft_pass_variant_count(uint64_t n, int pass);
 

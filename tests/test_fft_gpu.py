@@ -94,9 +94,9 @@ def test_multipass_lengths(gpu, log2n, batch):
 
 @pytest.mark.parametrize("log2n,batch", [(17, 3), (20, 2), (22, 2), (23, 1)])
 def test_every_global_pass_variant(log2n, batch):
-    """Every compiled candidate of every global pass (register-prefetch,
-    plain and LDS-copy pipelines), pinned in turn with the other passes on
-    their defaults, matches FP64 forward and inverse."""
+    """Every compiled candidate of every global pass (LDS-staged and direct
+    I/O, with and without the register prefetch), pinned in turn with the
+    other passes on their defaults, matches FP64 forward and inverse."""
     n = 1 << log2n
     x = oracle.synth_uniform((batch, n), 0x5EED0180 + log2n)
     for p in range(wgfft.GpuFft.pass_count(n)):

@@ -8,9 +8,12 @@
 // j = b / L, reads x_m = X[b + m N/R] (m < R), multiplies by W_{RL}^{m k},
 // does a length-R DFT and writes Y[j R L + k + m L].  The first pass reads
 // global memory and the last pass writes it; the passes between exchange
-// data through LDS.  Each thread owns E elements; the first and last pass
-// give thread t the element pairs (2t + 2T q, 2t + 1 + 2T q), so global
-// traffic is 16-byte vectors and a wavefront touches 1 KiB contiguous.
+// data through LDS.  Each thread owns E elements.  In the pair layout the
+// first and last pass give thread t the element pairs (2t + 2T q,
+// 2t + 1 + 2T q), so global traffic is 16-byte vectors and a wavefront
+// touches 1 KiB contiguous; in the single layout (radix = E) thread t holds
+// t + T i + m N/R in slot i R + m, 8-byte lanes, a wavefront 512 B
+// contiguous per slot.
 #pragma once
 
 #include <hip/hip_runtime.h>

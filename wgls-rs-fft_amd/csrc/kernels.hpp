@@ -6,7 +6,11 @@
 //  k_small        one thread per transform, register DFT (N = 1 .. 16).
 //  k_global_pass  one pass of a multi-pass Stockham for N > 16384: a
 //                 workgroup runs G radix-R butterflies (R up to 8192) as an
-//                 LDS sub-FFT, twiddles W_{RL}^{mk} fused at load.
+//                 LDS sub-FFT, twiddles W_{RL}^{mk} fused at load, tiles
+//                 staged through LDS.
+//  k_global_pass_direct  the same pass with each thread loading and storing
+//                 its sub-FFT slots straight from the strided rows (no LDS
+//                 staging of the I/O).
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -171,6 +171,14 @@ int wgfft_pass_variant_count(uint64_t n, int pass);
  * this handle; writes the plan description into desc (if non-NULL). */
 int wgfft_select_pass_variant(wgfft_t* h, uint64_t n, int pass, int index, char* desc, uint64_t desc_len);
 
+/* Number of compiled fused forward+adjoint / VJP variants for length n (0 when
+ * the length is served by the unfused path), and: make variant `index` the
+ * handle's choice for length n, as a tuning run would, writing its
+ * description into desc (if non-NULL).  No reference counterpart: the
+ * adjoint has none (the reference has no differentiation path). */
+int wgfft_adjoint_variant_count(uint64_t n);
+int wgfft_select_adjoint_variant(wgfft_t* h, uint64_t n, int index, char* desc, uint64_t desc_len);
+
 /* Fill nfloats floats with i.i.d. U[-1,1) = (u32 >> 8) * 2^-23 - 1 drawn
  * from PCG32 seeded through SplitMix64(seed) (input synthesis, host). */
 void wgfft_synth_uniform(float* out, uint64_t nfloats, uint64_t seed);

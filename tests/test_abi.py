@@ -87,9 +87,14 @@ def test_variant_registry_counts():
     for p in range(2):
         assert wgfft.GpuFft.pass_variant_count(1 << 22, p) >= 2
     assert wgfft.GpuFft.pass_variant_count(1 << 22, 2) == 0
+    for log2n in range(9, 15):
+        assert wgfft.GpuFft.adjoint_variant_count(1 << log2n) >= 1
+    assert wgfft.GpuFft.adjoint_variant_count(1 << 20) == 0
+    assert wgfft.GpuFft.adjoint_variant_count(12) == 0
     L = _lib.load()
     assert L.wgfft_select_variant(None, 4096, 0, None, 0) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
     assert L.wgfft_select_pass_variant(None, 1 << 20, 0, 0, None, 0) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
+    assert L.wgfft_select_adjoint_variant(None, 4096, 0, None, 0) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
 
 
 def test_synth_uniform_matches_oracle_stream():

@@ -125,3 +125,43 @@ def test_autograd_harness(gpu, torch_cuda):
     gx64, gw64 = oracle.vjp64(x, w, gy.astype(np.complex64))
     assert oracle.rel_l2(tx.grad.cpu().numpy(), gx64) <= 1e-5
     assert np.linalg.norm(tw.grad.cpu().numpy() - gw64) / np.linalg.norm(gw64) <= 1e-5
+
+
+@pytest.mark.parametrize("log2n", list(range(9, 15)))
+def test_every_fused_variant(torch_cuda, log2n):
+    """Every compiled fused forward+adjoint / VJP variant, pinned in turn,
+    matches the FP64 loss, Y, gx and gw (and the VJP entry its FP64 adjoint)
+    on a batch that leaves a ragged last round of transforms."""
+    import wgfft
+
+    torch = torch_cuda
+    n, batch = 1 << log2n, 37
+    x = oracle.synth_uniform((batch, n), 0x5EED0600 + log2n)
+    w, m = oracle.synth_window_mask(n, 0x5EED0610 + log2n)
+    loss64, gy64 = oracle.masked_loss64(x, w, m)
+    y64 = oracle.windowed_fwd64(x, w)
+    gx64, gw64 = oracle.vjp64(x, w, gy64.astype(np.complex64))
+    tx, tw_, tm = (torch.from_numpy(a).cuda() for a in (x, w, m))
+    tgy = torch.from_numpy(gy64.astype(np.complex64)).cuda()
+    for i in range(wgfft.GpuFft.adjoint_variant_count(n)):
+        g = wgfft.GpuFft.new()
+        try:
+            desc = g.select_adjoint_variant(n, i)
+            y, gx = torch.empty_like(tx), torch.empty_like(tx)
+            gw, loss = torch.empty_like(tw_), torch.zeros(1, dtype=torch.float32, device="cuda")
+            g.fwd_adjoint_masked_device(tx, tw_, tm, y, gx, gw, loss)
+            torch.cuda.synchronize()
+            e_l = abs(float(loss.item()) - loss64) / loss64
+            e_y = oracle.rel_l2(y.cpu().numpy(), y64)
+            e_gx = oracle.rel_l2(gx.cpu().numpy(), gx64)
+            e_gw = np.linalg.norm(gw.cpu().numpy() - gw64) / np.linalg.norm(gw64)
+            assert e_l <= 1e-5 and e_y <= 5e-6 and e_gx <= 1e-5 and e_gw <= 1e-5, (desc, e_l, e_y, e_gx, e_gw)
+            gx.zero_()
+            gw.zero_()
+            g.vjp_windowed_device(tx, tw_, tgy, gx, gw)
+            torch.cuda.synchronize()
+            e_gx = oracle.rel_l2(gx.cpu().numpy(), gx64)
+            e_gw = np.linalg.norm(gw.cpu().numpy() - gw64) / np.linalg.norm(gw64)
+            assert e_gx <= 5e-6 and e_gw <= 1e-5, (desc, "vjp", e_gx, e_gw)
+        finally:
+            g.close()

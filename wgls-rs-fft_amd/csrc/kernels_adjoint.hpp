@@ -42,18 +42,34 @@ __device__ __forceinline__ void load_real_pairs(float (&r)[E], const float* __re
     }
 }
 
-// KEEPX: hold x in registers from the load to the gw accumulation (one HBM
-// read, more registers) or re-read it after the adjoint transform (the
-// workgroup read it microseconds earlier, so it comes from cache; fewer
-// registers, higher occupancy).  The VJP (FWD = false) always re-reads.
-template <int N, int E, bool FWD, bool KEEPX, int... Rs>
+// Options (OPT bits):
+//  kAdjKeepX     hold x in registers from the load to the gw accumulation
+//                (one HBM read, more registers) instead of re-reading it after
+//                the adjoint transform (the workgroup read it microseconds
+//                earlier, so it comes from cache).  The VJP always re-reads.
+//  kAdjTwLds     twiddles of both transforms from two short LDS tables, so no
+//                transform waits on a global twiddle load.
+// (A register prefetch of the next transform's input on top of kAdjTwLds was
+// measured slower at N=4096: the extra registers cost a workgroup per CU.)
+constexpr int kAdjKeepX = 1, kAdjTwLds = 2;
+
+template <int N, int E, bool FWD, int OPT, int... Rs>
 __global__ void __launch_bounds__(N / E)
 k_adjoint(AdjointArgs a) {
     constexpr int T = N / E;
     constexpr int LEN = LdsLayout<N, E, Rs...>::len;
+    constexpr bool KEEPX = (OPT & kAdjKeepX) != 0;
+    constexpr bool TWLDS = (OPT & kAdjTwLds) != 0;
     using IO = Io<N, E, Rs...>;
     __shared__ __align__(16) float2 lds[LEN];
+    __shared__ __align__(16) float2 lds_tw[TWLDS ? twlds_len(ilog2(N)) : 1];
     __shared__ float red[T];
+    TwLds twl{};
+    if constexpr (TWLDS) {
+        twl = twlds_fill<N>(lds_tw, a.tw, threadIdx.x, T);
+        __syncthreads();
+    }
+    const float2* src = FWD ? a.x : a.gy;  // the input each transform starts from
 
     float acc[E];
 #pragma unroll
@@ -66,19 +82,20 @@ k_adjoint(AdjointArgs a) {
         const uint32_t t = launder_lane(threadIdx.x);  // keep per-thread addressing inside the loop
         float2 d[E];
         [[maybe_unused]] float2 xk[KEEPX && FWD ? E : 1];
+        float2 in0[E];
+        IO::load(in0, src + tr * N, t);
+        [[maybe_unused]] float wv0[FWD ? E : 1];
+        if constexpr (FWD) IO::load_real(wv0, a.w, t);
         if constexpr (FWD) {
-            {
-                float2 xs[E];
-                float wv[E];
-                IO::load(xs, a.x + tr * N, t);
-                IO::load_real(wv, a.w, t);
 #pragma unroll
-                for (int i = 0; i < E; ++i) {
-                    if constexpr (KEEPX) xk[i] = xs[i];
-                    d[i] = cscale(xs[i], wv[i]);
-                }
+            for (int i = 0; i < E; ++i) {
+                if constexpr (KEEPX) xk[i] = in0[i];
+                d[i] = cscale(in0[i], wv0[i]);
             }
-            Engine<N, E, T, false>::run(d, lds, a.tw, t, Radices<Rs...>{});
+            if constexpr (TWLDS)
+                Engine<N, E, T, false>::run_with(d, lds, twl, t, Radices<Rs...>{});
+            else
+                Engine<N, E, T, false>::run(d, lds, a.tw, t, Radices<Rs...>{});
             // keep the epilogue's loads out of the transform's register peak
             __builtin_amdgcn_sched_barrier(0);
             if (a.y) IO::template store<false>(a.y + tr * N, d, t, 1.f);
@@ -93,26 +110,39 @@ k_adjoint(AdjointArgs a) {
             WGFFT_LDS_SYNC();  // forward's last exchange read before the adjoint writes
             __builtin_amdgcn_sched_barrier(0);
         } else {
-            IO::load(d, a.gy + tr * N, t);
+#pragma unroll
+            for (int i = 0; i < E; ++i) d[i] = in0[i];
         }
-        Engine<N, E, T, true>::run(d, lds, a.tw, t, Radices<Rs...>{});
+        // the adjoint reads the same twiddle entries as the forward; an opaque
+        // table pointer makes it reload them (cache hits) instead of keeping
+        // every forward twiddle power live in registers across both transforms
+        // (the same for the thread index: the adjoint recomputes its LDS and
+        // table addresses instead of keeping the forward's live)
+        const uint32_t ta = launder_lane(t);
+        if constexpr (TWLDS)
+            Engine<N, E, T, true>::run_with(d, lds, twl, ta, Radices<Rs...>{});
+        else
+            Engine<N, E, T, true>::run(d, lds, launder(a.tw), ta, Radices<Rs...>{});
         __builtin_amdgcn_sched_barrier(0);
         // gu = d; gx = w gu; gw += Re(conj(gu) x)
+        // x and w are read again through opaque pointers: otherwise the
+        // compiler reuses the forward's loads of the same addresses and keeps
+        // x and w live in registers across both transforms
         float2 xs[E];
         if constexpr (KEEPX && FWD) {
 #pragma unroll
             for (int i = 0; i < E; ++i) xs[i] = xk[i];
         } else {
-            IO::load(xs, a.x + tr * N, t);
+            IO::load(xs, launder(a.x) + tr * N, ta);
         }
         float wv[E];
-        IO::load_real(wv, a.w, t);
+        IO::load_real(wv, launder(a.w), ta);
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             acc[i] += d[i].x * xs[i].x + d[i].y * xs[i].y;
             d[i] = cscale(d[i], wv[i]);
         }
-        IO::template store<false>(a.gx + tr * N, d, t, 1.f);
+        IO::template store<false>(a.gx + tr * N, d, ta, 1.f);
         WGFFT_LDS_SYNC();
     }
     const uint32_t t = threadIdx.x;
@@ -128,12 +158,12 @@ k_adjoint(AdjointArgs a) {
     }
 }
 
-template <int N, int E, bool KEEPX, int... Rs>
+template <int N, int E, int OPT, int... Rs>
 struct AdjointKernel {
     static constexpr int THREADS = N / E;
     template <bool FWD>
     static hipError_t go(const AdjointArgs& a) {
-        hipLaunchKernelGGL((k_adjoint<N, E, FWD, KEEPX, Rs...>), dim3((unsigned)a.grid), dim3(THREADS), 0, a.stream,
+        hipLaunchKernelGGL((k_adjoint<N, E, FWD, OPT, Rs...>), dim3((unsigned)a.grid), dim3(THREADS), 0, a.stream,
                            a);
         return hipGetLastError();
     }
@@ -141,7 +171,7 @@ struct AdjointKernel {
     template <bool FWD>
     static int occ() {
         int blocks = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_adjoint<N, E, FWD, KEEPX, Rs...>,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_adjoint<N, E, FWD, OPT, Rs...>,
                                                          THREADS, 0) != hipSuccess)
             return 1;
         return blocks > 0 ? blocks : 1;
@@ -152,7 +182,7 @@ struct AdjointKernel {
 struct AdjointVariant {
     int log2n;
     int E;
-    int keepx;
+    int opt;  // kAdj* bits
     const char* plan;
     hipError_t (*launch)(bool, const AdjointArgs&);
     int (*blocks_per_cu)(bool);
@@ -164,9 +194,9 @@ struct AdjointRegistrar {
     explicit AdjointRegistrar(const AdjointVariant& v) { adjoint_registry().push_back(v); }
 };
 
-#define WGFFT_REGISTER_ADJOINT(N, E, KEEPX, PLAN, ...)                                                 \
-    static ::wgfft::AdjointRegistrar _wgfft_adj_##N##_##E##_##KEEPX##_##PLAN(::wgfft::AdjointVariant{ \
-        ::wgfft::ilog2(N), E, KEEPX, #PLAN, &::wgfft::AdjointKernel<N, E, KEEPX, __VA_ARGS__>::launch, \
-        &::wgfft::AdjointKernel<N, E, KEEPX, __VA_ARGS__>::blocks_per_cu})
+#define WGFFT_REGISTER_ADJOINT(N, E, OPT, PLAN, ...)                                                 \
+    static ::wgfft::AdjointRegistrar _wgfft_adj_##N##_##E##_##OPT##_##PLAN(::wgfft::AdjointVariant{ \
+        ::wgfft::ilog2(N), E, OPT, #PLAN, &::wgfft::AdjointKernel<N, E, OPT, __VA_ARGS__>::launch,   \
+        &::wgfft::AdjointKernel<N, E, OPT, __VA_ARGS__>::blocks_per_cu})
 
 }  // namespace wgfft

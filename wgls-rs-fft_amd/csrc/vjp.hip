@@ -39,7 +39,8 @@ const std::vector<const AdjointVariant*> adjoint_variants(int log2n) {
 
 const char* adjoint_describe(const AdjointVariant* v, char* buf, uint64_t len) {
     if (buf && len)
-        std::snprintf(buf, len, "fused N=%d plan=%s E=%d keep_x=%d", 1 << v->log2n, v->plan, v->E, v->keepx);
+        std::snprintf(buf, len, "fused N=%d plan=%s E=%d keep_x=%d%s", 1 << v->log2n, v->plan, v->E,
+                      (v->opt & kAdjKeepX) ? 1 : 0, (v->opt & kAdjTwLds) ? " tw-lds" : "");
     return buf;
 }
 
@@ -220,6 +221,19 @@ int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x
     return WGFFT_OK;
 }
 
+// Zero fill by a kernel (not hipMemsetAsync), so the empty-batch results can
+// be recorded into a captured graph like every other launch.
+__global__ void __launch_bounds__(256) k_zero_fill(float* __restrict__ p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) p[i] = 0.f;
+}
+
+static hipError_t zero_fill(float* p, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<uint64_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_zero_fill, dim3(blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
 }  // namespace wgfft
 
 using namespace wgfft;
@@ -243,7 +257,7 @@ extern "C" int wgfft_vjp_windowed_device(wgfft_t* h, const float* d_x, const flo
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipStream_t s = (hipStream_t)stream;  // NULL: HIP default stream
     if (batch == 0) {
-        hipMemsetAsync(d_gw, 0, n * sizeof(float), s);
+        if ((e = zero_fill(d_gw, n, s)) != hipSuccess) return hip_fail(e, "zero fill of gw");
         return WGFFT_OK;
     }
     Plan* p;
@@ -273,8 +287,8 @@ extern "C" int wgfft_fwd_adjoint_masked_device(wgfft_t* h, const float* d_x, con
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipStream_t s = (hipStream_t)stream;  // NULL: HIP default stream
     if (batch == 0) {
-        hipMemsetAsync(d_gw, 0, n * sizeof(float), s);
-        hipMemsetAsync(d_loss, 0, sizeof(float), s);
+        if ((e = zero_fill(d_gw, n, s)) != hipSuccess || (e = zero_fill(d_loss, 1, s)) != hipSuccess)
+            return hip_fail(e, "zero fill of gw and loss");
         return WGFFT_OK;
     }
     Plan* p;
@@ -300,4 +314,24 @@ extern "C" int wgfft_fwd_adjoint_masked_device(wgfft_t* h, const float* d_x, con
     hipLaunchKernelGGL(k_sum_all, dim3(1), dim3(256), 0, s, (const float*)h->lossbuf.p, d_loss, blocks);
     if ((rc = h->run(p, kInverse, gy, (float2*)d_gx, nullptr, batch, 1.0f, s))) return rc;
     return vjp_finish(h, d_x, d_w, d_gx, d_gw, n, batch, s);
+}
+
+extern "C" int wgfft_adjoint_variant_count(uint64_t n) {
+    if (n == 0 || (n & (n - 1))) return 0;
+    int log2n = 0;
+    while ((1ull << log2n) < n) ++log2n;
+    return (int)adjoint_variants(log2n).size();
+}
+
+extern "C" int wgfft_select_adjoint_variant(wgfft_t* h, uint64_t n, int index, char* desc, uint64_t desc_len) {
+    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (n == 0 || (n & (n - 1))) return fail(WGFFT_ERR_INVALID_LENGTH, kMsgLength);
+    int log2n = 0;
+    while ((1ull << log2n) < n) ++log2n;
+    const auto vs = adjoint_variants(log2n);
+    if (index < 0 || index >= (int)vs.size()) return fail(WGFFT_ERR_INVALID_ARGUMENT, "variant index out of range");
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->tuned_adj[log2n] = vs[(size_t)index];
+    adjoint_describe(vs[(size_t)index], desc, desc_len);
+    return WGFFT_OK;
 }

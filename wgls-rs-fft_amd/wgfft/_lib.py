@@ -30,6 +30,7 @@ EXPORTS = (
     "wgfft_bench_device",
     "wgfft_autotune", "wgfft_plan_describe", "wgfft_variant_count", "wgfft_select_variant",
     "wgfft_pass_count", "wgfft_pass_variant_count", "wgfft_select_pass_variant",
+    "wgfft_adjoint_variant_count", "wgfft_select_adjoint_variant",
     "wgfft_synth_uniform", "wgfft_set_alloc_cap",
 )
 
@@ -95,6 +96,8 @@ def load() -> ctypes.CDLL:
     L.wgfft_pass_count.argtypes = [_u64]
     L.wgfft_pass_variant_count.argtypes = [_u64, _i]
     L.wgfft_select_pass_variant.argtypes = [_vp, _u64, _i, _i, ctypes.c_char_p, _u64]
+    L.wgfft_adjoint_variant_count.argtypes = [_u64]
+    L.wgfft_select_adjoint_variant.argtypes = [_vp, _u64, _i, ctypes.c_char_p, _u64]
     L.wgfft_synth_uniform.argtypes = [_vp, _u64, _u64]
     L.wgfft_synth_uniform.restype = None
     L.wgfft_set_alloc_cap.argtypes = [_u64]

@@ -221,6 +221,17 @@ class GpuFft(FftExecutor):
         check(self._L.wgfft_select_variant(self._h, n, index, buf, 256))
         return buf.value.decode()
 
+    @staticmethod
+    def adjoint_variant_count(n: int) -> int:
+        """Compiled fused forward+adjoint / VJP variants for length n."""
+        return int(_lib.load().wgfft_adjoint_variant_count(n))
+
+    def select_adjoint_variant(self, n: int, index: int) -> str:
+        """Pin fused variant `index` for length n (wgfft_select_adjoint_variant)."""
+        buf = ctypes.create_string_buffer(256)
+        check(self._L.wgfft_select_adjoint_variant(self._h, n, index, buf, 256))
+        return buf.value.decode()
+
 
 class RaggedPlan:
     """Mixed-length batch: transform i has length 2^log2n[i] at complex

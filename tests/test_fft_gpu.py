@@ -83,6 +83,13 @@ def test_select_variant_errors(gpu):
     with pytest.raises(wgfft.WgfftError):
         gpu.select_variant(1 << 20, 0)  # served by global passes
     assert wgfft.GpuFft.variant_count(3) == 0
+    with pytest.raises(wgfft.WgfftError):
+        gpu.select_adjoint_variant(4096, wgfft.GpuFft.adjoint_variant_count(4096))
+    with pytest.raises(wgfft.WgfftError):
+        gpu.select_adjoint_variant(1 << 20, 0)  # no fused kernel at this length
+    with pytest.raises(wgfft.WgfftError) as e:
+        gpu.select_adjoint_variant(3, 0)
+    assert e.value.code == wgfft.WGFFT_ERR_INVALID_LENGTH
 
 
 @pytest.mark.parametrize("log2n,batch", [(17, 3), (18, 2), (20, 2), (22, 2)])

@@ -118,6 +118,24 @@ def test_every_global_pass_variant(log2n, batch):
                 g.close()
 
 
+@pytest.mark.parametrize("log2n,batch", [(17, 2), (20, 2)])
+def test_every_first_pass_variant_windowed(log2n, batch):
+    """The first global pass applies the window: every candidate of that pass,
+    through the windowed entry, matches the FP64 transform of w * x."""
+    n = 1 << log2n
+    x = oracle.synth_uniform((batch, n), 0x5EED0190 + log2n)
+    w, _ = oracle.synth_window_mask(n, 0x5EED0191 + log2n)
+    want = oracle.windowed_fwd64(x, w)
+    for i in range(wgfft.GpuFft.pass_variant_count(n, 0)):
+        g = wgfft.GpuFft.new()
+        try:
+            desc = g.select_pass_variant(n, 0, i)
+            err = oracle.rel_l2(np.stack(g.fft_windowed(list(x), w)), want)
+            assert err <= rel_bar(n), (desc, err)
+        finally:
+            g.close()
+
+
 @pytest.mark.parametrize("log2n", [17, 20, 23])
 def test_multipass_autotuned(log2n):
     """The global-pass combination the autotuner keeps matches FP64."""

@@ -32,10 +32,14 @@ constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
 // iteration of a persistent loop, it keeps the per-thread twiddle loads (which
 // do not depend on the transform) inside the loop instead of being hoisted
 // and held in registers for every pass of every length the loop handles.
+// The pointer is offset by an opaque zero rather than passed through the asm
+// itself, so it keeps its global address space (global, not flat, loads:
+// a flat load also counts on lgkmcnt, and every LDS wait would wait for it).
 template <class P>
 __device__ __forceinline__ P* launder(P* p) {
-    asm volatile("" : "+s"(p));
-    return p;
+    uint32_t z = 0;
+    asm volatile("" : "+s"(z));
+    return p + z;
 }
 // The same for a per-lane value (the thread index of a persistent loop whose
 // body dispatches to many code paths): nothing derived from it is hoisted.

@@ -755,6 +755,233 @@ struct GlobalPassDirectKernel {
         ::wgfft::GlobalPassDirectKernel<R, E, G, PF, __VA_ARGS__>::LDS_BYTES, #PLAN,                     \
         &::wgfft::GlobalPassDirectKernel<R, E, G, PF, __VA_ARGS__>::launch})
 
+// ---------------------------------------------------------------------------
+// Direct global pass over 16-column tiles, i.e. 128-byte column runs, with
+// the LDS of an 8-column tile.  On MI355X a wavefront's 8-byte loads that
+// cover 64-byte runs of many rows stream at well under the rate of 128-byte
+// runs, and a tile of 16 radix-2048 columns (256 KB) does not fit in LDS.
+// Each column's sub-FFT has T = R/E virtual threads.  Real thread (g, t),
+// g = tid mod 16, t = tid / 16 < T/2, holds two slot sets: set 0 is column g
+// at virtual thread t, set 1 column g ^ 8 at virtual thread t + T/2.  A
+// wavefront's load or store of one slot of one set is then 4 rows x 16
+// adjacent columns.  The sub-FFTs run in two halves (columns 0-7, then
+// 8-15) through the same LDS, every thread working on one of its sets in
+// each half.
+// Pass twiddle bases of one column half (u = W^{v k s}, di = W^{T k s},
+// dm = W^{(R/RF) k s}) resolved to values, or the window at its E rows.
+template <int E, int MODE>
+struct GwSide {
+    float2 u, di, dm;
+    float w[MODE == kWindowed ? E : 1];
+};
+
+// Resolve the side values as soon as their loads arrive, so that two raw
+// (hi, lo) pairs are not held through a whole half.
+template <int E, int MODE>
+__device__ __forceinline__ void gw_resolve(GwSide<E, MODE>& o, const GdSide<TwTwoLevel, E, MODE>& sd, uint64_t L) {
+    if (L > 1) {
+        o.u = TwTwoLevel::value(sd.u);
+        o.di = TwTwoLevel::value(sd.di);
+        o.dm = TwTwoLevel::value(sd.dm);
+        asm volatile("" : "+v"(o.u.x), "+v"(o.u.y), "+v"(o.di.x), "+v"(o.di.y), "+v"(o.dm.x), "+v"(o.dm.y));
+    }
+    if constexpr (MODE == kWindowed) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) o.w[e] = sd.w[e];
+    }
+}
+
+template <int R, int E, int MODE, class TS, int... Rs>
+__device__ __forceinline__ void gw_half(float2 (&d)[E], const GwSide<E, MODE>& sd, const TS& ts,
+                                        uint64_t L, uint32_t c8, uint32_t v, float2* __restrict__ lds_all) {
+    using MP = GdMap<R, E, 8, Rs...>;
+    constexpr int RF = MP::RF;
+    constexpr bool INV = MODE == kInverse;
+    constexpr int SLOT = GdSlot<R, E, Rs...>::len;
+    if (L > 1) {
+        const float2 di = sd.di, dm = sd.dm;
+        float2 ui = sd.u;
+#pragma unroll
+        for (int i = 0; i < MP::NG; ++i) {
+            float2 w = ui;
+#pragma unroll
+            for (int m = 0; m < RF; ++m) {
+                d[i * RF + m] = twmul<INV>(d[i * RF + m], w);
+                if (m + 1 < RF) w = cmul(w, dm);
+            }
+            if (i + 1 < MP::NG) ui = cmul(ui, di);
+        }
+    } else if constexpr (MODE == kWindowed) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) d[e] = cscale(d[e], sd.w[e]);
+    }
+    Engine<R, E, MP::T, INV>::run_single(d, lds_all + c8 * SLOT, ts, v, Radices<Rs...>{});
+}
+
+// L = 1: the half's 8 columns (contiguous outputs, 8 R complex from
+// dst + b0h R) leave through one LDS transposition.
+template <int R, int E, int... Rs>
+__device__ __forceinline__ void gw_store_first(const float2 (&d)[E], float2* __restrict__ dst, uint64_t b0h, float sc,
+                                               uint32_t c8, uint32_t v, float2* __restrict__ lds_all, uint32_t tid) {
+    using MP = GdMap<R, E, 8, Rs...>;
+    constexpr int THREADS = 8 * MP::T;
+    constexpr int SLOT = GdSlot<R, E, Rs...>::len;
+    WGFFT_LDS_SYNC();  // the sub-FFT's last exchange has been read
+    // opaque here, so no address of this step is computed early and held
+    v = launder_lane(v);
+    tid = launder_lane(tid);
+    asm volatile("" : "+s"(b0h));
+    float2* lds = lds_all + c8 * SLOT;
+#pragma unroll
+    for (int e = 0; e < E; ++e) lds[pad_idx<5, 0>(MP::row(e, v))] = d[e];
+    WGFFT_LDS_SYNC();
+#pragma unroll
+    for (int e = 0; e < E / 2; ++e) {
+        const uint32_t f = 2 * (tid + THREADS * e);  // element pair f, f + 1 of the half's 8 x R outputs
+        const uint32_t gg = f / R, q = f % R;
+        const float2 a = cscale(lds_all[gg * SLOT + pad_idx<5, 0>(q)], sc);
+        const float2 b = cscale(lds_all[gg * SLOT + pad_idx<5, 0>(q + 1)], sc);
+        st16<true>(reinterpret_cast<float4*>(dst + (b0h + gg) * R + q), make_float4(a.x, a.y, b.x, b.y));
+    }
+}
+
+__device__ __forceinline__ void swap_if(float2& a, float2& b, bool c) {
+    const float2 x = c ? b : a, y = c ? a : b;
+    a = x;
+    b = y;
+}
+
+template <int R, int E, int MODE, int... Rs>
+__global__ void __launch_bounds__(8 * (R / E))
+k_global_pass_wide(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw_r,
+                   const float2* __restrict__ tw_hi, const float2* __restrict__ tw_lo,
+                   const float* __restrict__ win, uint32_t log2n, uint32_t h, uint64_t L, uint64_t batch,
+                   float scale) {
+    using MP = GdMap<R, E, 8, Rs...>;
+    constexpr int T = MP::T;       // virtual threads per column
+    constexpr int TH = T / 2;      // real threads per column pair
+    constexpr int THREADS = 16 * TH;
+    constexpr int SLOT = GdSlot<R, E, Rs...>::len;
+    constexpr int G = 16;
+    __shared__ __align__(16) float2 lds_all[8 * SLOT];
+    __shared__ __align__(16) float2 lds_tw[twlds_len(ilog2(R))];
+
+    const uint64_t N = 1ull << log2n;
+    const uint64_t NR = N / R;
+    const uint64_t groups_per_tr = NR / G;
+    const uint64_t total = groups_per_tr * batch;
+    const uint64_t nb = gridDim.x;  // a multiple of 8
+    const uint64_t pid = blockIdx.x;
+    const uint64_t lid0 = (pid & 7) * (nb >> 3) + (pid >> 3);
+    const bool last = (L * R == N);
+    const float sc = last ? scale : 1.0f;
+    const TwTwoLevel twN{tw_hi, tw_lo, h};
+    const TwLds twr = twlds_fill<R>(lds_tw, tw_r, threadIdx.x, THREADS);
+    __syncthreads();
+    using Side = GdSide<TwTwoLevel, E, MODE>;
+    for (uint64_t lid = lid0; lid < total; lid += nb) {
+        const uint64_t tr = lid / groups_per_tr, b0 = (lid % groups_per_tr) * G;
+        const uint32_t tid = launder_lane(threadIdx.x);
+        const uint32_t g = tid % G, t = tid / G, c8 = g & 7;
+        const bool hi = g >= 8;
+        // virtual threads of this thread in the halves (columns b0 + c8, b0 + 8 + c8)
+        const uint32_t va = t + (hi ? TH : 0), vb = t + (hi ? 0 : TH);
+        Side ra, rb;
+        gd_load_side<R, E, 8, MODE, TwTwoLevel, Rs...>(ra, twN, win, N, L, b0, c8 + 8 * va);
+        gd_load_side<R, E, 8, MODE, TwTwoLevel, Rs...>(rb, twN, win, N, L, b0 + 8, c8 + 8 * vb);
+        float2 s0[E], s1[E];
+        {
+            typedef float f2v __attribute__((ext_vector_type(2)));
+            const float2* src = in + tr * N + b0;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const f2v x0 = __builtin_nontemporal_load(
+                    reinterpret_cast<const f2v*>(src + g + (uint64_t)MP::row(e, t) * NR));
+                const f2v x1 = __builtin_nontemporal_load(
+                    reinterpret_cast<const f2v*>(src + (g ^ 8) + (uint64_t)MP::row(e, t + TH) * NR));
+                s0[e] = make_float2(x0.x, x0.y);
+                s1[e] = make_float2(x1.x, x1.y);
+            }
+        }
+        GwSide<E, MODE> sa, sb;
+        gw_resolve(sa, ra, L);
+        gw_resolve(sb, rb, L);
+        // s0: half A (column c8, virtual va), s1: half B (column 8 + c8, virtual vb)
+#pragma unroll
+        for (int e = 0; e < E; ++e) swap_if(s0[e], s1[e], hi);
+        float2* dst = out + tr * N;
+        gw_half<R, E, MODE, TwLds, Rs...>(s0, sa, twr, L, c8, va, lds_all);
+        if (L == 1) gw_store_first<R, E, Rs...>(s0, dst, b0, sc, c8, va, lds_all, tid);
+        WGFFT_LDS_SYNC();
+        // opaque after the barrier: nothing of half B's sub-FFT is computed
+        // early and held across half A
+        const uint32_t vb_ = launder_lane(vb), c8_ = launder_lane(c8);
+        gw_half<R, E, MODE, TwLds, Rs...>(s1, sb, twr, L, c8_, vb_, lds_all);
+        if (L == 1) {
+            gw_store_first<R, E, Rs...>(s1, dst, b0 + 8, sc, c8_, vb_, lds_all, launder_lane(tid));
+        } else {
+            // outputs b + m L of butterflies b0 .. b0 + 15, stored in the load pattern
+#pragma unroll
+            for (int e = 0; e < E; ++e) swap_if(s0[e], s1[e], hi);
+            const uint64_t bj = b0 / L, k = b0 & (L - 1);
+            uint64_t ob = bj * R * L + k;
+            asm volatile("" : "+s"(ob));
+            float2* o = dst + ob;
+            const uint32_t tid2 = launder_lane(tid);
+            const uint32_t gs = tid2 % G, ts = tid2 / G;
+            typedef float f2v __attribute__((ext_vector_type(2)));
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const float2 v0 = cscale(s0[e], sc), v1 = cscale(s1[e], sc);
+                const f2v n0 = {v0.x, v0.y}, n1 = {v1.x, v1.y};
+                __builtin_nontemporal_store(n0, reinterpret_cast<f2v*>(o + gs + (uint64_t)MP::row(e, ts) * L));
+                __builtin_nontemporal_store(n1, reinterpret_cast<f2v*>(o + (gs ^ 8) + (uint64_t)MP::row(e, ts + TH) * L));
+            }
+        }
+        WGFFT_LDS_SYNC();
+    }
+}
+
+template <int R, int E, int... Rs>
+struct GlobalPassWideKernel {
+    static constexpr int T = R / E;
+    static constexpr int THREADS = 8 * T;
+    static constexpr int LDS_BYTES = 8 * GdSlot<R, E, Rs...>::len * 8 + twlds_len(ilog2(R)) * 8;
+
+    template <int MODE>
+    static hipError_t go(const GlobalPassArgs& a) {
+        const uint64_t N = 1ull << a.log2n;
+        const uint64_t groups = (N / R) / 16 * a.batch;
+        if (groups == 0) return hipSuccess;
+        static const int per_cu = [] {
+            int blocks = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_global_pass_wide<R, E, MODE, Rs...>,
+                                                             THREADS, 0) != hipSuccess || blocks < 1)
+                blocks = 1;
+            return blocks;
+        }();
+        uint64_t grid = std::min<uint64_t>(groups, (uint64_t)per_cu * (uint64_t)a.num_cus);
+        if (grid % 8) grid += 8 - grid % 8;  // keep the XCD numbering a bijection; extra blocks exit
+        hipLaunchKernelGGL((k_global_pass_wide<R, E, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0, a.stream,
+                           a.in, a.out, a.tw_r, a.tw_hi, a.tw_lo, a.win, a.log2n, a.h, a.L, a.batch, a.scale);
+        return hipGetLastError();
+    }
+    static hipError_t launch(Mode m, const GlobalPassArgs& a) {
+        switch (m) {
+            case kForward: return go<kForward>(a);
+            case kInverse: return go<kInverse>(a);
+            default: return go<kWindowed>(a);
+        }
+    }
+};
+
+// prefetch = 5 in the registry marks the 16-column direct pipeline.
+#define WGFFT_REGISTER_GLOBAL_PASS_WIDE(R, E, PLAN, ...)                                               \
+    static ::wgfft::Registrar _wgfft_gpw_##R##_##E##_##PLAN(::wgfft::GlobalPassVariant{              \
+        ::wgfft::ilog2(R), E, 16, 5, ::wgfft::GlobalPassWideKernel<R, E, __VA_ARGS__>::THREADS,        \
+        ::wgfft::GlobalPassWideKernel<R, E, __VA_ARGS__>::LDS_BYTES, #PLAN,                            \
+        &::wgfft::GlobalPassWideKernel<R, E, __VA_ARGS__>::launch})
+
 // Persistent multi-pass kernel (tiles read and written non-temporally: each
 // element is touched once per pass): workgroups loop over the groups of all
 // transforms in the batch with the next group's tile prefetched into

@@ -158,6 +158,7 @@ static std::string describe_multipass(const Plan* p) {
         s += (i ? "," : "") + std::string("R") + std::to_string(1 << v->log2r) + "(" + v->plan + ",G" +
              std::to_string(v->G) + (v->prefetch == 3 ? ",direct"
                                       : v->prefetch == 4 ? ",direct,prefetch"
+                                      : v->prefetch == 5 ? ",direct,halves"
                                       : v->prefetch      ? ",prefetch"
                                                          : "") + ")";
     }

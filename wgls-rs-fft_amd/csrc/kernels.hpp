@@ -757,113 +757,112 @@ struct GlobalPassDirectKernel {
 
 // ---------------------------------------------------------------------------
 // Direct global pass over 16-column tiles, i.e. 128-byte column runs, with
-// the LDS of an 8-column tile.  On MI355X a wavefront's 8-byte loads that
+// the LDS of C = 16/Q columns.  On MI355X a wavefront's 8-byte loads that
 // cover 64-byte runs of many rows stream at well under the rate of 128-byte
 // runs, and a tile of 16 radix-2048 columns (256 KB) does not fit in LDS.
-// Each column's sub-FFT has T = R/E virtual threads.  Real thread (g, t),
-// g = tid mod 16, t = tid / 16 < T/2, holds two slot sets: set 0 is column g
-// at virtual thread t, set 1 column g ^ 8 at virtual thread t + T/2.  A
+// Each column's sub-FFT has T = R/E virtual threads; the workgroup has C T
+// threads.  Real thread (g, t), g = tid mod 16, t = tid / 16 < T/Q, holds Q
+// slot sets: set s is column g ^ (C s) at virtual thread t + (T/Q) s.  A
 // wavefront's load or store of one slot of one set is then 4 rows x 16
-// adjacent columns.  The sub-FFTs run in two halves (columns 0-7, then
-// 8-15) through the same LDS, every thread working on one of its sets in
-// each half.
-// Pass twiddle bases of one column half (u = W^{v k s}, di = W^{T k s},
-// dm = W^{(R/RF) k s}) resolved to values, or the window at its E rows.
-template <int E, int MODE>
-struct GwSide {
-    float2 u, di, dm;
-    float w[MODE == kWindowed ? E : 1];
-};
+// adjacent columns.  The sub-FFTs run in Q rounds of C columns (round q:
+// columns C q .. C q + C - 1) through the same LDS; in round q thread (g, t)
+// works on its set q ^ (g / C), which an XOR permutation of the sets by
+// g / C (conditional swaps, one stage per bit) moves to index q.
 
-// Resolve the side values as soon as their loads arrive, so that two raw
-// (hi, lo) pairs are not held through a whole half.
-template <int E, int MODE>
-__device__ __forceinline__ void gw_resolve(GwSide<E, MODE>& o, const GdSide<TwTwoLevel, E, MODE>& sd, uint64_t L) {
-    if (L > 1) {
-        o.u = TwTwoLevel::value(sd.u);
-        o.di = TwTwoLevel::value(sd.di);
-        o.dm = TwTwoLevel::value(sd.dm);
-        asm volatile("" : "+v"(o.u.x), "+v"(o.u.y), "+v"(o.di.x), "+v"(o.di.y), "+v"(o.dm.x), "+v"(o.dm.y));
-    }
-    if constexpr (MODE == kWindowed) {
+// d[i] <- d[i ^ a] for i < Q (a < Q), one conditional-swap stage per bit of a
+template <int Q, int E>
+__device__ __forceinline__ void gw_permute(float2 (&d)[Q][E], uint32_t a) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) o.w[e] = sd.w[e];
+    for (int b = 1; b < Q; b <<= 1) {
+        const bool c = (a & b) != 0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            if (i & b) continue;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const float2 x = c ? d[i + b][e] : d[i][e], y = c ? d[i][e] : d[i + b][e];
+                d[i][e] = x;
+                d[i + b][e] = y;
+            }
+        }
     }
 }
 
-template <int R, int E, int MODE, class TS, int... Rs>
-__device__ __forceinline__ void gw_half(float2 (&d)[E], const GwSide<E, MODE>& sd, const TS& ts,
-                                        uint64_t L, uint32_t c8, uint32_t v, float2* __restrict__ lds_all) {
-    using MP = GdMap<R, E, 8, Rs...>;
+// Pass twiddle bases of one column (u = W^{v k s}, di = W^{T k s},
+// dm = W^{(R/RF) k s}) resolved to values.
+struct GwTw {
+    float2 u, di, dm;
+};
+
+// The sub-FFT of one round for the thread's column c (slot c mod C of the
+// LDS) at virtual thread v, with the pass twiddles applied first (L > 1).
+template <int R, int E, int Q, bool INV, class TS, int... Rs>
+__device__ __forceinline__ void gw_round(float2 (&d)[E], const GwTw& tw, const TS& ts, uint64_t L, uint32_t cc,
+                                         uint32_t v, float2* __restrict__ lds_all) {
+    using MP = GdMap<R, E, 16 / Q, Rs...>;
     constexpr int RF = MP::RF;
-    constexpr bool INV = MODE == kInverse;
     constexpr int SLOT = GdSlot<R, E, Rs...>::len;
     if (L > 1) {
-        const float2 di = sd.di, dm = sd.dm;
-        float2 ui = sd.u;
+        float2 ui = tw.u;
 #pragma unroll
         for (int i = 0; i < MP::NG; ++i) {
             float2 w = ui;
 #pragma unroll
             for (int m = 0; m < RF; ++m) {
                 d[i * RF + m] = twmul<INV>(d[i * RF + m], w);
-                if (m + 1 < RF) w = cmul(w, dm);
+                if (m + 1 < RF) w = cmul(w, tw.dm);
             }
-            if (i + 1 < MP::NG) ui = cmul(ui, di);
+            if (i + 1 < MP::NG) ui = cmul(ui, tw.di);
         }
-    } else if constexpr (MODE == kWindowed) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) d[e] = cscale(d[e], sd.w[e]);
     }
-    Engine<R, E, MP::T, INV>::run_single(d, lds_all + c8 * SLOT, ts, v, Radices<Rs...>{});
+    Engine<R, E, MP::T, INV>::run_single(d, lds_all + cc * SLOT, ts, v, Radices<Rs...>{});
 }
 
-// L = 1: the half's 8 columns (contiguous outputs, 8 R complex from
-// dst + b0h R) leave through one LDS transposition.
-template <int R, int E, int... Rs>
-__device__ __forceinline__ void gw_store_first(const float2 (&d)[E], float2* __restrict__ dst, uint64_t b0h, float sc,
-                                               uint32_t c8, uint32_t v, float2* __restrict__ lds_all, uint32_t tid) {
-    using MP = GdMap<R, E, 8, Rs...>;
-    constexpr int THREADS = 8 * MP::T;
+// L = 1: the round's C columns (contiguous outputs, C R complex from
+// dst + c0 R) leave through one LDS transposition.
+template <int R, int E, int Q, int... Rs>
+__device__ __forceinline__ void gw_store_first(const float2 (&d)[E], float2* __restrict__ dst, uint64_t c0, float sc,
+                                               uint32_t cc, uint32_t v, float2* __restrict__ lds_all, uint32_t tid) {
+    constexpr int C = 16 / Q;
+    using MP = GdMap<R, E, C, Rs...>;
+    constexpr int THREADS = C * MP::T;
     constexpr int SLOT = GdSlot<R, E, Rs...>::len;
     WGFFT_LDS_SYNC();  // the sub-FFT's last exchange has been read
     // opaque here, so no address of this step is computed early and held
     v = launder_lane(v);
     tid = launder_lane(tid);
-    asm volatile("" : "+s"(b0h));
-    float2* lds = lds_all + c8 * SLOT;
+    asm volatile("" : "+s"(c0));
+    float2* lds = lds_all + cc * SLOT;
 #pragma unroll
     for (int e = 0; e < E; ++e) lds[pad_idx<5, 0>(MP::row(e, v))] = d[e];
     WGFFT_LDS_SYNC();
 #pragma unroll
     for (int e = 0; e < E / 2; ++e) {
-        const uint32_t f = 2 * (tid + THREADS * e);  // element pair f, f + 1 of the half's 8 x R outputs
+        const uint32_t f = 2 * (tid + THREADS * e);  // element pair f, f + 1 of the round's C x R outputs
         const uint32_t gg = f / R, q = f % R;
         const float2 a = cscale(lds_all[gg * SLOT + pad_idx<5, 0>(q)], sc);
         const float2 b = cscale(lds_all[gg * SLOT + pad_idx<5, 0>(q + 1)], sc);
-        st16<true>(reinterpret_cast<float4*>(dst + (b0h + gg) * R + q), make_float4(a.x, a.y, b.x, b.y));
+        st16<true>(reinterpret_cast<float4*>(dst + (c0 + gg) * R + q), make_float4(a.x, a.y, b.x, b.y));
     }
 }
 
-__device__ __forceinline__ void swap_if(float2& a, float2& b, bool c) {
-    const float2 x = c ? b : a, y = c ? a : b;
-    a = x;
-    b = y;
-}
-
-template <int R, int E, int MODE, int... Rs>
-__global__ void __launch_bounds__(8 * (R / E))
+template <int R, int E, int Q, int MODE, int... Rs>
+__global__ void __launch_bounds__((16 / Q) * (R / E))
 k_global_pass_wide(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw_r,
                    const float2* __restrict__ tw_hi, const float2* __restrict__ tw_lo,
                    const float* __restrict__ win, uint32_t log2n, uint32_t h, uint64_t L, uint64_t batch,
                    float scale) {
-    using MP = GdMap<R, E, 8, Rs...>;
-    constexpr int T = MP::T;       // virtual threads per column
-    constexpr int TH = T / 2;      // real threads per column pair
-    constexpr int THREADS = 16 * TH;
-    constexpr int SLOT = GdSlot<R, E, Rs...>::len;
     constexpr int G = 16;
-    __shared__ __align__(16) float2 lds_all[8 * SLOT];
+    constexpr int C = G / Q;         // columns per round
+    using MP = GdMap<R, E, C, Rs...>;
+    constexpr int T = MP::T;         // virtual threads per column
+    constexpr int TQ = T / Q;        // real threads per column group of a row run
+    constexpr int THREADS = C * T;
+    constexpr int SLOT = GdSlot<R, E, Rs...>::len;
+    constexpr bool INV = MODE == kInverse;
+    constexpr int TWMODE = MODE == kWindowed ? kForward : MODE;  // the window is applied at load
+    static_assert(Q == 2 || Q == 4, "two or four rounds");
+    __shared__ __align__(16) float2 lds_all[C * SLOT];
     __shared__ __align__(16) float2 lds_tw[twlds_len(ilog2(R))];
 
     const uint64_t N = 1ull << log2n;
@@ -878,75 +877,81 @@ k_global_pass_wide(const float2* __restrict__ in, float2* __restrict__ out, cons
     const TwTwoLevel twN{tw_hi, tw_lo, h};
     const TwLds twr = twlds_fill<R>(lds_tw, tw_r, threadIdx.x, THREADS);
     __syncthreads();
-    using Side = GdSide<TwTwoLevel, E, MODE>;
+    typedef float f2v __attribute__((ext_vector_type(2)));
     for (uint64_t lid = lid0; lid < total; lid += nb) {
         const uint64_t tr = lid / groups_per_tr, b0 = (lid % groups_per_tr) * G;
         const uint32_t tid = launder_lane(threadIdx.x);
-        const uint32_t g = tid % G, t = tid / G, c8 = g & 7;
-        const bool hi = g >= 8;
-        // virtual threads of this thread in the halves (columns b0 + c8, b0 + 8 + c8)
-        const uint32_t va = t + (hi ? TH : 0), vb = t + (hi ? 0 : TH);
-        Side ra, rb;
-        gd_load_side<R, E, 8, MODE, TwTwoLevel, Rs...>(ra, twN, win, N, L, b0, c8 + 8 * va);
-        gd_load_side<R, E, 8, MODE, TwTwoLevel, Rs...>(rb, twN, win, N, L, b0 + 8, c8 + 8 * vb);
-        float2 s0[E], s1[E];
-        {
-            typedef float f2v __attribute__((ext_vector_type(2)));
-            const float2* src = in + tr * N + b0;
+        const uint32_t g = tid % G, t = tid / G, a = g / C, cc = g % C;
+        // round q: column C q + cc at virtual thread t + TQ (q ^ a)
+        float2 d[Q][E];
+        const float2* src = in + tr * N + b0;
+#pragma unroll
+        for (int s = 0; s < Q; ++s)
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                const f2v x0 = __builtin_nontemporal_load(
-                    reinterpret_cast<const f2v*>(src + g + (uint64_t)MP::row(e, t) * NR));
-                const f2v x1 = __builtin_nontemporal_load(
-                    reinterpret_cast<const f2v*>(src + (g ^ 8) + (uint64_t)MP::row(e, t + TH) * NR));
-                s0[e] = make_float2(x0.x, x0.y);
-                s1[e] = make_float2(x1.x, x1.y);
+                const f2v x = __builtin_nontemporal_load(
+                    reinterpret_cast<const f2v*>(src + (g ^ (C * s)) + (uint64_t)MP::row(e, t + TQ * s) * NR));
+                d[s][e] = make_float2(x.x, x.y);
+            }
+        if constexpr (MODE == kWindowed) {
+            if (L == 1) {
+                const float* wsrc = win + b0;
+#pragma unroll
+                for (int s = 0; s < Q; ++s)
+#pragma unroll
+                    for (int e = 0; e < E; ++e)
+                        d[s][e] = cscale(d[s][e], wsrc[(g ^ (C * s)) + (uint64_t)MP::row(e, t + TQ * s) * NR]);
             }
         }
-        GwSide<E, MODE> sa, sb;
-        gw_resolve(sa, ra, L);
-        gw_resolve(sb, rb, L);
-        // s0: half A (column c8, virtual va), s1: half B (column 8 + c8, virtual vb)
-#pragma unroll
-        for (int e = 0; e < E; ++e) swap_if(s0[e], s1[e], hi);
+        gw_permute<Q, E>(d, a);  // d[q]: round q's set
         float2* dst = out + tr * N;
-        gw_half<R, E, MODE, TwLds, Rs...>(s0, sa, twr, L, c8, va, lds_all);
-        if (L == 1) gw_store_first<R, E, Rs...>(s0, dst, b0, sc, c8, va, lds_all, tid);
-        WGFFT_LDS_SYNC();
-        // opaque after the barrier: nothing of half B's sub-FFT is computed
-        // early and held across half A
-        const uint32_t vb_ = launder_lane(vb), c8_ = launder_lane(c8);
-        gw_half<R, E, MODE, TwLds, Rs...>(s1, sb, twr, L, c8_, vb_, lds_all);
-        if (L == 1) {
-            gw_store_first<R, E, Rs...>(s1, dst, b0 + 8, sc, c8_, vb_, lds_all, launder_lane(tid));
-        } else {
-            // outputs b + m L of butterflies b0 .. b0 + 15, stored in the load pattern
 #pragma unroll
-            for (int e = 0; e < E; ++e) swap_if(s0[e], s1[e], hi);
+        for (int q = 0; q < Q; ++q) {
+            // opaque per round: nothing of a later round's sub-FFT is computed
+            // early and held across an earlier one
+            const uint32_t v = launder_lane(t + TQ * (q ^ a)), c = launder_lane(cc);
+            // pass twiddle bases of the round, loaded when the round starts (held
+            // for every round from before the tile loads, they cost spills)
+            GwTw twq{};
+            if (L > 1) {
+                GdSide<TwTwoLevel, E, TWMODE> raw;
+                gd_load_side<R, E, C, TWMODE, TwTwoLevel, Rs...>(raw, twN, nullptr, N, L, b0 + C * q, c + C * v);
+                twq.u = TwTwoLevel::value(raw.u);
+                twq.di = TwTwoLevel::value(raw.di);
+                twq.dm = TwTwoLevel::value(raw.dm);
+            }
+            gw_round<R, E, Q, INV, TwLds, Rs...>(d[q], twq, twr, L, c, v, lds_all);
+            if (L == 1) gw_store_first<R, E, Q, Rs...>(d[q], dst, b0 + C * q, sc, c, v, lds_all, tid);
+            if (q + 1 < Q) WGFFT_LDS_SYNC();
+        }
+        if (L > 1) {
+            // outputs b + m L of butterflies b0 .. b0 + 15, stored in the load pattern
+            gw_permute<Q, E>(d, a);
             const uint64_t bj = b0 / L, k = b0 & (L - 1);
             uint64_t ob = bj * R * L + k;
             asm volatile("" : "+s"(ob));
             float2* o = dst + ob;
             const uint32_t tid2 = launder_lane(tid);
             const uint32_t gs = tid2 % G, ts = tid2 / G;
-            typedef float f2v __attribute__((ext_vector_type(2)));
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const float2 v0 = cscale(s0[e], sc), v1 = cscale(s1[e], sc);
-                const f2v n0 = {v0.x, v0.y}, n1 = {v1.x, v1.y};
-                __builtin_nontemporal_store(n0, reinterpret_cast<f2v*>(o + gs + (uint64_t)MP::row(e, ts) * L));
-                __builtin_nontemporal_store(n1, reinterpret_cast<f2v*>(o + (gs ^ 8) + (uint64_t)MP::row(e, ts + TH) * L));
-            }
+            for (int s = 0; s < Q; ++s)
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const float2 v = cscale(d[s][e], sc);
+                    const f2v nv = {v.x, v.y};
+                    __builtin_nontemporal_store(
+                        nv, reinterpret_cast<f2v*>(o + (gs ^ (C * s)) + (uint64_t)MP::row(e, ts + TQ * s) * L));
+                }
         }
         WGFFT_LDS_SYNC();
     }
 }
 
-template <int R, int E, int... Rs>
+template <int R, int E, int Q, int... Rs>
 struct GlobalPassWideKernel {
     static constexpr int T = R / E;
-    static constexpr int THREADS = 8 * T;
-    static constexpr int LDS_BYTES = 8 * GdSlot<R, E, Rs...>::len * 8 + twlds_len(ilog2(R)) * 8;
+    static constexpr int THREADS = (16 / Q) * T;
+    static constexpr int LDS_BYTES = (16 / Q) * GdSlot<R, E, Rs...>::len * 8 + twlds_len(ilog2(R)) * 8;
 
     template <int MODE>
     static hipError_t go(const GlobalPassArgs& a) {
@@ -955,15 +960,17 @@ struct GlobalPassWideKernel {
         if (groups == 0) return hipSuccess;
         static const int per_cu = [] {
             int blocks = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_global_pass_wide<R, E, MODE, Rs...>,
-                                                             THREADS, 0) != hipSuccess || blocks < 1)
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &blocks, (const void*)k_global_pass_wide<R, E, Q, MODE, Rs...>, THREADS, 0) != hipSuccess ||
+                blocks < 1)
                 blocks = 1;
             return blocks;
         }();
         uint64_t grid = std::min<uint64_t>(groups, (uint64_t)per_cu * (uint64_t)a.num_cus);
         if (grid % 8) grid += 8 - grid % 8;  // keep the XCD numbering a bijection; extra blocks exit
-        hipLaunchKernelGGL((k_global_pass_wide<R, E, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0, a.stream,
-                           a.in, a.out, a.tw_r, a.tw_hi, a.tw_lo, a.win, a.log2n, a.h, a.L, a.batch, a.scale);
+        hipLaunchKernelGGL((k_global_pass_wide<R, E, Q, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0,
+                           a.stream, a.in, a.out, a.tw_r, a.tw_hi, a.tw_lo, a.win, a.log2n, a.h, a.L, a.batch,
+                           a.scale);
         return hipGetLastError();
     }
     static hipError_t launch(Mode m, const GlobalPassArgs& a) {
@@ -975,12 +982,13 @@ struct GlobalPassWideKernel {
     }
 };
 
-// prefetch = 5 in the registry marks the 16-column direct pipeline.
-#define WGFFT_REGISTER_GLOBAL_PASS_WIDE(R, E, PLAN, ...)                                               \
-    static ::wgfft::Registrar _wgfft_gpw_##R##_##E##_##PLAN(::wgfft::GlobalPassVariant{              \
-        ::wgfft::ilog2(R), E, 16, 5, ::wgfft::GlobalPassWideKernel<R, E, __VA_ARGS__>::THREADS,        \
-        ::wgfft::GlobalPassWideKernel<R, E, __VA_ARGS__>::LDS_BYTES, #PLAN,                            \
-        &::wgfft::GlobalPassWideKernel<R, E, __VA_ARGS__>::launch})
+// prefetch = 5 (two rounds) or 6 (four rounds) in the registry marks the
+// 16-column direct pipeline.
+#define WGFFT_REGISTER_GLOBAL_PASS_WIDE(R, E, Q, PLAN, ...)                                            \
+    static ::wgfft::Registrar _wgfft_gpw_##R##_##E##_##Q##_##PLAN(::wgfft::GlobalPassVariant{        \
+        ::wgfft::ilog2(R), E, 16, Q == 2 ? 5 : 6, ::wgfft::GlobalPassWideKernel<R, E, Q, __VA_ARGS__>::THREADS, \
+        ::wgfft::GlobalPassWideKernel<R, E, Q, __VA_ARGS__>::LDS_BYTES, #PLAN,                         \
+        &::wgfft::GlobalPassWideKernel<R, E, Q, __VA_ARGS__>::launch})
 
 // Persistent multi-pass kernel (tiles read and written non-temporally: each
 // element is touched once per pass): workgroups loop over the groups of all

@@ -159,6 +159,7 @@ static std::string describe_multipass(const Plan* p) {
              std::to_string(v->G) + (v->prefetch == 3 ? ",direct"
                                       : v->prefetch == 4 ? ",direct,prefetch"
                                       : v->prefetch == 5 ? ",direct,halves"
+                                      : v->prefetch == 6 ? ",direct,quarters"
                                       : v->prefetch      ? ",prefetch"
                                                          : "") + ")";
     }

@@ -47,6 +47,7 @@ WGFFT_REGISTER_GLOBAL_PASS_DIRECT(2048, 32, 8, 1, 16x8x16, 16, 8, 16);
 WGFFT_REGISTER_GLOBAL_PASS_DIRECT(4096, 16, 2, 0, 16x16x16, 16, 16, 16);
 
 // 16-column direct tiles (128-byte runs) with the LDS of 8 columns: the
-// sub-FFTs run in two halves.
-WGFFT_REGISTER_GLOBAL_PASS_WIDE(1024, 16, 16x4x16, 16, 4, 16);
-WGFFT_REGISTER_GLOBAL_PASS_WIDE(2048, 32, 16x8x16, 16, 8, 16);
+// sub-FFTs run in two rounds.  (Four rounds of 4 columns at E = 16 were
+// measured slower at radix 2048.)
+WGFFT_REGISTER_GLOBAL_PASS_WIDE(1024, 16, 2, 16x4x16, 16, 4, 16);
+WGFFT_REGISTER_GLOBAL_PASS_WIDE(2048, 32, 2, 16x8x16, 16, 8, 16);

@@ -60,17 +60,21 @@ def test_all_lengths_vs_fp64_and_reference(gpu, log2n):
 def test_every_compiled_variant(log2n):
     """Each compiled single-kernel variant (radix split, E, G, store and
     twiddle-source flags), pinned in turn, matches FP64 forward and inverse
-    on a batch that leaves a partial group."""
+    on a batch that leaves a partial group, and the windowed forward."""
     n = 1 << log2n
     count = wgfft.GpuFft.variant_count(n)
     assert count >= 1
     x = oracle.synth_uniform((max(3, min(45, (1 << 18) // n)), n), 0x5EED0150 + log2n)
+    w, _ = oracle.synth_window_mask(n, 0x5EED0151 + log2n)
+    want_w = oracle.windowed_fwd64(x[:2], w)
     for i in range(count):
         g = wgfft.GpuFft.new()
         desc = g.select_variant(n, i)
         assert g.describe(n) == desc
         try:
             check_fwd_inv(g, x)
+            err = oracle.rel_l2(np.stack(g.fft_windowed(list(x[:2]), w)), want_w)
+            assert err <= rel_bar(n), ("windowed", err)
         except AssertionError as e:
             raise AssertionError(f"variant {i}: {desc}: {e}") from None
         finally:

@@ -41,6 +41,7 @@ constexpr int kFlagNtStore = 1;  // non-temporal output stores
 constexpr int kFlagTwRegs = 2;   // twiddle base powers held in registers across transforms
 constexpr int kFlagTwLds = 4;    // twiddle base powers from two short LDS tables
 constexpr int kFlagNoPrefetch = 8;  // no register prefetch of the next transform (fewer registers)
+constexpr int kFlagSplit2 = 256;    // k_split2: a radix-2 stage, then the two halves in turn through LDS
 // bits 4..7: minimum waves per SIMD the register allocation must allow (0: compiler's choice)
 constexpr int flag_min_waves(int fl) { return (fl >> 4) & 15; }
 
@@ -989,6 +990,120 @@ struct GlobalPassWideKernel {
         ::wgfft::ilog2(R), E, 16, Q == 2 ? 5 : 6, ::wgfft::GlobalPassWideKernel<R, E, Q, __VA_ARGS__>::THREADS, \
         ::wgfft::GlobalPassWideKernel<R, E, Q, __VA_ARGS__>::LDS_BYTES, #PLAN,                         \
         &::wgfft::GlobalPassWideKernel<R, E, Q, __VA_ARGS__>::launch})
+
+// ---------------------------------------------------------------------------
+// N = 2 H transforms with the LDS of an H-point one: a decimation-in-frequency
+// radix-2 stage in registers, y0[n] = x[n] + x[n + H] and
+// y1[n] = (x[n] - x[n + H]) W_N^n, then the two H-point sub-FFTs in turn
+// through one LDS footprint, and X[2k] = Y0[k], X[2k + 1] = Y1[k] stored as
+// 16-byte pairs.  At N = 16384 the footprint is 64 KB instead of 128 KB, so
+// two workgroups share a CU and one's loads overlap the other's transform.
+// Thread t holds the single-layout slots of its H-point sub-FFTs: slot
+// i RF + m is index t + T i + m H/RF of both halves (RF the first radix).
+template <int N, int E, int MODE, int... Rs>
+__global__ void __launch_bounds__((N / 2) / E) __attribute__((amdgpu_waves_per_eu(4)))
+k_split2(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
+         const float* __restrict__ win, uint64_t batch, float scale) {
+    constexpr int H = N / 2;
+    constexpr int T = H / E;
+    constexpr bool INV = MODE == kInverse;
+    using MP = GdMap<H, E, 1, Rs...>;
+    using EN = Engine<H, E, T, INV>;
+    __shared__ __align__(16) float2 lds[LdsLayoutSingle<H, E, Rs...>::len];
+    __shared__ __align__(16) float2 lds_tw[twlds_len(ilog2(N))];
+    const TwLds twn = twlds_fill<N>(lds_tw, tw, threadIdx.x, T);
+    __syncthreads();
+    const TwLds twh = twn.shifted(1);  // W_H for the halves
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    for (uint64_t tr = blockIdx.x; tr < batch; tr += gridDim.x) {
+        const uint32_t t = launder_lane(threadIdx.x);
+        const float2* x = in + tr * N;
+        float2 y0[E], y1[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t n = MP::row(e, t);
+            const f2v va = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(x + n));
+            const f2v vb = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(x + n + H));
+            y0[e] = make_float2(va.x, va.y);
+            y1[e] = make_float2(vb.x, vb.y);
+        }
+        if constexpr (MODE == kWindowed) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t n = MP::row(e, t);
+                y0[e] = cscale(y0[e], win[n]);
+                y1[e] = cscale(y1[e], win[n + H]);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t n = MP::row(e, t);  // < H = N/2, the range the tables hold
+            const float2 w = cmul(twn.hi[n >> twn.h], twn.lo[n & ((1u << twn.h) - 1)]);
+            const float2 a = y0[e], b = y1[e];
+            y0[e] = cadd(a, b);
+            y1[e] = twmul<INV>(csub(a, b), w);
+        }
+        EN::run_single(y0, lds, twh, t, Radices<Rs...>{});
+        WGFFT_LDS_SYNC();  // the first half's last exchange has been read
+        // opaque after the barrier: nothing of the second half is computed early
+        const uint32_t t1 = launder_lane(t);
+        EN::run_single(y1, lds, twh, t1, Radices<Rs...>{});
+        float2* o = out + tr * N;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t k = MP::row(e, t1);
+            const float2 a = cscale(y0[e], scale), b = cscale(y1[e], scale);
+            st16<true>(reinterpret_cast<float4*>(o + 2 * k), make_float4(a.x, a.y, b.x, b.y));
+        }
+        WGFFT_LDS_SYNC();  // the next transform's first pass overwrites the exchange buffer
+    }
+}
+
+template <int N, int E, int... Rs>
+struct Split2Kernel {
+    static constexpr int T = (N / 2) / E;
+    static constexpr int THREADS = T;
+    static constexpr int LDS_BYTES = LdsLayoutSingle<N / 2, E, Rs...>::len * 8 + twlds_len(ilog2(N)) * 8;
+
+    template <int MODE>
+    static hipError_t go(const BatchedArgs& a) {
+        uint64_t grid = a.grid > 0 ? (uint64_t)a.grid : a.batch;
+        if (grid > a.batch) grid = a.batch;
+        if (grid == 0) return hipSuccess;
+        hipLaunchKernelGGL((k_split2<N, E, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0, a.stream, a.in,
+                           a.out, a.tw, a.win, a.batch, a.scale);
+        return hipGetLastError();
+    }
+    static hipError_t launch(Mode m, const BatchedArgs& a) {
+        switch (m) {
+            case kForward: return go<kForward>(a);
+            case kInverse: return go<kInverse>(a);
+            default: return go<kWindowed>(a);
+        }
+    }
+    template <int MODE>
+    static int occ() {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_split2<N, E, MODE, Rs...>, THREADS,
+                                                         0) != hipSuccess)
+            return 1;
+        return blocks > 0 ? blocks : 1;
+    }
+    static int blocks_per_cu(Mode m) {
+        switch (m) {
+            case kForward: return occ<kForward>();
+            case kInverse: return occ<kInverse>();
+            default: return occ<kWindowed>();
+        }
+    }
+};
+
+// A k_split2 variant of length N; PLAN is the radix split of N/2.
+#define WGFFT_REGISTER_SPLIT2(N, E, PLAN, ...)                                                            \
+    static ::wgfft::Registrar _wgfft_split2_##N##_##E##_##PLAN(::wgfft::BatchedVariant{                 \
+        ::wgfft::ilog2(N), E, 1, ::wgfft::kFlagSplit2, ::wgfft::Split2Kernel<N, E, __VA_ARGS__>::THREADS, \
+        ::wgfft::Split2Kernel<N, E, __VA_ARGS__>::LDS_BYTES, "2x" #PLAN,                                 \
+        &::wgfft::Split2Kernel<N, E, __VA_ARGS__>::launch, &::wgfft::Split2Kernel<N, E, __VA_ARGS__>::blocks_per_cu})
 
 // Persistent multi-pass kernel (tiles read and written non-temporally: each
 // element is touched once per pass): workgroups loop over the groups of all

@@ -84,6 +84,8 @@ def test_variant_registry_counts():
     assert wgfft.GpuFft.pass_count(4096) == 0
     assert wgfft.GpuFft.pass_count(1 << 22) == 2
     assert wgfft.GpuFft.pass_count(1 << 27) == 3
+    assert wgfft.GpuFft.pass_count(1 << 24) == 3  # the measured default: three radix-256 passes
+    assert wgfft.GpuFft.pass_count(1 << 26) == 3
     for p in range(2):
         assert wgfft.GpuFft.pass_variant_count(1 << 22, p) >= 2
     assert wgfft.GpuFft.pass_variant_count(1 << 22, 2) == 0

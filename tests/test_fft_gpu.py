@@ -140,6 +140,20 @@ def test_every_first_pass_variant_windowed(log2n, batch):
             g.close()
 
 
+def test_default_plans():
+    """Without autotuning, a handle uses the plans measured fastest on MI355X."""
+    g = wgfft.GpuFft.new()
+    try:
+        assert "nt-store" in g.describe(4096)
+        assert "split2" in g.describe(16384)
+        assert g.describe(1 << 22).count("direct,halves") == 2
+        assert g.describe(1 << 26).count("),R") == 2  # three passes
+        x = oracle.synth_uniform((2, 1 << 16), 0x5EED0260)
+        check_fwd_inv(g, x)
+    finally:
+        g.close()
+
+
 @pytest.mark.parametrize("log2n", [17, 20, 23])
 def test_multipass_autotuned(log2n):
     """The global-pass combination the autotuner keeps matches FP64."""

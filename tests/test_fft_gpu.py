@@ -404,6 +404,36 @@ def test_device_api_inplace_and_stream(gpu, torch_cuda):
     assert oracle.rel_l2(z.cpu().numpy(), x) <= 5e-6
 
 
+@pytest.mark.parametrize("n,batch", [(4096, 4096 + 37), (16384, 700), (64, 300000), (1 << 22, 3)])
+def test_host_pipeline_matches_device(gpu, torch_cuda, n, batch):
+    """Host-buffer calls of 8 MB and more stream through pinned staging in
+    chunks of whole transforms (here with a partial last chunk, and at 2^22 one
+    transform per chunk): bit-identical to the device entry point, forward,
+    inverse, in place and windowed."""
+    torch = torch_cuda
+    L = wgfft._lib
+    x = wgfft.synth_uniform((batch, n), 0x5EED0700 + n)
+    t = torch.from_numpy(x).cuda()
+    d = torch.empty_like(t)
+    for inverse in (False, True):
+        gpu.exec_device(t, d, inverse=inverse)
+        torch.cuda.synchronize()
+        want = d.cpu().numpy()
+        assert np.array_equal(gpu.fft_array(x, inverse=inverse), want), inverse
+        buf = x.copy()  # in == out through the C-ABI
+        L.check(gpu._L.wgfft_exec_host(gpu._h, buf.ctypes.data, buf.ctypes.data, n, batch,
+                                           L.WGFFT_INVERSE if inverse else L.WGFFT_FORWARD))
+        assert np.array_equal(buf, want), inverse
+    rows = [0, batch // 2, batch - 1]
+    assert oracle.rel_l2(want[rows], oracle.fft64(x[rows], +1) / n) <= rel_bar(n)
+    w, _ = oracle.synth_window_mask(n, 0x5EED0701 + n)
+    gpu.exec_windowed_device(t, torch.from_numpy(w).cuda(), d)
+    torch.cuda.synchronize()
+    yw = np.empty_like(x)
+    L.check(gpu._L.wgfft_exec_windowed_host(gpu._h, x.ctypes.data, w.ctypes.data, yw.ctypes.data, n, batch))
+    assert np.array_equal(yw, d.cpu().numpy())
+
+
 def test_windowed_device_c5_shape(gpu, torch_cuda, parity):
     torch = torch_cuda
     n, batch = 8192, 1024

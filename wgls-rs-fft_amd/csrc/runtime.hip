@@ -491,6 +491,8 @@ int wgfft_create(wgfft_t** out, int device) {
         return hip_fail(e, "hipStreamCreateWithFlags");
     }
     if (const char* env = std::getenv("WGFFT_MP_CHUNK_BYTES")) h->mp_chunk_bytes = std::strtoull(env, nullptr, 10);
+    if (const char* env = std::getenv("WGFFT_HOST_CHUNK_BYTES"))
+        h->host_chunk_bytes = std::max<uint64_t>(1ull << 20, std::strtoull(env, nullptr, 10));
     load_tune_file(h);
     *out = h;
     return WGFFT_OK;
@@ -504,6 +506,7 @@ void wgfft_destroy(wgfft_t* h) {
         hipStreamSynchronize(h->stream);
         for (auto& t : h->tables) hipFree(t.p);
         h->tables.clear();
+        h->free_pipe();
         for (DevBuf* b : {&h->io_in, &h->io_out, &h->io_w, &h->io_aux, &h->scratch, &h->scratch2, &h->red, &h->red2, &h->lossbuf})
             h->release(*b);
         hipStreamDestroy(h->stream);
@@ -539,12 +542,17 @@ int wgfft_exec_host(wgfft_t* h, const float* in, float* out, uint64_t n, uint64_
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     const uint64_t bytes = n * batch * sizeof(float2);
-    if ((rc = h->ensure(h->io_in, bytes)) || (rc = h->ensure(h->io_out, bytes))) return rc;
     Plan* p;
     if ((rc = h->get_plan(n, &p))) return rc;
+    bool inv = dir == WGFFT_INVERSE;
+    if (h->host_pipelined(n, batch))
+        return h->run_host(in, out, n, batch, [&](const float2* d_in, float2* d_out, uint64_t b) {
+            return h->run(p, inv ? kInverse : kForward, d_in, d_out, nullptr, b, inv ? 1.0f / (float)n : 1.0f,
+                          h->stream);
+        });
+    if ((rc = h->ensure(h->io_in, bytes)) || (rc = h->ensure(h->io_out, bytes))) return rc;
     if ((e = hipMemcpyAsync(h->io_in.p, in, bytes, hipMemcpyHostToDevice, h->stream)) != hipSuccess)
         return hip_fail(e, "hipMemcpyAsync(H2D)");
-    bool inv = dir == WGFFT_INVERSE;
     if ((rc = h->run(p, inv ? kInverse : kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch,
                      inv ? 1.0f / (float)n : 1.0f, h->stream)))
         return rc;
@@ -581,13 +589,17 @@ int wgfft_exec_windowed_host(wgfft_t* h, const float* x, const float* w, float* 
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     const uint64_t bytes = n * batch * sizeof(float2);
-    if ((rc = h->ensure(h->io_in, bytes)) || (rc = h->ensure(h->io_out, bytes)) ||
-        (rc = h->ensure(h->io_w, n * sizeof(float) + 16)))
-        return rc;
+    if ((rc = h->ensure(h->io_w, n * sizeof(float) + 16))) return rc;
     Plan* p;
     if ((rc = h->get_plan(n, &p))) return rc;
-    if ((e = hipMemcpyAsync(h->io_in.p, x, bytes, hipMemcpyHostToDevice, h->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(h->io_w.p, w, n * sizeof(float), hipMemcpyHostToDevice, h->stream)) != hipSuccess)
+    if ((e = hipMemcpyAsync(h->io_w.p, w, n * sizeof(float), hipMemcpyHostToDevice, h->stream)) != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync(H2D)");
+    if (h->host_pipelined(n, batch))
+        return h->run_host(x, y, n, batch, [&](const float2* d_in, float2* d_out, uint64_t b) {
+            return h->run(p, kWindowed, d_in, d_out, (const float*)h->io_w.p, b, 1.0f, h->stream);
+        });
+    if ((rc = h->ensure(h->io_in, bytes)) || (rc = h->ensure(h->io_out, bytes))) return rc;
+    if ((e = hipMemcpyAsync(h->io_in.p, x, bytes, hipMemcpyHostToDevice, h->stream)) != hipSuccess)
         return hip_fail(e, "hipMemcpyAsync(H2D)");
     if ((rc = h->run(p, kWindowed, (const float2*)h->io_in.p, (float2*)h->io_out.p, (const float*)h->io_w.p, batch,
                      1.0f, h->stream)))

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -44,6 +45,7 @@ struct Plan {
 };
 
 struct Handle;
+struct HostPipe;
 struct AdjointVariant;
 void load_tune_file(Handle* h);
 const AdjointVariant* find_adjoint(int log2n);
@@ -72,6 +74,12 @@ struct Handle {
     DevBuf io_in, io_out, io_w, io_aux, scratch, scratch2, red, red2, lossbuf;
     uint64_t held = 0;
     uint64_t mp_chunk_bytes = 2ull << 30;  // batch chunk of the multipass path (bounds its scratch)
+    // Host-buffer calls of at least kHostPipeMinBytes move through pinned
+    // staging in chunks of whole transforms of about host_chunk_bytes
+    // (transforms larger than that are copied directly).
+    static constexpr uint64_t kHostPipeMinBytes = 8ull << 20;
+    uint64_t host_chunk_bytes = 64ull << 20;
+    HostPipe* pipe = nullptr;
 
     int ensure(DevBuf& b, uint64_t bytes);
     void release(DevBuf& b);
@@ -81,6 +89,12 @@ struct Handle {
     int blocks_per_cu(const BatchedVariant* v, Mode m);
     int run(Plan* p, Mode mode, const float2* in, float2* out, const float* win, uint64_t batch, float scale,
             hipStream_t s, const BatchedVariant* force = nullptr);
+    bool host_pipelined(uint64_t n, uint64_t batch) const;
+    // Host in -> host out through the staging pipeline; run_chunk(d_in, d_out,
+    // transforms) enqueues one chunk's transform on `stream`.
+    int run_host(const float* in, float* out, uint64_t n, uint64_t batch,
+                 const std::function<int(const float2*, float2*, uint64_t)>& run_chunk);
+    void free_pipe();
 };
 
 }  // namespace wgfft

@@ -135,10 +135,14 @@ static int pipe_ready(Handle* h, uint64_t bytes) {
         }
         // 8 threads in all copy at ~89 GB/s on the MI355X host (1 thread: 27 GB/s)
         const unsigned hw = std::thread::hardware_concurrency();
-        hp->pool = new (std::nothrow) CopyPool((int)std::min(7u, hw > 1 ? hw - 1 : 0u));
+        try {  // nothing may throw across the C-ABI
+            hp->pool = new CopyPool((int)std::min(7u, hw > 1 ? hw - 1 : 0u));
+        } catch (...) {
+            hp->pool = nullptr;
+        }
         if (!hp->pool) {
             h->free_pipe();
-            return fail(WGFFT_ERR_OUT_OF_MEMORY, "copy pool allocation failed");
+            return fail(WGFFT_ERR_OUT_OF_MEMORY, "copy pool threads could not be started");
         }
     }
     HostPipe* hp = h->pipe;

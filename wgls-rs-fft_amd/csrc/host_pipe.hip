@@ -30,16 +30,14 @@ namespace wgfft {
 class CopyPool {
 public:
     explicit CopyPool(int helpers) {
-        for (int i = 0; i < helpers; ++i) threads_.emplace_back([this, i] { work(i + 1); });
-    }
-    ~CopyPool() {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
+        try {
+            for (int i = 0; i < helpers; ++i) threads_.emplace_back([this, i] { work(i + 1); });
+        } catch (...) {
+            stop_all();  // the threads already started are joined before the exception leaves
+            throw;
         }
-        cv_.notify_all();
-        for (auto& t : threads_) t.join();
     }
+    ~CopyPool() { stop_all(); }
     void copy(void* dst, const void* src, uint64_t bytes) {
         const int parts = (int)threads_.size() + 1;
         {
@@ -57,6 +55,15 @@ public:
     }
 
 private:
+    void stop_all() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : threads_) t.join();
+        threads_.clear();
+    }
     void share(int k, int parts) {
         // 4 KB-aligned shares
         const uint64_t per = ((bytes_ / parts) + 4095) & ~4095ull;

@@ -1,9 +1,10 @@
 // variants_global.hip -- sub-FFT variants for the passes of N > 16384:
 // radix R as an LDS sub-FFT over G butterflies per workgroup with E
 // elements per thread, with (PF = 1) or without the next group's tile
-// prefetched into registers.  For each R the first registration is the
-// default; wgfft_autotune times the others.  E = 16 halves the registers of
-// the sub-FFT (more workgroups per CU), E = 32 halves its LDS passes.
+// prefetched into registers.  The runtime's default per length is the
+// measured split and pass variants (first registration per R beyond 2^26);
+// wgfft_autotune times the others.  E = 16 halves the registers of the
+// sub-FFT (more workgroups per CU), E = 32 halves its LDS passes.
 #include "kernels.hpp"
 
 WGFFT_REGISTER_GLOBAL_PASS(128, 16, 16, 0, 4x4x8, 4, 4, 8);

@@ -1,5 +1,6 @@
 // variants_small_n.hip -- LDS-resident variants for N = 32 .. 512.
-// For each N the first registration is the default split.
+// The runtime's default per N is the measured variant; the others are
+// autotuner candidates.
 #include "kernels.hpp"
 
 WGFFT_REGISTER_BATCHED(32, 32, 256, 16x2, 16, 2);

@@ -46,8 +46,8 @@ def test_ragged_small_batches_all_modes(gpu, torch_cuda, count, seed):
 
 @pytest.mark.parametrize("kmax", [13, 14, 15])
 def test_ragged_scratch_sized_by_largest_length(gpu, torch_cuda, kmax):
-    """Batches whose largest length is 2^13 (no two-pass unit, no scratch),
-    2^14 or 2^15 (scratch slots of that size) in every launch mode."""
+    """Batches whose largest length is 2^13 or 2^14 (no two-pass unit, no
+    scratch) or 2^15 (scratch slots of that size) in every launch mode."""
     torch = torch_cuda
     k = oracle.synth_ragged_log2(700, 6, kmax, 0x5EED0503 + kmax)
     offsets, total = wgfft.ragged_layout(k)

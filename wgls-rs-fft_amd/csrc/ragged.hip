@@ -5,9 +5,12 @@
 // The host sorts the transforms by length (largest first) and cuts them into
 // work units: for N <= 4096 a unit is G transforms of one length that a
 // 256-thread workgroup transforms together in LDS (T*G = 256, 16 elements
-// per thread); for N = 2^13 .. 2^16 a unit is one transform that the
-// workgroup does as a two-pass Stockham (radix R1 then R2 LDS sub-FFTs)
-// through its own slot of global scratch.  Units never depend on each other, so the only shared
+// per thread); N = 2^13 is one transform in LDS (32 elements per thread);
+// N = 2^14 is one transform held in registers as a radix-2 stage and two
+// 8192-point halves through the LDS; for N = 2^15, 2^16 a unit is one
+// transform that the workgroup does as a two-pass Stockham (radix R1 then
+// R2 LDS sub-FFTs) through its own slot of global scratch.  Units never
+// depend on each other, so the only shared
 // state is the queue head.  A persistent grid (occupancy x CUs) dequeues
 // units with one atomic ticket per unit; the workgroup holding the launch's
 // last ticket resets the counter, so the launch can be captured once into a
@@ -38,7 +41,7 @@ constexpr int kRaggedThreads = 256;
 // Inputs and outputs are streamed non-temporally (read and written once), so
 // they displace less of the cached big-unit scratch.
 constexpr bool NTIO = true;
-constexpr int kMinLog2 = 6, kMaxLog2 = 16, kDirectMaxLog2 = 13;
+constexpr int kMinLog2 = 6, kMaxLog2 = 16, kDirectMaxLog2 = 13, kSplitLog2 = 14;
 
 // elements per thread of the direct (LDS-resident) unit for length 2^k
 constexpr int direct_e(int log2n) { return log2n == 13 ? 32 : 16; }
@@ -57,7 +60,8 @@ constexpr int kRaggedLds = cmax(
          cmax(direct_lds<256, 16, 8, 4, 8>(), direct_lds<512, 16, 8, 8, 8>())),
     cmax(cmax(direct_lds<1024, 16, 8, 16, 8>(), direct_lds<2048, 16, 8, 4, 8, 8>()),
          cmax(cmax(direct_lds<4096, 16, 8, 8, 8, 8>(), direct_lds<8192, 32, 16, 16, 16, 2>()),
-              cmax(pass_lds<128, 16, 4, 8, 4>(), pass_lds<256, 16, 16, 16>()))));
+              cmax(cmax(pass_lds<128, 16, 4, 8, 4>(), pass_lds<256, 16, 16, 16>()),
+                   LdsLayoutSingle<8192, 32, 8, 16, 8, 8>::len))));
 
 // Unit handlers take the thread index through launder_lane() once per unit,
 // so no handler's index arithmetic is hoisted out of the persistent loop and
@@ -76,6 +80,53 @@ __device__ __forceinline__ void ragged_direct(const float2* __restrict__ in, flo
     if (active) Io<N, E, Rs...>::template load<NTIO>(d, in + off, t);
     Engine<N, E, T, INV>::run_with(d, lds_raw + g * LEN, tl.shifted(kMaxLog2 - ilog2(N)), t, Radices<Rs...>{});
     if (active) Io<N, E, Rs...>::template store<INV, NTIO>(out + off, d, t, 1.0f / (float)N);
+}
+
+// One length-N signal (N = 2^14) in one workgroup without scratch: a
+// decimation-in-frequency radix-2 stage in registers, y0[n] = x[n] + x[n + H]
+// and y1[n] = (x[n] - x[n + H]) W_N^n (H = N/2), then the two H-point halves
+// in turn through the LDS, and X[2k] = Y0[k], X[2k + 1] = Y1[k] stored as
+// 16-byte pairs.  Thread t holds single-layout slot i RF + m, index
+// t + T i + m H/RF, of both halves (RF the first radix).
+template <int N, int E, bool INV, int... Rs>
+__device__ __forceinline__ void ragged_split2(const float2* __restrict__ src, float2* __restrict__ dst,
+                                              const TwLds& tl, float2* __restrict__ lds, uint32_t tid) {
+    constexpr int H = N / 2;
+    constexpr int T = H / E;
+    static_assert(T == kRaggedThreads, "the halves must fill the workgroup");
+    using MP = GdMap<H, E, 1, Rs...>;
+    using EN = Engine<H, E, T, INV>;
+    const TwLds tn = tl.shifted(kMaxLog2 - ilog2(N));  // W_N
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    float2 y0[E], y1[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t n = MP::row(e, tid);
+        const f2v a = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(src + n));
+        const f2v b = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(src + n + H));
+        y0[e] = make_float2(a.x, a.y);
+        y1[e] = make_float2(b.x, b.y);
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t j = MP::row(e, tid) << tn.sh;  // W_N^n as an index of the 2^16 tables (< 2^15)
+        const float2 w = cmul(tn.hi[j >> tn.h], tn.lo[j & ((1u << tn.h) - 1)]);
+        const float2 a = y0[e], b = y1[e];
+        y0[e] = cadd(a, b);
+        y1[e] = twmul<INV>(csub(a, b), w);
+    }
+    const TwLds th = tl.shifted(kMaxLog2 - ilog2(H));  // W_H
+    EN::run_single(y0, lds, th, tid, Radices<Rs...>{});
+    WGFFT_LDS_SYNC();  // the first half's last exchange has been read
+    const uint32_t t1 = launder_lane(tid);
+    EN::run_single(y1, lds, th, t1, Radices<Rs...>{});
+    const float sc = INV ? 1.0f / (float)N : 1.0f;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t k = MP::row(e, t1);
+        const float2 a = cscale(y0[e], sc), b = cscale(y1[e], sc);
+        st16<NTIO>(reinterpret_cast<float4*>(dst + 2 * k), make_float4(a.x, a.y, b.x, b.y));
+    }
 }
 
 // Two-pass transform of one length-N signal (N = R1 * R2) by one workgroup:
@@ -193,8 +244,7 @@ k_ragged(const float2* __restrict__ in, float2* __restrict__ out, const RaggedUn
             case 13: ragged_direct<8192, 32, INV, 16, 16, 16, 2>(in, out, offs, un.first, un.count, tl, lds, tid); break;
             case 14: {
                 const uint64_t o = offs[un.first];
-                RaggedBig<14, 128, 16, Radices<4, 8, 4>, 128, 16, Radices<4, 8, 4>, INV, PF>::run(in + o, out + o,
-                                                                                              my_scratch, tw_all, tl, lds, tid);
+                ragged_split2<16384, 32, INV, 8, 16, 8, 8>(in + o, out + o, tl, lds, tid);
                 break;
             }
             case 15: {
@@ -332,7 +382,7 @@ extern "C" int wgfft_ragged_create(wgfft_t* h, const uint64_t* offsets, const ui
     }
     tw[0] = make_float2(1.f, 0.f);
     // per-workgroup scratch only for the two-pass lengths, sized for the largest present
-    for (const auto& u : units) p->slot_log2 = std::max(p->slot_log2, u.log2n > (uint32_t)kDirectMaxLog2 ? u.log2n : 0u);
+    for (const auto& u : units) p->slot_log2 = std::max(p->slot_log2, u.log2n > (uint32_t)kSplitLog2 ? u.log2n : 0u);
     const uint64_t scratch_bytes = p->slot_log2 ? ((uint64_t)p->grid_max << p->slot_log2) * sizeof(float2) : 0;
     struct {
         void** dst;

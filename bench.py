@@ -15,6 +15,9 @@ The same JSON line also carries, measured in the same run:
   * mixed_batch: 20,000 transforms of lengths 2^k, k ~ U{6..16}, as one
     persistent launch, its hipGraph replay, and one launch per length;
   * large_n: N = 2^22 x 16 and N = 2^26 x 1 forward transforms;
+  * c5_windowed: N = 8192 x 1024 with a real window: the windowed forward,
+    the VJP (gx, gw from gY) and the fused masked forward + adjoint, each
+    against the plain forward of the same shape;
   * c1_and_host_api: N=1024 x 8 device latency, and the host-buffer API
     (H2D + transform + D2H, PCIe-inclusive) at 1024 x 8 and 4096 x 4096;
   * cpu_baseline: the reference algorithm (the oracle's C restatement of the
@@ -159,6 +162,34 @@ def bench_c1_and_host(torch, gpu, wgfft) -> dict:
     return out
 
 
+def bench_c5(torch, gpu, wgfft, steps: int, tune: bool) -> dict:
+    """N = 8192 x 1024 with a window (Hann): windowed forward, VJP and fused
+    masked forward + adjoint; algorithmic bytes 16 B/sample for the forward
+    transforms, ratios against the plain forward of the same shape."""
+    import numpy as np
+
+    n, batch = 8192, 1024
+    x = (torch.rand(2 * n * batch, device="cuda") * 2 - 1).view(torch.complex64).view(batch, n)
+    y = torch.empty_like(x)
+    gy = (torch.rand(2 * n * batch, device="cuda") * 2 - 1).view(torch.complex64).view(batch, n)
+    gx = torch.empty_like(x)
+    w = torch.from_numpy(np.hanning(n).astype(np.float32)).cuda()
+    m = (torch.rand(n, device="cuda") < 0.3).float()
+    gw = torch.empty(n, dtype=torch.float32, device="cuda")
+    loss = torch.empty(1, dtype=torch.float32, device="cuda")
+    if tune:
+        gpu.autotune(n, batch)
+    k = max(10, steps // 4)
+    fwd = timed(torch, lambda: gpu.exec_device(x, y), k, 5)
+    win = timed(torch, lambda: gpu.exec_windowed_device(x, w, y), k, 5)
+    vjp = timed(torch, lambda: gpu.vjp_windowed_device(x, w, gy, gx, gw), k, 5)
+    fused = timed(torch, lambda: gpu.fwd_adjoint_masked_device(x, w, m, None, gx, gw, loss), k, 5)
+    gbs = 16.0 * n * batch / (win * 1e-3) / 1e9
+    return {"fwd_ms": round(fwd, 4), "windowed_fwd_ms": round(win, 4), "windowed_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "vjp_ms": round(vjp, 4), "vjp_ratio": round(vjp / fwd, 3),
+            "fused_fwd_adjoint_ms": round(fused, 4), "fused_ratio": round(fused / fwd, 3), "plan": gpu.describe(n)}
+
+
 def bench_large(torch, gpu, steps: int, tune: bool) -> dict:
     out = {}
     for log2n, batch in ((22, 16), (26, 1)):
@@ -186,7 +217,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--headline-only", action="store_true", help="only the N=4096x4096 forward (profiling)")
-    ap.add_argument("--only", choices=("shapes", "mixed", "large", "c1"),
+    ap.add_argument("--only", choices=("shapes", "mixed", "large", "c1", "c5"),
                     help="profiling: skip the headline and run one secondary section")
     ap.add_argument("--no-tune", action="store_true", help="use the default kernel variants")
     args = ap.parse_args()
@@ -204,7 +235,8 @@ def main() -> int:
         section = {"shapes": lambda: bench_shapes(torch, gpu, args.steps, not args.no_tune),
                    "mixed": lambda: bench_mixed(torch, gpu, wgfft, args.steps),
                    "large": lambda: bench_large(torch, gpu, args.steps, not args.no_tune),
-                   "c1": lambda: bench_c1_and_host(torch, gpu, wgfft)}[args.only]
+                   "c1": lambda: bench_c1_and_host(torch, gpu, wgfft),
+                   "c5": lambda: bench_c5(torch, gpu, wgfft, args.steps, not args.no_tune)}[args.only]
         print(json.dumps({args.only: section()}))
         gpu.close()
         return 0
@@ -263,6 +295,7 @@ def main() -> int:
         out["other_shapes"] = bench_shapes(torch, gpu, args.steps, not args.no_tune)
         out["mixed_batch"] = bench_mixed(torch, gpu, wgfft, args.steps)
         out["large_n"] = bench_large(torch, gpu, args.steps, not args.no_tune)
+        out["c5_windowed"] = bench_c5(torch, gpu, wgfft, args.steps, not args.no_tune)
         out["c1_and_host_api"] = bench_c1_and_host(torch, gpu, wgfft)
         del x_keep, y_keep
     if not args.no_cpu:

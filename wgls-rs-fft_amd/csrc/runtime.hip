@@ -124,7 +124,8 @@ int Handle::upload_table(const std::vector<float2>& host, float2** dev) {
 
 // Default kernel variant of each single-kernel length: the autotuner's pick
 // on MI355X at about 2^24 samples per call (7% to 58% faster than the first
-// registration, which remains the fallback).  An entry made by
+// registration, which remains the fallback); at 2^13 the split kernel, which
+// is 3% behind at 8192 x 2048 but 24% ahead at 8192 x 1024.  An entry made by
 // wgfft_autotune or read from the tune file overrides it.
 namespace {
 struct DefaultBatched {
@@ -136,7 +137,7 @@ const DefaultBatched kDefaultBatched[] = {
     {5, "8x4", 16, 128, 0},       {6, "8x8", 16, 64, 0},          {7, "4x4x8", 16, 32, 0},
     {8, "8x4x8", 16, 16, 0},      {9, "8x8x8", 16, 8, 0},         {10, "16x4x16", 16, 4, 0},
     {11, "16x8x16", 16, 2, 0},    {12, "16x16x16", 16, 1, kFlagNtStore},
-    {13, "16x16x16x2", 32, 1, kFlagNoPrefetch},                    {14, "2x16x8x4x16", 16, 1, kFlagSplit2},
+    {13, "2x16x16x16", 16, 1, kFlagSplit2},                        {14, "2x16x8x4x16", 16, 1, kFlagSplit2},
 };
 
 // Default multipass plans of 2^15 .. 2^26, picked the same way (1.2x to 5.8x

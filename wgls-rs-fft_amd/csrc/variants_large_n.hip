@@ -17,10 +17,14 @@ WGFFT_REGISTER_BATCHED_F(16384, 16, 1, 9, 16x4x16x16, 16, 4, 16, 16);
 WGFFT_REGISTER_BATCHED_F(16384, 32, 1, 8, 16x16x16x4, 16, 16, 16, 4);
 WGFFT_REGISTER_BATCHED_F(16384, 32, 1, 9, 16x16x16x4, 16, 16, 16, 4);
 
-// N = 16384 as a radix-2 stage and two 8192-point halves through one 74 KB
-// LDS footprint, so two workgroups share a CU; PLAN is the split of 8192.
-// (The same split of 8192 and 4096 was measured no faster than their
-// single-footprint kernels, which already fit several workgroups per CU.)
+// N = 16384 (8192) as a radix-2 stage and two 8192-point (4096-point)
+// halves through one LDS footprint of the half length: 74 KB (36-38 KB), so
+// two (four) workgroups share a CU; PLAN is the split of the half length.
+// At 8192 x 1024 all transforms are then in flight at once (21.4 us against
+// 26.5 us); at 8192 x 2048 the single-footprint kernel is 3% ahead.  (The
+// same split of 4096 is slower.)
 WGFFT_REGISTER_SPLIT2(16384, 16, 8x16x8x8, 8, 16, 8, 8);
 WGFFT_REGISTER_SPLIT2(16384, 16, 8x8x16x8, 8, 8, 16, 8);
 WGFFT_REGISTER_SPLIT2(16384, 16, 16x8x4x16, 16, 8, 4, 16);
+WGFFT_REGISTER_SPLIT2(8192, 16, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_SPLIT2(8192, 16, 8x8x8x8, 8, 8, 8, 8);

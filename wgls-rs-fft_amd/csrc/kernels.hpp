@@ -42,6 +42,7 @@ constexpr int kFlagTwRegs = 2;   // twiddle base powers held in registers across
 constexpr int kFlagTwLds = 4;    // twiddle base powers from two short LDS tables
 constexpr int kFlagNoPrefetch = 8;  // no register prefetch of the next transform (fewer registers)
 constexpr int kFlagSplit2 = 256;    // k_split2: a radix-2 stage, then the two halves in turn through LDS
+constexpr int kFlagPrio = 512;      // raised wave priority while issuing the prefetch loads and the stores
 // bits 4..7: minimum waves per SIMD the register allocation must allow (0: compiler's choice)
 constexpr int flag_min_waves(int fl) { return (fl >> 4) & 15; }
 
@@ -272,7 +273,9 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
 #pragma unroll
             for (int i = 0; i < E; ++i) d[i] = nxt[i];
             const uint64_t ngrp = grp + gridDim.x;
+            if constexpr ((FL & kFlagPrio) != 0) __builtin_amdgcn_s_setprio(2);
             if (ngrp < ngroups && ngrp * G + g < batch) IO::load(nxt, in + (ngrp * G + g) * N, t);
+            if constexpr ((FL & kFlagPrio) != 0) __builtin_amdgcn_s_setprio(0);
         } else {
             if (active) IO::load(d, in + tr * N, t);
         }
@@ -286,7 +289,9 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
             EN::run_with(d, lds, twl, t, Radices<Rs...>{});
         else
             EN::run(d, lds, tw, t, Radices<Rs...>{});
+        if constexpr ((FL & kFlagPrio) != 0) __builtin_amdgcn_s_setprio(2);
         if (active) IO::template store<INV, NT>(out + tr * N, d, t, scale);
+        if constexpr ((FL & kFlagPrio) != 0) __builtin_amdgcn_s_setprio(0);
         WGFFT_LDS_SYNC();  // the next group's first pass overwrites the exchange buffer
     }
 }

@@ -149,8 +149,8 @@ DEFAULT_PLANS = {
     7: 'plan=4x4x8 E=16 G=32',
     8: 'plan=8x4x8 E=16 G=16',
     9: 'plan=8x8x8 E=16 G=8',
-    10: 'plan=16x4x16 E=16 G=4',
-    11: 'plan=16x8x16 E=16 G=2',
+    10: 'plan=16x4x16 E=16 G=4 prio',
+    11: 'plan=16x8x16 E=16 G=2 prio',
     12: 'plan=16x16x16 E=16 G=1 nt-store prio',
     13: 'plan=2x16x16x16 E=16 G=1 split2',
     14: 'plan=2x16x8x4x16 E=16 G=1 split2',

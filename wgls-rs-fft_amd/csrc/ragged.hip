@@ -43,6 +43,11 @@ constexpr int kRaggedThreads = 256;
 constexpr bool NTIO = true;
 constexpr int kMinLog2 = 6, kMaxLog2 = 16, kDirectMaxLog2 = 13, kSplitLog2 = 14;
 
+// The wave priority is raised while a wave issues global loads and stores, so
+// the waves of a CU keep the memory pipeline fed ahead of their LDS and ALU work.
+__device__ __forceinline__ void prio_up() { __builtin_amdgcn_s_setprio(2); }
+__device__ __forceinline__ void prio_down() { __builtin_amdgcn_s_setprio(0); }
+
 // elements per thread of the direct (LDS-resident) unit for length 2^k
 constexpr int direct_e(int log2n) { return log2n == 13 ? 32 : 16; }
 // transforms per direct unit for length 2^k
@@ -77,9 +82,13 @@ __device__ __forceinline__ void ragged_direct(const float2* __restrict__ in, flo
     const bool active = g < count;
     const uint64_t off = active ? offs[first + g] : 0;
     float2 d[E];
+    prio_up();
     if (active) Io<N, E, Rs...>::template load<NTIO>(d, in + off, t);
+    prio_down();
     Engine<N, E, T, INV>::run_with(d, lds_raw + g * LEN, tl.shifted(kMaxLog2 - ilog2(N)), t, Radices<Rs...>{});
+    prio_up();
     if (active) Io<N, E, Rs...>::template store<INV, NTIO>(out + off, d, t, 1.0f / (float)N);
+    prio_down();
 }
 
 // One length-N signal (N = 2^14) in one workgroup without scratch: a
@@ -99,6 +108,7 @@ __device__ __forceinline__ void ragged_split2(const float2* __restrict__ src, fl
     const TwLds tn = tl.shifted(kMaxLog2 - ilog2(N));  // W_N
     typedef float f2v __attribute__((ext_vector_type(2)));
     float2 y0[E], y1[E];
+    prio_up();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const uint32_t n = MP::row(e, tid);
@@ -107,6 +117,7 @@ __device__ __forceinline__ void ragged_split2(const float2* __restrict__ src, fl
         y0[e] = make_float2(a.x, a.y);
         y1[e] = make_float2(b.x, b.y);
     }
+    prio_down();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const uint32_t j = MP::row(e, tid) << tn.sh;  // W_N^n as an index of the 2^16 tables (< 2^15)
@@ -121,12 +132,14 @@ __device__ __forceinline__ void ragged_split2(const float2* __restrict__ src, fl
     const uint32_t t1 = launder_lane(tid);
     EN::run_single(y1, lds, th, t1, Radices<Rs...>{});
     const float sc = INV ? 1.0f / (float)N : 1.0f;
+    prio_up();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const uint32_t k = MP::row(e, t1);
         const float2 a = cscale(y0[e], sc), b = cscale(y1[e], sc);
         st16<NTIO>(reinterpret_cast<float4*>(dst + 2 * k), make_float4(a.x, a.y, b.x, b.y));
     }
+    prio_down();
 }
 
 // Two-pass transform of one length-N signal (N = R1 * R2) by one workgroup:
@@ -154,18 +167,24 @@ struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV, PF
         // pass 1 (L = 1): src -> scratch (no twiddles, so no side loads)
         float2 cur[E1];
         Side sd;
+        prio_up();
         if (PF) gd_load<R1, E1, G1, NTIO, Rs1...>(cur, src, N / R1, 0, tid);
+        prio_down();
 #pragma unroll 1
         for (uint64_t b0 = 0; b0 < N / R1; b0 += G1) {
             if constexpr (PF) {
                 float2 nxt[E1];
+                prio_up();
                 if (b0 + G1 < N / R1) gd_load<R1, E1, G1, NTIO, Rs1...>(nxt, src, N / R1, b0 + G1, tid);
+                prio_down();
                 gd_compute<R1, E1, G1, M, TwOneLevel, TwLds, false, Rs1...>(cur, sd, scratch, t1, N, 1, b0, 1.0f, lds,
                                                                             tid);
 #pragma unroll
                 for (int i = 0; i < E1; ++i) cur[i] = nxt[i];
             } else {
+                prio_up();
                 gd_load<R1, E1, G1, NTIO, Rs1...>(cur, src, N / R1, b0, tid);
+                prio_down();
                 gd_compute<R1, E1, G1, M, TwOneLevel, TwLds, false, Rs1...>(cur, sd, scratch, t1, N, 1, b0, 1.0f, lds,
                                                                             tid);
             }
@@ -176,7 +195,9 @@ struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV, PF
         const float sc = INV ? 1.0f / (float)N : 1.0f;
         if (PF) {
             gd_load_side<R2, E2, G2, M, TwOneLevel, Rs2...>(sd, twN, nullptr, N, R1, 0, tid);
+            prio_up();
             gd_load<R2, E2, G2, false, Rs2...>(cur, scratch, N / R2, 0, tid);
+            prio_down();
         }
 #pragma unroll 1
         for (uint64_t b0 = 0; b0 < N / R2; b0 += G2) {
@@ -185,7 +206,9 @@ struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV, PF
                 Side nsd;
                 if (b0 + G2 < N / R2) {
                     gd_load_side<R2, E2, G2, M, TwOneLevel, Rs2...>(nsd, twN, nullptr, N, R1, b0 + G2, tid);
+                    prio_up();
                     gd_load<R2, E2, G2, false, Rs2...>(nxt, scratch, N / R2, b0 + G2, tid);
+                    prio_down();
                 }
                 gd_compute<R2, E2, G2, M, TwOneLevel, TwLds, NTIO, Rs2...>(cur, sd, dst, t2, N, R1, b0, sc, lds, tid);
 #pragma unroll
@@ -193,7 +216,9 @@ struct RaggedBig<LOGN, R1, E1, Radices<Rs1...>, R2, E2, Radices<Rs2...>, INV, PF
                 sd = nsd;
             } else {
                 gd_load_side<R2, E2, G2, M, TwOneLevel, Rs2...>(sd, twN, nullptr, N, R1, b0, tid);
+                prio_up();
                 gd_load<R2, E2, G2, false, Rs2...>(cur, scratch, N / R2, b0, tid);
+                prio_down();
                 gd_compute<R2, E2, G2, M, TwOneLevel, TwLds, NTIO, Rs2...>(cur, sd, dst, t2, N, R1, b0, sc, lds, tid);
             }
             WGFFT_LDS_SYNC();

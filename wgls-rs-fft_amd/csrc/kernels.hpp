@@ -1005,6 +1005,8 @@ struct GlobalPassWideKernel {
 // two workgroups share a CU and one's loads overlap the other's transform.
 // Thread t holds the single-layout slots of its H-point sub-FFTs: slot
 // i RF + m is index t + T i + m H/RF of both halves (RF the first radix).
+// The wave priority is raised while it issues the global loads and stores
+// (at 8192 x 1024, four workgroups per CU: 23-24 against 24-25 us).
 template <int N, int E, int MODE, int... Rs>
 __global__ void __launch_bounds__((N / 2) / E) __attribute__((amdgpu_waves_per_eu(4)))
 k_split2(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
@@ -1024,6 +1026,7 @@ k_split2(const float2* __restrict__ in, float2* __restrict__ out, const float2* 
         const uint32_t t = launder_lane(threadIdx.x);
         const float2* x = in + tr * N;
         float2 y0[E], y1[E];
+        __builtin_amdgcn_s_setprio(2);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const uint32_t n = MP::row(e, t);
@@ -1032,6 +1035,7 @@ k_split2(const float2* __restrict__ in, float2* __restrict__ out, const float2* 
             y0[e] = make_float2(va.x, va.y);
             y1[e] = make_float2(vb.x, vb.y);
         }
+        __builtin_amdgcn_s_setprio(0);
         if constexpr (MODE == kWindowed) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
@@ -1054,12 +1058,14 @@ k_split2(const float2* __restrict__ in, float2* __restrict__ out, const float2* 
         const uint32_t t1 = launder_lane(t);
         EN::run_single(y1, lds, twh, t1, Radices<Rs...>{});
         float2* o = out + tr * N;
+        __builtin_amdgcn_s_setprio(2);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const uint32_t k = MP::row(e, t1);
             const float2 a = cscale(y0[e], scale), b = cscale(y1[e], scale);
             st16<true>(reinterpret_cast<float4*>(o + 2 * k), make_float4(a.x, a.y, b.x, b.y));
         }
+        __builtin_amdgcn_s_setprio(0);
         WGFFT_LDS_SYNC();  // the next transform's first pass overwrites the exchange buffer
     }
 }

@@ -83,9 +83,11 @@ k_adjoint(AdjointArgs a) {
         float2 d[E];
         [[maybe_unused]] float2 xk[KEEPX && FWD ? E : 1];
         float2 in0[E];
+        __builtin_amdgcn_s_setprio(2);  // raised while the wave issues its global loads
         IO::load(in0, src + tr * N, t);
         [[maybe_unused]] float wv0[FWD ? E : 1];
         if constexpr (FWD) IO::load_real(wv0, a.w, t);
+        __builtin_amdgcn_s_setprio(0);
         if constexpr (FWD) {
 #pragma unroll
             for (int i = 0; i < E; ++i) {
@@ -142,7 +144,9 @@ k_adjoint(AdjointArgs a) {
             acc[i] += d[i].x * xs[i].x + d[i].y * xs[i].y;
             d[i] = cscale(d[i], wv[i]);
         }
+        __builtin_amdgcn_s_setprio(2);  // and its stores
         IO::template store<false>(a.gx + tr * N, d, ta, 1.f);
+        __builtin_amdgcn_s_setprio(0);
         WGFFT_LDS_SYNC();
     }
     const uint32_t t = threadIdx.x;

@@ -701,7 +701,9 @@ k_global_pass_direct(const float2* __restrict__ in, float2* __restrict__ out, co
             if (nl < total) {
                 const uint64_t ntr = nl / groups_per_tr, nb0 = (nl % groups_per_tr) * G;
                 gd_load_side<R, E, G, MODE, TwTwoLevel, Rs...>(nsd, twN, win, N, L, nb0, tid);
+                __builtin_amdgcn_s_setprio(2);
                 gd_load<R, E, G, true, Rs...>(nxt, in + ntr * N, NR, nb0, tid);
+                __builtin_amdgcn_s_setprio(0);
             }
             gd_compute<R, E, G, MODE, TwTwoLevel, TwLds, true, Rs...>(cur, sd, out + tr * N, twr, N, L, b0,
                                                                       last ? scale : 1.0f, lds_all, tid);
@@ -710,7 +712,9 @@ k_global_pass_direct(const float2* __restrict__ in, float2* __restrict__ out, co
             sd = nsd;
         } else {
             gd_load_side<R, E, G, MODE, TwTwoLevel, Rs...>(sd, twN, win, N, L, b0, tid);
+            __builtin_amdgcn_s_setprio(2);
             gd_load<R, E, G, true, Rs...>(cur, in + tr * N, NR, b0, tid);
+            __builtin_amdgcn_s_setprio(0);
             gd_compute<R, E, G, MODE, TwTwoLevel, TwLds, true, Rs...>(cur, sd, out + tr * N, twr, N, L, b0,
                                                                       last ? scale : 1.0f, lds_all, tid);
         }
@@ -891,6 +895,7 @@ k_global_pass_wide(const float2* __restrict__ in, float2* __restrict__ out, cons
         // round q: column C q + cc at virtual thread t + TQ (q ^ a)
         float2 d[Q][E];
         const float2* src = in + tr * N + b0;
+        __builtin_amdgcn_s_setprio(2);
 #pragma unroll
         for (int s = 0; s < Q; ++s)
 #pragma unroll
@@ -899,6 +904,7 @@ k_global_pass_wide(const float2* __restrict__ in, float2* __restrict__ out, cons
                     reinterpret_cast<const f2v*>(src + (g ^ (C * s)) + (uint64_t)MP::row(e, t + TQ * s) * NR));
                 d[s][e] = make_float2(x.x, x.y);
             }
+        __builtin_amdgcn_s_setprio(0);
         if constexpr (MODE == kWindowed) {
             if (L == 1) {
                 const float* wsrc = win + b0;
@@ -1165,7 +1171,9 @@ k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const flo
             if (nl < total) {
                 const uint64_t ntr = nl / groups_per_tr, nb0 = (nl % groups_per_tr) * G;
                 gp_load_side<R, E, G, MODE>(nsd, twN, win, N, L, nb0, tid);
+                __builtin_amdgcn_s_setprio(2);
                 gp_load<R, E, G, true>(nxt, in + ntr * N, NR, nb0, tid);
+                __builtin_amdgcn_s_setprio(0);
             }
             gp_compute<R, E, G, MODE, TwTwoLevel, TwLds, true, Rs...>(cur, sd, out + tr * N, twr, N, L, b0,
                                                                last ? scale : 1.0f, lds_all, tid);
@@ -1174,7 +1182,9 @@ k_global_pass(const float2* __restrict__ in, float2* __restrict__ out, const flo
             sd = nsd;
         } else {
             gp_load_side<R, E, G, MODE>(sd, twN, win, N, L, b0, tid);
+            __builtin_amdgcn_s_setprio(2);
             gp_load<R, E, G, true>(cur, in + tr * N, NR, b0, tid);
+            __builtin_amdgcn_s_setprio(0);
             gp_compute<R, E, G, MODE, TwTwoLevel, TwLds, true, Rs...>(cur, sd, out + tr * N, twr, N, L, b0,
                                                                last ? scale : 1.0f, lds_all, tid);
         }

@@ -2,7 +2,7 @@
 //
 // The reference's GpuFft::fft/ifft take host vectors and return host vectors
 // (src/lib.rs:540-568: flatten, upload, run, read back).  Copying a caller's
-// pageable buffer straight to the device runs at the driver's staged rate
+// pageable buffer straight to the device runs at the HIP runtime's staged rate
 // (measured on MI355X: 30 GB/s up, 18 GB/s down), and the transform then waits
 // for the whole upload.  Here the batch moves in chunks of whole transforms
 // through two pinned staging slots per direction: host threads copy chunk

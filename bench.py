@@ -11,6 +11,9 @@ The same JSON line also carries, measured in the same run:
     the bandwidth of a plain device copy of the same 256 MiB;
   * fwd_adjoint: the fused forward + adjoint of L = sum m |FFT(w x)|^2 on the
     same shape and its time relative to the plain forward;
+  * spectral_filter: y = IFFT(FFT(x) * s) on the same shape in one fused
+    pass, against the plain forward and against the unfused forward, spectral
+    multiply and inverse (the reference pipeline's stages);
   * other_shapes: the same 16.8 M samples as 256 x 65536 and 16384 x 1024;
   * mixed_batch: 20,000 transforms of lengths 2^k, k ~ U{6..16}, as one
     persistent launch, its hipGraph replay, and one launch per length;
@@ -289,6 +292,18 @@ def main() -> int:
                       max(10, args.steps // 4), max(3, args.warmup // 4))
         out["fwd_adjoint"] = {"ms": round(fa_ms, 5), "plain_fwd_ms": round(kern_ms, 5),
                               "ratio": round(fa_ms / kern_ms, 3)}
+        sp = torch.from_numpy(np.hanning(N + 1)[:N].astype(np.float32)).cuda()
+        k = max(10, args.steps // 4)
+        filt_ms = timed(torch, lambda: gpu.filter_device(x, sp, y), k, max(3, args.warmup // 4))
+
+        def unfused():
+            gpu.exec_device(x, y)
+            y.mul_(sp)
+            gpu.exec_device(y, y, inverse=True)
+
+        unf_ms = timed(torch, unfused, k, max(3, args.warmup // 4))
+        out["spectral_filter"] = {"ms": round(filt_ms, 5), "ratio_to_fwd": round(filt_ms / kern_ms, 3),
+                                  "unfused_ms": round(unf_ms, 5), "speedup_vs_unfused": round(unf_ms / filt_ms, 3)}
         del gx
         x_keep, y_keep = x, y
         del x, y

@@ -112,6 +112,14 @@ int wgfft_fwd_adjoint_masked_device(wgfft_t* h, const float* d_x, const float* d
                                     float* d_y, float* d_gx, float* d_gw, float* d_loss, uint64_t n,
                                     uint64_t batch, void* stream);
 
+/* Spectral filter y = IFFT(FFT(x) * s), 1/N scaled, s real (n floats): the
+ * reference pipeline's FFT, spectral Hann window and IFFT + 1/N stages
+ * (examples/parallel_pipeline.rs:68-105, run at :586-642) in one pass over
+ * memory for N = 512 .. 16384, else on device as forward, multiply and
+ * inverse launches.  d_x == d_y is allowed. */
+int wgfft_filter_device(wgfft_t* h, const float* d_x, const float* d_s, float* d_y, uint64_t n, uint64_t batch,
+                        void* stream);
+
 /* ---- mixed-length batches (no counterpart in the reference: it needs one
  *      GpuFft call per length, src/lib.rs:527-531) ---------------------- */
 

@@ -162,6 +162,65 @@ k_adjoint(AdjointArgs a) {
     }
 }
 
+// Spectral filter y = F^{-1}(s * F(x)) (1/N scaled, s real): the reference
+// pipeline's FFT, spectral window and IFFT + 1/N stages
+// (examples/parallel_pipeline.rs:68-105, run at :586-642) in one pass over
+// memory.  The forward output stays in registers, is multiplied by s at its
+// own indices and goes back through the conjugate-twiddle transform with the
+// same plan, as the adjoint half of k_adjoint does.
+struct FilterArgs {
+    const float2* x;
+    const float* s;
+    float2* y;
+    const float2* tw;
+    uint64_t batch;
+    int grid;
+    hipStream_t stream;
+};
+
+template <int N, int E, int OPT, int... Rs>
+__global__ void __launch_bounds__(N / E)
+k_filter(FilterArgs a) {
+    constexpr int T = N / E;
+    constexpr int LEN = LdsLayout<N, E, Rs...>::len;
+    constexpr bool TWLDS = (OPT & kAdjTwLds) != 0;
+    using IO = Io<N, E, Rs...>;
+    __shared__ __align__(16) float2 lds[LEN];
+    __shared__ __align__(16) float2 lds_tw[TWLDS ? twlds_len(ilog2(N)) : 1];
+    TwLds twl{};
+    if constexpr (TWLDS) {
+        twl = twlds_fill<N>(lds_tw, a.tw, threadIdx.x, T);
+        __syncthreads();
+    }
+    for (uint64_t tr = blockIdx.x; tr < a.batch; tr += gridDim.x) {
+        const uint32_t t = launder_lane(threadIdx.x);
+        float2 d[E];
+        __builtin_amdgcn_s_setprio(2);  // raised while the wave issues its global loads
+        IO::load(d, a.x + tr * N, t);
+        __builtin_amdgcn_s_setprio(0);
+        if constexpr (TWLDS)
+            Engine<N, E, T, false>::run_with(d, lds, twl, t, Radices<Rs...>{});
+        else
+            Engine<N, E, T, false>::run(d, lds, a.tw, t, Radices<Rs...>{});
+        __builtin_amdgcn_sched_barrier(0);
+        float sv[E];
+        IO::load_real(sv, a.s, t);
+#pragma unroll
+        for (int i = 0; i < E; ++i) d[i] = cscale(d[i], sv[i]);
+        WGFFT_LDS_SYNC();  // the forward's last exchange read before the inverse writes
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t ta = launder_lane(t);
+        if constexpr (TWLDS)
+            Engine<N, E, T, true>::run_with(d, lds, twl, ta, Radices<Rs...>{});
+        else
+            Engine<N, E, T, true>::run(d, lds, launder(a.tw), ta, Radices<Rs...>{});
+        __builtin_amdgcn_s_setprio(2);  // and its stores
+        IO::template store<true, true>(a.y + tr * N, d, ta, 1.0f / (float)N);
+        __builtin_amdgcn_s_setprio(0);
+        WGFFT_LDS_SYNC();
+    }
+}
+
 template <int N, int E, int OPT, int... Rs>
 struct AdjointKernel {
     static constexpr int THREADS = N / E;
@@ -181,6 +240,17 @@ struct AdjointKernel {
         return blocks > 0 ? blocks : 1;
     }
     static int blocks_per_cu(bool fwd) { return fwd ? occ<true>() : occ<false>(); }
+    static hipError_t filter(const FilterArgs& a) {
+        hipLaunchKernelGGL((k_filter<N, E, OPT, Rs...>), dim3((unsigned)a.grid), dim3(THREADS), 0, a.stream, a);
+        return hipGetLastError();
+    }
+    static int filter_blocks_per_cu() {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_filter<N, E, OPT, Rs...>, THREADS,
+                                                         0) != hipSuccess)
+            return 1;
+        return blocks > 0 ? blocks : 1;
+    }
 };
 
 struct AdjointVariant {
@@ -190,6 +260,8 @@ struct AdjointVariant {
     const char* plan;
     hipError_t (*launch)(bool, const AdjointArgs&);
     int (*blocks_per_cu)(bool);
+    hipError_t (*filter)(const FilterArgs&);  // the spectral filter with the same plan
+    int (*filter_blocks_per_cu)();
 };
 
 std::vector<AdjointVariant>& adjoint_registry();
@@ -201,6 +273,8 @@ struct AdjointRegistrar {
 #define WGFFT_REGISTER_ADJOINT(N, E, OPT, PLAN, ...)                                                 \
     static ::wgfft::AdjointRegistrar _wgfft_adj_##N##_##E##_##OPT##_##PLAN(::wgfft::AdjointVariant{ \
         ::wgfft::ilog2(N), E, OPT, #PLAN, &::wgfft::AdjointKernel<N, E, OPT, __VA_ARGS__>::launch,   \
-        &::wgfft::AdjointKernel<N, E, OPT, __VA_ARGS__>::blocks_per_cu})
+        &::wgfft::AdjointKernel<N, E, OPT, __VA_ARGS__>::blocks_per_cu,                              \
+        &::wgfft::AdjointKernel<N, E, OPT, __VA_ARGS__>::filter,                                     \
+        &::wgfft::AdjointKernel<N, E, OPT, __VA_ARGS__>::filter_blocks_per_cu})
 
 }  // namespace wgfft

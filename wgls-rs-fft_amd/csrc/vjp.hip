@@ -234,6 +234,17 @@ static hipError_t zero_fill(float* p, uint64_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
+// y[b][k] *= s[k]: the unfused spectral multiply for lengths without a fused
+// filter kernel.
+__global__ void __launch_bounds__(256) k_spectral_mul(float2* __restrict__ y, const float* __restrict__ sp,
+                                                      uint64_t n, uint64_t total) {
+    for (uint64_t e = (uint64_t)blockIdx.x * 256u + threadIdx.x; e < total; e += (uint64_t)gridDim.x * 256u) {
+        const float m = sp[e & (n - 1)];
+        const float2 v = y[e];
+        y[e] = make_float2(m * v.x, m * v.y);
+    }
+}
+
 }  // namespace wgfft
 
 using namespace wgfft;
@@ -334,4 +345,40 @@ extern "C" int wgfft_select_adjoint_variant(wgfft_t* h, uint64_t n, int index, c
     h->tuned_adj[log2n] = vs[(size_t)index];
     adjoint_describe(vs[(size_t)index], desc, desc_len);
     return WGFFT_OK;
+}
+
+// y = F^{-1}(s * F(x)), 1/N scaled: the reference pipeline's FFT, spectral
+// window and IFFT + 1/N stages (examples/parallel_pipeline.rs:68-105,586-642).
+// One fused pass where the length has fused kernels (N = 512 .. 16384), else
+// forward, multiply and inverse launches on device.
+extern "C" int wgfft_filter_device(wgfft_t* h, const float* d_x, const float* d_s, float* d_y, uint64_t n,
+                                   uint64_t batch, void* stream) {
+    int rc;
+    if ((rc = vjp_checks(h, n, batch))) return rc;
+    if (batch == 0) return WGFFT_OK;
+    if (!d_x || !d_s || !d_y) return fail(WGFFT_ERR_INVALID_ARGUMENT, "device buffer is NULL");
+    if ((((uintptr_t)d_x | (uintptr_t)d_y | (uintptr_t)d_s) & 15u) != 0)
+        return fail(WGFFT_ERR_INVALID_ARGUMENT, "device buffers must be 16-byte aligned");
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;  // NULL: HIP default stream
+    Plan* p;
+    if ((rc = h->get_plan(n, &p))) return rc;
+    if (p->kind == Plan::kBatched) {
+        const AdjointVariant* v = h->tuned_adj.count(p->log2n) ? h->tuned_adj[p->log2n] : find_adjoint(p->log2n);
+        if (v) {
+            int grid = v->filter_blocks_per_cu() * h->num_cus;
+            if ((uint64_t)grid > batch) grid = (int)batch;
+            FilterArgs a{(const float2*)d_x, d_s, (float2*)d_y, p->tw, batch, grid, s};
+            if ((e = v->filter(a)) != hipSuccess) return hip_fail(e, "filter kernel launch");
+            return WGFFT_OK;
+        }
+    }
+    if ((rc = h->run(p, kForward, (const float2*)d_x, (float2*)d_y, nullptr, batch, 1.0f, s))) return rc;
+    const uint64_t total = n * batch;
+    const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_spectral_mul, dim3(blocks), dim3(256), 0, s, (float2*)d_y, d_s, n, total);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "spectral multiply launch");
+    return h->run(p, kInverse, (const float2*)d_y, (float2*)d_y, nullptr, batch, 1.0f / (float)n, s);
 }

@@ -176,6 +176,15 @@ class GpuFft(FftExecutor):
             self._h, x.data_ptr(), w.data_ptr(), mask.data_ptr(), 0 if y is None else y.data_ptr(),
             gx.data_ptr(), gw.data_ptr(), loss.data_ptr(), n, batch, self._stream(stream)))
 
+    def filter_device(self, x, s, out, stream=None) -> None:
+        """out = ifft(fft(x) * s), 1/N scaled: a (batch, n) complex64 CUDA
+        tensor filtered by the real spectral window s (n floats); out may be x."""
+        _check_tensor(x)
+        _check_tensor(out)
+        batch, n = x.shape
+        check(self._L.wgfft_filter_device(self._h, x.data_ptr(), s.data_ptr(), out.data_ptr(), n, batch,
+                                          self._stream(stream)))
+
     def bench_device(self, x, out, inverse: bool = False, warmup: int = 3, iters: int = 50) -> float:
         batch, n = x.shape
         sec = ctypes.c_double(0.0)

@@ -23,6 +23,8 @@ The same JSON line also carries, measured in the same run:
     against the plain forward of the same shape;
   * c1_and_host_api: N=1024 x 8 device latency, and the host-buffer API
     (H2D + transform + D2H, PCIe-inclusive) at 1024 x 8 and 4096 x 4096;
+  * rocfft: the same shapes through torch.fft (hipFFT/rocFFT), timed the same
+    way, as an external comparator outside the library path;
   * cpu_baseline: the reference algorithm (the oracle's C restatement of the
     wgls-rs-fft f32 Stockham R4/R2 path) on the host cores, bounded sample.
 """
@@ -193,6 +195,23 @@ def bench_c5(torch, gpu, wgfft, steps: int, tune: bool) -> dict:
             "fused_fwd_adjoint_ms": round(fused, 4), "fused_ratio": round(fused / fwd, 3), "plan": gpu.describe(n)}
 
 
+def bench_rocfft(torch, gpu, steps: int) -> dict:
+    """torch.fft.fft (hipFFT -> rocFFT on ROCm) on the bench shapes, out of
+    place, against this library's default plan on the same tensors."""
+    out = {}
+    for n, batch in ((4096, 4096), (256, 65536), (16384, 1024), (8192, 1024), (1 << 22, 16)):
+        x = (torch.rand(2 * n * batch, device="cuda") * 2 - 1).view(torch.complex64).view(batch, n)
+        y = torch.empty_like(x)
+        k = max(10, steps // 10)
+        ours = timed(torch, lambda: gpu.exec_device(x, y), k, 5)
+        theirs = timed(torch, lambda: torch.fft.fft(x, dim=1), k, 5)
+        out[f"{n}x{batch}"] = {"rocfft_ms": round(theirs, 4), "wgfft_ms": round(ours, 4),
+                               "speedup": round(theirs / ours, 3)}
+        del x, y
+        torch.cuda.empty_cache()
+    return out
+
+
 def bench_large(torch, gpu, steps: int, tune: bool) -> dict:
     out = {}
     for log2n, batch in ((22, 16), (26, 1)):
@@ -220,7 +239,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--headline-only", action="store_true", help="only the N=4096x4096 forward (profiling)")
-    ap.add_argument("--only", choices=("shapes", "mixed", "large", "c1", "c5"),
+    ap.add_argument("--only", choices=("shapes", "mixed", "large", "c1", "c5", "rocfft"),
                     help="profiling: skip the headline and run one secondary section")
     ap.add_argument("--no-tune", action="store_true", help="use the default kernel variants")
     args = ap.parse_args()
@@ -239,7 +258,8 @@ def main() -> int:
                    "mixed": lambda: bench_mixed(torch, gpu, wgfft, args.steps),
                    "large": lambda: bench_large(torch, gpu, args.steps, not args.no_tune),
                    "c1": lambda: bench_c1_and_host(torch, gpu, wgfft),
-                   "c5": lambda: bench_c5(torch, gpu, wgfft, args.steps, not args.no_tune)}[args.only]
+                   "c5": lambda: bench_c5(torch, gpu, wgfft, args.steps, not args.no_tune),
+                   "rocfft": lambda: bench_rocfft(torch, gpu, args.steps)}[args.only]
         print(json.dumps({args.only: section()}))
         gpu.close()
         return 0
@@ -312,6 +332,7 @@ def main() -> int:
         out["large_n"] = bench_large(torch, gpu, args.steps, not args.no_tune)
         out["c5_windowed"] = bench_c5(torch, gpu, wgfft, args.steps, not args.no_tune)
         out["c1_and_host_api"] = bench_c1_and_host(torch, gpu, wgfft)
+        out["rocfft"] = bench_rocfft(torch, gpu, args.steps)
         del x_keep, y_keep
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

@@ -90,6 +90,17 @@ int wgfft_exec_host(wgfft_t* h, const float* in, float* out, uint64_t n, uint64_
 int wgfft_exec_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, uint64_t batch, int dir,
                       void* stream);
 
+/* Batched FFT of strided device layouts: element j of signal b is at complex
+ * offset b*idist + j*istride of d_in (b*odist + j*ostride of d_out); 8-byte
+ * aligned buffers; the output layout must not overlap itself.  The reference
+ * lists configurable strides and distances as future work (REQUIREMENTS.md:14,
+ * "Stride Management").  Unit stride with distance n is the plain device call;
+ * other layouts are gathered into a contiguous scratch buffer, transformed in
+ * place there and scattered, on the caller's stream. */
+int wgfft_exec_strided_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, uint64_t batch,
+                              uint64_t istride, uint64_t idist, uint64_t ostride, uint64_t odist, int dir,
+                              void* stream);
+
 /* Windowed forward Y_b = FFT(w * x_b), w real (n floats) shared by the batch,
  * multiplied at load (no extra pass).  Replaces apply_hann_window + fft
  * (tests/test_utils.rs:39-50, examples/windowed_fft.rs:14-23). */

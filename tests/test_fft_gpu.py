@@ -461,6 +461,34 @@ def test_host_pipeline_matches_device(gpu, torch_cuda, n, batch):
     assert np.array_equal(yw, d.cpu().numpy())
 
 
+@pytest.mark.parametrize("n,batch", [(256, 33), (4096, 17), (1 << 17, 3)])
+def test_strided_layouts(gpu, torch_cuda, n, batch):
+    """Every other element of a wider buffer in, a transposed layout out, and
+    the plain layout through the same entry: the same results as the
+    contiguous device call, forward and inverse."""
+    torch = torch_cuda
+    x = oracle.synth_uniform((batch, n), 0x5EED0900 + n)
+    t = torch.from_numpy(x).cuda()
+    wide = torch.zeros((batch, 2 * n), dtype=torch.complex64, device="cuda")
+    wide[:, ::2] = t
+    src = wide[:, ::2]  # strides (2n, 2)
+    for inverse in (False, True):
+        ref = torch.empty_like(t)
+        gpu.exec_device(t, ref, inverse=inverse)
+        dst = torch.empty((n, batch), dtype=torch.complex64, device="cuda").t()  # strides (1, batch)
+        gpu.exec_strided_device(src, dst, inverse=inverse)
+        plain = torch.empty_like(t)
+        gpu.exec_strided_device(t, plain, inverse=inverse)
+        torch.cuda.synchronize()
+        want = ref.cpu().numpy()
+        assert np.array_equal(dst.cpu().numpy(), want), inverse
+        assert np.array_equal(plain.cpu().numpy(), want), inverse
+    assert oracle.rel_l2(want, oracle.fft64(x, +1) / n) <= rel_bar(n)
+    with pytest.raises(wgfft.WgfftError):  # a zero stride
+        wgfft._lib.check(gpu._L.wgfft_exec_strided_device(gpu._h, t.data_ptr(), t.data_ptr(), n, batch, 0, n, 1, n,
+                                                          -1, None))
+
+
 def test_windowed_device_c5_shape(gpu, torch_cuda, parity):
     torch = torch_cuda
     n, batch = 8192, 1024

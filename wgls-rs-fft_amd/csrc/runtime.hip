@@ -399,6 +399,23 @@ int Handle::run(Plan* p, Mode mode, const float2* in, float2* out, const float* 
     return WGFFT_OK;
 }
 
+// Strided layouts: element j of signal b at base[b dist + j stride] (complex
+// elements).  Gathered into a contiguous buffer, transformed in place there,
+// scattered out; n is a power of two.
+__global__ void __launch_bounds__(256) k_gather(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                uint32_t log2n, uint64_t total, uint64_t stride, uint64_t dist) {
+    const uint64_t mask = (1ull << log2n) - 1;
+    for (uint64_t e = (uint64_t)blockIdx.x * 256u + threadIdx.x; e < total; e += (uint64_t)gridDim.x * 256u)
+        dst[e] = src[(e >> log2n) * dist + (e & mask) * stride];
+}
+
+__global__ void __launch_bounds__(256) k_scatter(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                 uint32_t log2n, uint64_t total, uint64_t stride, uint64_t dist) {
+    const uint64_t mask = (1ull << log2n) - 1;
+    for (uint64_t e = (uint64_t)blockIdx.x * 256u + threadIdx.x; e < total; e += (uint64_t)gridDim.x * 256u)
+        dst[(e >> log2n) * dist + (e & mask) * stride] = src[e];
+}
+
 }  // namespace wgfft
 
 using namespace wgfft;
@@ -531,6 +548,42 @@ int wgfft_exec_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, u
     bool inv = dir == WGFFT_INVERSE;
     return h->run(p, inv ? kInverse : kForward, (const float2*)d_in, (float2*)d_out, nullptr, batch,
                   inv ? 1.0f / (float)n : 1.0f, pick_stream(h, stream));
+}
+
+int wgfft_exec_strided_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, uint64_t batch,
+                              uint64_t istride, uint64_t idist, uint64_t ostride, uint64_t odist, int dir,
+                              void* stream) {
+    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
+    int rc;
+    if ((rc = check_len(n)) || (rc = check_dir(dir)) || (rc = check_size(n, batch))) return rc;
+    if (batch == 0) return WGFFT_OK;
+    if (!d_in || !d_out) return fail(WGFFT_ERR_INVALID_ARGUMENT, "device buffer is NULL");
+    if (istride == 0 || ostride == 0 || (batch > 1 && (idist == 0 || odist == 0)))
+        return fail(WGFFT_ERR_INVALID_ARGUMENT, "strides and distances must be positive");
+    if (((uintptr_t)d_in | (uintptr_t)d_out) & 7u)
+        return fail(WGFFT_ERR_INVALID_ARGUMENT, "device buffers must be 8-byte aligned");
+    if (istride == 1 && idist == n && ostride == 1 && odist == n && ((((uintptr_t)d_in | (uintptr_t)d_out) & 15u) == 0))
+        return wgfft_exec_device(h, d_in, d_out, n, batch, dir, stream);
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    Plan* p;
+    if ((rc = h->get_plan(n, &p))) return rc;
+    const uint64_t total = n * batch;
+    if ((rc = h->ensure(h->io_aux, total * sizeof(float2)))) return rc;
+    hipStream_t s = pick_stream(h, stream);
+    const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 8192);
+    float2* aux = (float2*)h->io_aux.p;
+    const uint32_t log2n = (uint32_t)log2u(n);
+    hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), 0, s, (const float2*)d_in, aux, log2n, total, istride, idist);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "gather launch");
+    const bool inv = dir == WGFFT_INVERSE;
+    if ((rc = h->run(p, inv ? kInverse : kForward, aux, aux, nullptr, batch, inv ? 1.0f / (float)n : 1.0f, s)))
+        return rc;
+    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, (const float2*)aux, (float2*)d_out, log2n, total,
+                       ostride, odist);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "scatter launch");
+    return WGFFT_OK;
 }
 
 int wgfft_exec_host(wgfft_t* h, const float* in, float* out, uint64_t n, uint64_t batch, int dir) {

@@ -176,6 +176,21 @@ class GpuFft(FftExecutor):
             self._h, x.data_ptr(), w.data_ptr(), mask.data_ptr(), 0 if y is None else y.data_ptr(),
             gx.data_ptr(), gw.data_ptr(), loss.data_ptr(), n, batch, self._stream(stream)))
 
+    def exec_strided_device(self, x, out, inverse: bool = False, stream=None) -> None:
+        """Transform each row of a (batch, n) complex64 CUDA tensor view with
+        any element strides (a slice, a transpose) into `out`, another such
+        view; the tensors' own strides give the layouts."""
+        if x.dtype != out.dtype or str(x.dtype) != "torch.complex64" or not x.is_cuda or not out.is_cuda:
+            raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT, "expected complex64 CUDA tensors")
+        if x.dim() != 2 or tuple(out.shape) != tuple(x.shape):
+            raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT, "expected two (batch, n) views of the same shape")
+        batch, n = x.shape
+        idist, istride = x.stride()
+        odist, ostride = out.stride()
+        check(self._L.wgfft_exec_strided_device(self._h, x.data_ptr(), out.data_ptr(), n, batch, istride, idist,
+                                                ostride, odist, WGFFT_INVERSE if inverse else WGFFT_FORWARD,
+                                                self._stream(stream)))
+
     def filter_device(self, x, s, out, stream=None) -> None:
         """out = ifft(fft(x) * s), 1/N scaled: a (batch, n) complex64 CUDA
         tensor filtered by the real spectral window s (n floats); out may be x."""

@@ -131,6 +131,13 @@ int wgfft_fwd_adjoint_masked_device(wgfft_t* h, const float* d_x, const float* d
 int wgfft_filter_device(wgfft_t* h, const float* d_x, const float* d_s, float* d_y, uint64_t n, uint64_t batch,
                         void* stream);
 
+/* The pipeline demo's Gaussian noise stage (examples/parallel_pipeline.rs:
+ * 29-63, NOISE_WGSL), in place on d_x (8-byte aligned): sample t of signal b
+ * gets sigma * sqrt(-2 ln u1) * (cos, sin)(2 pi u2) from the demo's hash of
+ * (t, b, seed); the demo uses sigma = 0.1, seed 0. */
+int wgfft_add_noise_device(wgfft_t* h, float* d_x, uint64_t n, uint64_t batch, uint32_t seed, float sigma,
+                           void* stream);
+
 /* ---- mixed-length batches (no counterpart in the reference: it needs one
  *      GpuFft call per length, src/lib.rs:527-531) ---------------------- */
 

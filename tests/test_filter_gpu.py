@@ -77,3 +77,72 @@ def test_filter_errors(gpu, torch_cuda):
     with pytest.raises(wgfft.WgfftError) as e:
         gpu.filter_device(x, s, x)
     assert e.value.code == wgfft.WGFFT_ERR_INVALID_LENGTH
+
+
+def _noise_ref(n, rows, cols, seed, sigma):
+    """The pipeline demo's noise formula in FP64 from its 32-bit hash, for the
+    samples (rows x cols) of a batch: a check of the index mapping and the hash,
+    not of the arithmetic (which the moment tests below pin)."""
+    M = 0xFFFFFFFF
+
+    def pcg(v):
+        s = (v * 747796405 + 2891336453) & M
+        w = (((s >> ((s >> 28) + 4)) ^ s) * 277803737) & M
+        return (w >> 22) ^ w
+
+    out = np.empty((len(rows), len(cols)), np.complex128)
+    for i, b in enumerate(rows):
+        for j, t in enumerate(cols):
+            h1 = pcg((t * 1234567 + b * 7654321 + seed) & M)
+            h2 = pcg((h1 + 999983) & M)
+            u1 = max(h1 / 2.0 ** 32, 1e-7)
+            u2 = h2 / 2.0 ** 32 * 2 * np.pi
+            out[i, j] = sigma * np.sqrt(-2 * np.log(u1)) * np.exp(1j * u2)
+    return out
+
+
+def test_noise_stage(gpu, torch_cuda):
+    """Deterministic per seed, N(0, sigma^2) per component (mean, variance,
+    re/im correlation over 2^22 samples), added in place, and the demo's index
+    mapping: sample t of signal b hashed from (t, b, seed)."""
+    torch = torch_cuda
+    n, batch, sigma = 4096, 1024, 0.1
+    z = torch.zeros((batch, n), dtype=torch.complex64, device="cuda")
+    gpu.add_noise_device(z, seed=0, sigma=sigma)
+    a = z.cpu().numpy().astype(np.complex128)
+    z2 = torch.zeros_like(z)
+    gpu.add_noise_device(z2, seed=0, sigma=sigma)
+    assert np.array_equal(z2.cpu().numpy(), z.cpu().numpy())
+    z3 = torch.zeros_like(z)
+    gpu.add_noise_device(z3, seed=12345, sigma=sigma)
+    assert not np.array_equal(z3.cpu().numpy(), z.cpu().numpy())
+    m = a.size
+    for comp in (a.real, a.imag):
+        assert abs(comp.mean()) < 4 * sigma / np.sqrt(m)
+        assert abs(comp.std() / sigma - 1) < 0.01
+    assert abs(np.mean(a.real * a.imag)) / sigma ** 2 < 4 / np.sqrt(m)
+    rows, cols = [0, 1, 513, batch - 1], [0, 1, 7, 2048, n - 1]
+    ref = _noise_ref(n, rows, cols, 0, sigma)
+    assert np.abs(a[np.ix_(rows, cols)] - ref).max() < 1e-5
+    # added in place on top of a signal
+    x = oracle.synth_uniform((batch, n), 0x5EED0820)
+    tx = torch.from_numpy(x).cuda()
+    gpu.add_noise_device(tx, seed=0, sigma=sigma)
+    assert np.abs(tx.cpu().numpy() - (x + a)).max() < 1e-6
+
+
+def test_pipeline_noise_then_filter(gpu, torch_cuda):
+    """The demo's chain on device: noise, then FFT, spectral Hann window
+    (N - 1 denominator), IFFT and 1/N as the fused filter, against FP64."""
+    torch = torch_cuda
+    n, batch = 4096, 64
+    x = oracle.synth_uniform((batch, n), 0x5EED0830)
+    tx = torch.from_numpy(x).cuda()
+    gpu.add_noise_device(tx, seed=7, sigma=0.1)
+    noisy = tx.cpu().numpy()
+    k = np.arange(n)
+    s = (0.5 * (1 - np.cos(2 * np.pi * k / (n - 1)))).astype(np.float32)
+    ty = torch.empty_like(tx)
+    gpu.filter_device(tx, torch.from_numpy(s).cuda(), ty)
+    torch.cuda.synchronize()
+    assert oracle.rel_l2(ty.cpu().numpy(), filter64(noisy, s)) <= 5e-6

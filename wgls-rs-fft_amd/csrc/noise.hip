@@ -1,0 +1,68 @@
+// noise.hip -- the reference pipeline's Gaussian noise stage on device.
+//
+// examples/parallel_pipeline.rs:29-63 (NOISE_WGSL) adds, in place, to sample
+// t of signal b: r (cos u2, sin u2) with r = sigma sqrt(-2 ln u1) (Box-Muller),
+// u1 = max(h1 / 2^32, 1e-7), u2 = 2 pi h2 / 2^32, h1 = H(t 1234567 + b 7654321
+// + seed), h2 = H(h1 + 999983) in wrapping 32-bit arithmetic, where H is the
+// PCG-RXS-M-XS integer hash (O'Neill, "PCG: A Family of Simple Fast
+// Space-Efficient Statistically Good Algorithms for Random Number
+// Generation", 2014; as a GPU hash in Jarzynski and Olano, "Hash Functions
+// for GPU Rendering", JCGT 2020).  The demo uses sigma = 0.1 and seed 0.
+// The same index mapping and hash are kept so that a signal gets the same
+// noise pattern as in the demo.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/wgfft.h"
+#include "runtime_internal.hpp"
+
+namespace wgfft {
+
+__device__ __forceinline__ uint32_t pcg_hash(uint32_t v) {
+    const uint32_t s = v * 747796405u + 2891336453u;
+    const uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    return (w >> 22u) ^ w;
+}
+
+__global__ void __launch_bounds__(256) k_add_noise(float2* __restrict__ x, uint32_t log2n, uint64_t total,
+                                                   uint32_t seed, float sigma) {
+    const uint64_t mask = (1ull << log2n) - 1;
+    for (uint64_t e = (uint64_t)blockIdx.x * 256u + threadIdx.x; e < total; e += (uint64_t)gridDim.x * 256u) {
+        const uint32_t t = (uint32_t)(e & mask), b = (uint32_t)(e >> log2n);
+        const uint32_t h1 = pcg_hash(t * 1234567u + b * 7654321u + seed);
+        const uint32_t h2 = pcg_hash(h1 + 999983u);
+        const float u1 = fmaxf((float)h1 * 2.3283064365386963e-10f, 1e-7f);
+        const float u2 = (float)h2 * 2.3283064365386963e-10f * 6.283185307179586f;
+        const float r = sqrtf(-2.0f * logf(u1)) * sigma;
+        float sn, cs;
+        sincosf(u2, &sn, &cs);
+        const float2 v = x[e];
+        x[e] = make_float2(v.x + r * cs, v.y + r * sn);
+    }
+}
+
+}  // namespace wgfft
+
+using namespace wgfft;
+
+extern "C" int wgfft_add_noise_device(wgfft_t* h, float* d_x, uint64_t n, uint64_t batch, uint32_t seed,
+                                      float sigma, void* stream) {
+    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (n == 0 || (n & (n - 1))) return fail(WGFFT_ERR_INVALID_LENGTH, kMsgLength);
+    if (batch && n > (~0ull) / 8 / batch) return fail(WGFFT_ERR_INVALID_ARGUMENT, "n * batch overflows");
+    if (batch == 0) return WGFFT_OK;
+    if (!d_x) return fail(WGFFT_ERR_INVALID_ARGUMENT, "device buffer is NULL");
+    if ((uintptr_t)d_x & 7u) return fail(WGFFT_ERR_INVALID_ARGUMENT, "device buffer must be 8-byte aligned");
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    uint32_t log2n = 0;
+    while ((1ull << log2n) < n) ++log2n;
+    const uint64_t total = n * batch;
+    const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_add_noise, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (float2*)d_x, log2n, total, seed,
+                       sigma);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "noise launch");
+    return WGFFT_OK;
+}

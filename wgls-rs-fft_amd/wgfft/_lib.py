@@ -26,7 +26,7 @@ EXPORTS = (
     "wgfft_device_available", "wgfft_create", "wgfft_destroy", "wgfft_last_error", "wgfft_name",
     "wgfft_validate_lengths", "wgfft_exec_host", "wgfft_exec_device", "wgfft_exec_windowed_host",
     "wgfft_exec_windowed_device", "wgfft_vjp_windowed_device", "wgfft_fwd_adjoint_masked_device",
-    "wgfft_filter_device", "wgfft_exec_strided_device",
+    "wgfft_filter_device", "wgfft_exec_strided_device", "wgfft_add_noise_device",
     "wgfft_ragged_create", "wgfft_ragged_destroy", "wgfft_ragged_exec_device", "wgfft_ragged_samples", "wgfft_ragged_describe",
     "wgfft_bench_device",
     "wgfft_autotune", "wgfft_plan_describe", "wgfft_variant_count", "wgfft_select_variant",
@@ -84,6 +84,7 @@ def load() -> ctypes.CDLL:
     L.wgfft_fwd_adjoint_masked_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _u64, _vp]
     L.wgfft_filter_device.argtypes = [_vp, _vp, _vp, _vp, _u64, _u64, _vp]
     L.wgfft_exec_strided_device.argtypes = [_vp, _vp, _vp, _u64, _u64, _u64, _u64, _u64, _u64, _i, _vp]
+    L.wgfft_add_noise_device.argtypes = [_vp, _vp, _u64, _u64, ctypes.c_uint32, ctypes.c_float, _vp]
     L.wgfft_ragged_create.argtypes = [_vp, _vp, _vp, _u64, ctypes.POINTER(_vp)]
     L.wgfft_ragged_destroy.argtypes = [_vp]
     L.wgfft_ragged_destroy.restype = None

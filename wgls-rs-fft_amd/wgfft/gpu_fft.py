@@ -191,6 +191,14 @@ class GpuFft(FftExecutor):
                                                 ostride, odist, WGFFT_INVERSE if inverse else WGFFT_FORWARD,
                                                 self._stream(stream)))
 
+    def add_noise_device(self, x, seed: int = 0, sigma: float = 0.1, stream=None) -> None:
+        """The reference pipeline's Gaussian noise stage, in place on a
+        (batch, n) complex64 CUDA tensor."""
+        _check_tensor(x)
+        batch, n = x.shape
+        check(self._L.wgfft_add_noise_device(self._h, x.data_ptr(), n, batch, seed & 0xFFFFFFFF, sigma,
+                                             self._stream(stream)))
+
     def filter_device(self, x, s, out, stream=None) -> None:
         """out = ifft(fft(x) * s), 1/N scaled: a (batch, n) complex64 CUDA
         tensor filtered by the real spectral window s (n floats); out may be x."""

@@ -123,13 +123,15 @@ int wgfft_fwd_adjoint_masked_device(wgfft_t* h, const float* d_x, const float* d
                                     float* d_y, float* d_gx, float* d_gw, float* d_loss, uint64_t n,
                                     uint64_t batch, void* stream);
 
-/* Spectral filter y = IFFT(FFT(x) * s), 1/N scaled, s real (n floats): the
- * reference pipeline's FFT, spectral Hann window and IFFT + 1/N stages
- * (examples/parallel_pipeline.rs:68-105, run at :586-642) in one pass over
- * memory for N = 512 .. 16384, else on device as forward, multiply and
- * inverse launches.  d_x == d_y is allowed. */
+/* Spectral filter y = IFFT(FFT(x + noise) * s), 1/N scaled, s real (n
+ * floats): the reference pipeline's Gaussian noise (when sigma != 0; the
+ * noise of wgfft_add_noise_device), FFT, spectral Hann window and IFFT + 1/N
+ * stages (examples/parallel_pipeline.rs:29-105, run at :586-642) in one pass
+ * over memory for N = 512 .. 16384, else on device as noise, forward,
+ * multiply and inverse launches.  d_x == d_y is allowed; d_x is not changed
+ * unless it is d_y. */
 int wgfft_filter_device(wgfft_t* h, const float* d_x, const float* d_s, float* d_y, uint64_t n, uint64_t batch,
-                        void* stream);
+                        uint32_t seed, float sigma, void* stream);
 
 /* The pipeline demo's Gaussian noise stage (examples/parallel_pipeline.rs:
  * 29-63, NOISE_WGSL), in place on d_x (8-byte aligned): sample t of signal b

@@ -48,7 +48,7 @@ def test_null_handle_paths():
     L = _lib.load()
     assert L.wgfft_exec_host(None, None, None, 8, 1, -1) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
     assert L.wgfft_exec_device(None, None, None, 8, 1, -1, None) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
-    assert L.wgfft_filter_device(None, None, None, None, 8, 1, None) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
+    assert L.wgfft_filter_device(None, None, None, None, 8, 1, 0, 0.0, None) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
     assert L.wgfft_exec_strided_device(None, None, None, 8, 1, 1, 8, 1, 8, -1, None) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
     assert L.wgfft_add_noise_device(None, None, 8, 1, 0, 0.1, None) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
     assert "handle" in _lib.last_error()

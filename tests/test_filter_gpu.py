@@ -131,18 +131,31 @@ def test_noise_stage(gpu, torch_cuda):
     assert np.abs(tx.cpu().numpy() - (x + a)).max() < 1e-6
 
 
-def test_pipeline_noise_then_filter(gpu, torch_cuda):
+@pytest.mark.parametrize("n,batch", [(4096, 64), (256, 64), (1 << 17, 2)])
+def test_pipeline_noise_then_filter(gpu, torch_cuda, n, batch):
     """The demo's chain on device: noise, then FFT, spectral Hann window
-    (N - 1 denominator), IFFT and 1/N as the fused filter, against FP64."""
+    (N - 1 denominator), IFFT and 1/N, as two calls and as the one filter call
+    with the noise at its load (fused at 4096, launches at 256 and 2^17),
+    against FP64; x itself is left unchanged by the one-call form."""
     torch = torch_cuda
-    n, batch = 4096, 64
-    x = oracle.synth_uniform((batch, n), 0x5EED0830)
+    x = oracle.synth_uniform((batch, n), 0x5EED0830 + n)
+    k = np.arange(n)
+    s = (0.5 * (1 - np.cos(2 * np.pi * k / (n - 1)))).astype(np.float32)
+    ts = torch.from_numpy(s).cuda()
     tx = torch.from_numpy(x).cuda()
     gpu.add_noise_device(tx, seed=7, sigma=0.1)
     noisy = tx.cpu().numpy()
-    k = np.arange(n)
-    s = (0.5 * (1 - np.cos(2 * np.pi * k / (n - 1)))).astype(np.float32)
+    want = filter64(noisy, s)
     ty = torch.empty_like(tx)
-    gpu.filter_device(tx, torch.from_numpy(s).cuda(), ty)
+    gpu.filter_device(tx, ts, ty)
     torch.cuda.synchronize()
-    assert oracle.rel_l2(ty.cpu().numpy(), filter64(noisy, s)) <= 5e-6
+    two_calls = ty.cpu().numpy()
+    assert oracle.rel_l2(two_calls, want) <= 5e-6
+    tx2 = torch.from_numpy(x).cuda()
+    ty2 = torch.empty_like(tx2)
+    gpu.filter_device(tx2, ts, ty2, noise_sigma=0.1, seed=7)
+    torch.cuda.synchronize()
+    assert np.array_equal(tx2.cpu().numpy(), x)
+    one_call = ty2.cpu().numpy()
+    assert oracle.rel_l2(one_call, want) <= 5e-6
+    assert oracle.rel_l2(one_call, two_calls) <= 1e-6

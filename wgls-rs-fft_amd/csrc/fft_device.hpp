@@ -55,6 +55,25 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 }
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 
+// The pipeline demo's Gaussian noise for sample t of signal b
+// (examples/parallel_pipeline.rs:29-63): Box-Muller from two PCG-RXS-M-XS
+// hashes of (t, b, seed) in wrapping 32-bit arithmetic.
+__device__ __forceinline__ uint32_t pcg_hash(uint32_t v) {
+    const uint32_t s = v * 747796405u + 2891336453u;
+    const uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    return (w >> 22u) ^ w;
+}
+__device__ __forceinline__ float2 pipeline_noise(uint32_t t, uint32_t b, uint32_t seed, float sigma) {
+    const uint32_t h1 = pcg_hash(t * 1234567u + b * 7654321u + seed);
+    const uint32_t h2 = pcg_hash(h1 + 999983u);
+    const float u1 = fmaxf((float)h1 * 2.3283064365386963e-10f, 1e-7f);
+    const float u2 = (float)h2 * 2.3283064365386963e-10f * 6.283185307179586f;
+    const float r = sqrtf(-2.0f * logf(u1)) * sigma;
+    float sn, cs;
+    sincosf(u2, &sn, &cs);
+    return make_float2(r * cs, r * sn);
+}
+
 // a * W_4 with W_4 = -i (forward) or +i (inverse)
 template <bool INV>
 __device__ __forceinline__ float2 mul_w4(float2 a) {

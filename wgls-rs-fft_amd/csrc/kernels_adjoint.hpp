@@ -162,10 +162,10 @@ k_adjoint(AdjointArgs a) {
     }
 }
 
-// Spectral filter y = F^{-1}(s * F(x)) (1/N scaled, s real): the reference
-// pipeline's FFT, spectral window and IFFT + 1/N stages
-// (examples/parallel_pipeline.rs:68-105, run at :586-642) in one pass over
-// memory.  The forward output stays in registers, is multiplied by s at its
+// Spectral filter y = F^{-1}(s * F(x + noise)) (1/N scaled, s real): the
+// reference pipeline's noise (optional), FFT, spectral window and IFFT + 1/N
+// stages (examples/parallel_pipeline.rs:29-105, run at :586-642) in one pass
+// over memory.  The forward output stays in registers, is multiplied by s at its
 // own indices and goes back through the conjugate-twiddle transform with the
 // same plan, as the adjoint half of k_adjoint does.
 struct FilterArgs {
@@ -176,6 +176,8 @@ struct FilterArgs {
     uint64_t batch;
     int grid;
     hipStream_t stream;
+    uint32_t seed;  // the pipeline's noise stage on the loaded samples when sigma != 0
+    float sigma;
 };
 
 template <int N, int E, int OPT, int... Rs>
@@ -198,6 +200,10 @@ k_filter(FilterArgs a) {
         __builtin_amdgcn_s_setprio(2);  // raised while the wave issues its global loads
         IO::load(d, a.x + tr * N, t);
         __builtin_amdgcn_s_setprio(0);
+        if (a.sigma != 0.f) {
+#pragma unroll
+            for (int i = 0; i < E; ++i) d[i] = cadd(d[i], pipeline_noise(IO::elem(i, t), (uint32_t)tr, a.seed, a.sigma));
+        }
         if constexpr (TWLDS)
             Engine<N, E, T, false>::run_with(d, lds, twl, t, Radices<Rs...>{});
         else

@@ -9,7 +9,8 @@
 // Generation", 2014; as a GPU hash in Jarzynski and Olano, "Hash Functions
 // for GPU Rendering", JCGT 2020).  The demo uses sigma = 0.1 and seed 0.
 // The same index mapping and hash are kept so that a signal gets the same
-// noise pattern as in the demo.
+// noise pattern as in the demo (pipeline_noise in fft_device.hpp, shared with
+// the fused filter, which adds it to the samples it loads).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,27 +20,22 @@
 
 namespace wgfft {
 
-__device__ __forceinline__ uint32_t pcg_hash(uint32_t v) {
-    const uint32_t s = v * 747796405u + 2891336453u;
-    const uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
-    return (w >> 22u) ^ w;
-}
-
-__global__ void __launch_bounds__(256) k_add_noise(float2* __restrict__ x, uint32_t log2n, uint64_t total,
+// dst may be src (in place): no restrict qualifiers
+__global__ void __launch_bounds__(256) k_add_noise(const float2* src, float2* dst, uint32_t log2n, uint64_t total,
                                                    uint32_t seed, float sigma) {
     const uint64_t mask = (1ull << log2n) - 1;
-    for (uint64_t e = (uint64_t)blockIdx.x * 256u + threadIdx.x; e < total; e += (uint64_t)gridDim.x * 256u) {
-        const uint32_t t = (uint32_t)(e & mask), b = (uint32_t)(e >> log2n);
-        const uint32_t h1 = pcg_hash(t * 1234567u + b * 7654321u + seed);
-        const uint32_t h2 = pcg_hash(h1 + 999983u);
-        const float u1 = fmaxf((float)h1 * 2.3283064365386963e-10f, 1e-7f);
-        const float u2 = (float)h2 * 2.3283064365386963e-10f * 6.283185307179586f;
-        const float r = sqrtf(-2.0f * logf(u1)) * sigma;
-        float sn, cs;
-        sincosf(u2, &sn, &cs);
-        const float2 v = x[e];
-        x[e] = make_float2(v.x + r * cs, v.y + r * sn);
-    }
+    for (uint64_t e = (uint64_t)blockIdx.x * 256u + threadIdx.x; e < total; e += (uint64_t)gridDim.x * 256u)
+        dst[e] = cadd(src[e], pipeline_noise((uint32_t)(e & mask), (uint32_t)(e >> log2n), seed, sigma));
+}
+
+hipError_t launch_add_noise(const float2* src, float2* dst, uint64_t n, uint64_t batch, uint32_t seed, float sigma,
+                            hipStream_t s) {
+    uint32_t log2n = 0;
+    while ((1ull << log2n) < n) ++log2n;
+    const uint64_t total = n * batch;
+    const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_add_noise, dim3(blocks), dim3(256), 0, s, src, dst, log2n, total, seed, sigma);
+    return hipGetLastError();
 }
 
 }  // namespace wgfft
@@ -57,12 +53,8 @@ extern "C" int wgfft_add_noise_device(wgfft_t* h, float* d_x, uint64_t n, uint64
     std::lock_guard<std::mutex> lk(h->mu);
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    uint32_t log2n = 0;
-    while ((1ull << log2n) < n) ++log2n;
-    const uint64_t total = n * batch;
-    const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_add_noise, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (float2*)d_x, log2n, total, seed,
-                       sigma);
-    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "noise launch");
+    if ((e = launch_add_noise((const float2*)d_x, (float2*)d_x, n, batch, seed, sigma, (hipStream_t)stream)) !=
+        hipSuccess)
+        return hip_fail(e, "noise launch");
     return WGFFT_OK;
 }

@@ -56,6 +56,8 @@ int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x
 const char* adjoint_describe(const AdjointVariant* v, char* buf, uint64_t len);
 void synth_uniform(float* out, uint64_t nfloats, uint64_t seed);
 const BatchedVariant* default_variant(int log2n);
+hipError_t launch_add_noise(const float2* src, float2* dst, uint64_t n, uint64_t batch, uint32_t seed,
+                            float sigma, hipStream_t s);
 std::vector<int> multipass_radices(int log2n);
 std::vector<std::vector<int>> multipass_splits(int log2n);
 const BatchedVariant* find_variant(int log2n, const char* plan, int E, int G, int flags);

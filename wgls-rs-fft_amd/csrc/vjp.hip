@@ -347,12 +347,13 @@ extern "C" int wgfft_select_adjoint_variant(wgfft_t* h, uint64_t n, int index, c
     return WGFFT_OK;
 }
 
-// y = F^{-1}(s * F(x)), 1/N scaled: the reference pipeline's FFT, spectral
-// window and IFFT + 1/N stages (examples/parallel_pipeline.rs:68-105,586-642).
+// y = F^{-1}(s * F(x + noise)), 1/N scaled: the reference pipeline's noise
+// (sigma != 0), FFT, spectral window and IFFT + 1/N stages
+// (examples/parallel_pipeline.rs:29-105,586-642).
 // One fused pass where the length has fused kernels (N = 512 .. 16384), else
 // forward, multiply and inverse launches on device.
 extern "C" int wgfft_filter_device(wgfft_t* h, const float* d_x, const float* d_s, float* d_y, uint64_t n,
-                                   uint64_t batch, void* stream) {
+                                   uint64_t batch, uint32_t seed, float sigma, void* stream) {
     int rc;
     if ((rc = vjp_checks(h, n, batch))) return rc;
     if (batch == 0) return WGFFT_OK;
@@ -370,12 +371,18 @@ extern "C" int wgfft_filter_device(wgfft_t* h, const float* d_x, const float* d_
         if (v) {
             int grid = v->filter_blocks_per_cu() * h->num_cus;
             if ((uint64_t)grid > batch) grid = (int)batch;
-            FilterArgs a{(const float2*)d_x, d_s, (float2*)d_y, p->tw, batch, grid, s};
+            FilterArgs a{(const float2*)d_x, d_s, (float2*)d_y, p->tw, batch, grid, s, seed, sigma};
             if ((e = v->filter(a)) != hipSuccess) return hip_fail(e, "filter kernel launch");
             return WGFFT_OK;
         }
     }
-    if ((rc = h->run(p, kForward, (const float2*)d_x, (float2*)d_y, nullptr, batch, 1.0f, s))) return rc;
+    const float2* src = (const float2*)d_x;
+    if (sigma != 0.f) {  // y = x + noise, then the forward in place
+        if ((e = launch_add_noise(src, (float2*)d_y, n, batch, seed, sigma, s)) != hipSuccess)
+            return hip_fail(e, "noise launch");
+        src = (const float2*)d_y;
+    }
+    if ((rc = h->run(p, kForward, src, (float2*)d_y, nullptr, batch, 1.0f, s))) return rc;
     const uint64_t total = n * batch;
     const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 4096);
     hipLaunchKernelGGL(k_spectral_mul, dim3(blocks), dim3(256), 0, s, (float2*)d_y, d_s, n, total);

@@ -82,7 +82,7 @@ def load() -> ctypes.CDLL:
     L.wgfft_exec_windowed_device.argtypes = [_vp, _vp, _vp, _vp, _u64, _u64, _vp]
     L.wgfft_vjp_windowed_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _u64, _u64, _vp]
     L.wgfft_fwd_adjoint_masked_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _u64, _vp]
-    L.wgfft_filter_device.argtypes = [_vp, _vp, _vp, _vp, _u64, _u64, _vp]
+    L.wgfft_filter_device.argtypes = [_vp, _vp, _vp, _vp, _u64, _u64, ctypes.c_uint32, ctypes.c_float, _vp]
     L.wgfft_exec_strided_device.argtypes = [_vp, _vp, _vp, _u64, _u64, _u64, _u64, _u64, _u64, _i, _vp]
     L.wgfft_add_noise_device.argtypes = [_vp, _vp, _u64, _u64, ctypes.c_uint32, ctypes.c_float, _vp]
     L.wgfft_ragged_create.argtypes = [_vp, _vp, _vp, _u64, ctypes.POINTER(_vp)]

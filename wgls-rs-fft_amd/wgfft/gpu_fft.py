@@ -199,14 +199,16 @@ class GpuFft(FftExecutor):
         check(self._L.wgfft_add_noise_device(self._h, x.data_ptr(), n, batch, seed & 0xFFFFFFFF, sigma,
                                              self._stream(stream)))
 
-    def filter_device(self, x, s, out, stream=None) -> None:
-        """out = ifft(fft(x) * s), 1/N scaled: a (batch, n) complex64 CUDA
-        tensor filtered by the real spectral window s (n floats); out may be x."""
+    def filter_device(self, x, s, out, noise_sigma: float = 0.0, seed: int = 0, stream=None) -> None:
+        """out = ifft(fft(x + noise) * s), 1/N scaled: a (batch, n) complex64
+        CUDA tensor filtered by the real spectral window s (n floats), with the
+        pipeline's Gaussian noise of add_noise_device when noise_sigma != 0;
+        out may be x."""
         _check_tensor(x)
         _check_tensor(out)
         batch, n = x.shape
         check(self._L.wgfft_filter_device(self._h, x.data_ptr(), s.data_ptr(), out.data_ptr(), n, batch,
-                                          self._stream(stream)))
+                                          seed & 0xFFFFFFFF, noise_sigma, self._stream(stream)))
 
     def bench_device(self, x, out, inverse: bool = False, warmup: int = 3, iters: int = 50) -> float:
         batch, n = x.shape

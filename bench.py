@@ -13,7 +13,8 @@ The same JSON line also carries, measured in the same run:
     same shape and its time relative to the plain forward;
   * spectral_filter: y = IFFT(FFT(x) * s) on the same shape in one fused
     pass, against the plain forward and against the unfused forward, spectral
-    multiply and inverse (the reference pipeline's stages);
+    multiply and inverse (the reference pipeline's stages), and with the
+    pipeline's noise stage fused at the load;
   * other_shapes: the same 16.8 M samples as 256 x 65536 and 16384 x 1024;
   * mixed_batch: 20,000 transforms of lengths 2^k, k ~ U{6..16}, as one
     persistent launch, its hipGraph replay, and one launch per length;
@@ -322,8 +323,11 @@ def main() -> int:
             gpu.exec_device(y, y, inverse=True)
 
         unf_ms = timed(torch, unfused, k, max(3, args.warmup // 4))
+        noisy_ms = timed(torch, lambda: gpu.filter_device(x, sp, y, noise_sigma=0.1, seed=1), k,
+                         max(3, args.warmup // 4))
         out["spectral_filter"] = {"ms": round(filt_ms, 5), "ratio_to_fwd": round(filt_ms / kern_ms, 3),
-                                  "unfused_ms": round(unf_ms, 5), "speedup_vs_unfused": round(unf_ms / filt_ms, 3)}
+                                  "unfused_ms": round(unf_ms, 5), "speedup_vs_unfused": round(unf_ms / filt_ms, 3),
+                                  "with_noise_ms": round(noisy_ms, 5)}
         del gx
         x_keep, y_keep = x, y
         del x, y

@@ -180,7 +180,7 @@ struct FilterArgs {
     float sigma;
 };
 
-template <int N, int E, int OPT, int... Rs>
+template <int N, int E, int OPT, bool NOISE, int... Rs>
 __global__ void __launch_bounds__(N / E)
 k_filter(FilterArgs a) {
     constexpr int T = N / E;
@@ -200,7 +200,7 @@ k_filter(FilterArgs a) {
         __builtin_amdgcn_s_setprio(2);  // raised while the wave issues its global loads
         IO::load(d, a.x + tr * N, t);
         __builtin_amdgcn_s_setprio(0);
-        if (a.sigma != 0.f) {
+        if constexpr (NOISE) {  // a separate instantiation: the hash's registers cost the plain filter occupancy
 #pragma unroll
             for (int i = 0; i < E; ++i) d[i] = cadd(d[i], pipeline_noise(IO::elem(i, t), (uint32_t)tr, a.seed, a.sigma));
         }
@@ -247,14 +247,18 @@ struct AdjointKernel {
     }
     static int blocks_per_cu(bool fwd) { return fwd ? occ<true>() : occ<false>(); }
     static hipError_t filter(const FilterArgs& a) {
-        hipLaunchKernelGGL((k_filter<N, E, OPT, Rs...>), dim3((unsigned)a.grid), dim3(THREADS), 0, a.stream, a);
+        if (a.sigma != 0.f)
+            hipLaunchKernelGGL((k_filter<N, E, OPT, true, Rs...>), dim3((unsigned)a.grid), dim3(THREADS), 0, a.stream,
+                               a);
+        else
+            hipLaunchKernelGGL((k_filter<N, E, OPT, false, Rs...>), dim3((unsigned)a.grid), dim3(THREADS), 0,
+                               a.stream, a);
         return hipGetLastError();
     }
-    static int filter_blocks_per_cu() {
+    static int filter_blocks_per_cu(bool noise) {
         int blocks = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_filter<N, E, OPT, Rs...>, THREADS,
-                                                         0) != hipSuccess)
-            return 1;
+        const void* f = noise ? (const void*)k_filter<N, E, OPT, true, Rs...> : (const void*)k_filter<N, E, OPT, false, Rs...>;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, THREADS, 0) != hipSuccess) return 1;
         return blocks > 0 ? blocks : 1;
     }
 };
@@ -267,7 +271,7 @@ struct AdjointVariant {
     hipError_t (*launch)(bool, const AdjointArgs&);
     int (*blocks_per_cu)(bool);
     hipError_t (*filter)(const FilterArgs&);  // the spectral filter with the same plan
-    int (*filter_blocks_per_cu)();
+    int (*filter_blocks_per_cu)(bool noise);
 };
 
 std::vector<AdjointVariant>& adjoint_registry();

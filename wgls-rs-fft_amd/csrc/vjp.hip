@@ -369,7 +369,7 @@ extern "C" int wgfft_filter_device(wgfft_t* h, const float* d_x, const float* d_
     if (p->kind == Plan::kBatched) {
         const AdjointVariant* v = h->tuned_adj.count(p->log2n) ? h->tuned_adj[p->log2n] : find_adjoint(p->log2n);
         if (v) {
-            int grid = v->filter_blocks_per_cu() * h->num_cus;
+            int grid = v->filter_blocks_per_cu(sigma != 0.f) * h->num_cus;
             if ((uint64_t)grid > batch) grid = (int)batch;
             FilterArgs a{(const float2*)d_x, d_s, (float2*)d_y, p->tw, batch, grid, s, seed, sigma};
             if ((e = v->filter(a)) != hipSuccess) return hip_fail(e, "filter kernel launch");

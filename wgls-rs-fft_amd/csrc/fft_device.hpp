@@ -67,11 +67,12 @@ __device__ __forceinline__ float2 pipeline_noise(uint32_t t, uint32_t b, uint32_
     const uint32_t h1 = pcg_hash(t * 1234567u + b * 7654321u + seed);
     const uint32_t h2 = pcg_hash(h1 + 999983u);
     const float u1 = fmaxf((float)h1 * 2.3283064365386963e-10f, 1e-7f);
-    const float u2 = (float)h2 * 2.3283064365386963e-10f * 6.283185307179586f;
-    const float r = sqrtf(-2.0f * logf(u1)) * sigma;
-    float sn, cs;
-    sincosf(u2, &sn, &cs);
-    return make_float2(r * cs, r * sn);
+    // the angle 2 pi u2 with u2 = h2 / 2^32 in [0, 1]: the hardware sine and
+    // cosine take their argument in turns, and the hardware log2 needs no
+    // denormal path since u1 >= 1e-7 (-2 ln u1 = -2 ln 2 log2 u1)
+    const float u2 = (float)h2 * 2.3283064365386963e-10f;
+    const float r = __builtin_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1)) * sigma;
+    return make_float2(r * __builtin_amdgcn_cosf(u2), r * __builtin_amdgcn_sinf(u2));
 }
 
 // a * W_4 with W_4 = -i (forward) or +i (inverse)

@@ -17,7 +17,12 @@
  *   hipStream_t passed as void* (NULL = HIP's default null stream) and are
  *   stream-ordered: they return once the work is enqueued.
  *   A handle may be used by one thread at a time (calls are serialised by an
- *   internal mutex); distinct handles are independent.
+ *   internal mutex); distinct handles are independent.  Device calls of one
+ *   handle on different streams may run concurrently, except that those using
+ *   the handle's scratch buffers (global passes for n > 16384, strided
+ *   layouts, VJP reductions) are ordered after one another by an event.  A
+ *   mixed-length plan has one scratch area of its own: its calls on
+ *   different streams must be ordered by the caller.
  */
 #ifndef WGFFT_H
 #define WGFFT_H

@@ -489,6 +489,42 @@ def test_strided_layouts(gpu, torch_cuda, n, batch):
                                                           -1, None))
 
 
+def test_scratch_shared_across_streams(gpu, torch_cuda):
+    """Calls of one handle on two streams that both use the handle's scratch
+    (global passes at 2^20, the strided gather/scatter buffer, the VJP's
+    reductions) give the results of the same calls made one after another."""
+    torch = torch_cuda
+    n, batch = 1 << 20, 4
+    xa = torch.from_numpy(oracle.synth_uniform((batch, n), 0x5EED0910)).cuda()
+    xb = torch.from_numpy(oracle.synth_uniform((batch, n), 0x5EED0911)).cuda()
+    m, mb = 4096, 64
+    xs = torch.from_numpy(oracle.synth_uniform((mb, m), 0x5EED0912)).cuda()
+    w = torch.from_numpy(np.hanning(m).astype(np.float32)).cuda()
+    gy = torch.from_numpy(oracle.synth_uniform((mb, m), 0x5EED0913)).cuda()
+    ra, rb, rs = torch.empty_like(xa), torch.empty_like(xb), torch.empty_like(xs)
+    gx_ref, gw_ref = torch.empty_like(xs), torch.empty(m, dtype=torch.float32, device="cuda")
+    gpu.exec_device(xa, ra)
+    gpu.exec_device(xb, rb, inverse=True)
+    gpu.exec_strided_device(xs.t().contiguous().t(), rs)
+    gpu.vjp_windowed_device(xs, w, gy, gx_ref, gw_ref)
+    torch.cuda.synchronize()
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    ya, yb = torch.empty_like(xa), torch.empty_like(xb)
+    ys = torch.empty((m, mb), dtype=torch.complex64, device="cuda").t()
+    gx, gw = torch.empty_like(xs), torch.empty_like(gw_ref)
+    xst = xs.t().contiguous().t()
+    torch.cuda.synchronize()
+    for _ in range(3):
+        gpu.exec_device(xa, ya, stream=sa.cuda_stream)
+        gpu.exec_strided_device(xst, ys, stream=sb.cuda_stream)
+        gpu.exec_device(xb, yb, inverse=True, stream=sb.cuda_stream)
+        gpu.vjp_windowed_device(xs, w, gy, gx, gw, stream=sa.cuda_stream)
+    torch.cuda.synchronize()
+    assert torch.equal(ya, ra) and torch.equal(yb, rb)
+    assert torch.equal(ys, rs)
+    assert torch.equal(gx, gx_ref) and torch.equal(gw, gw_ref)
+
+
 def test_windowed_device_c5_shape(gpu, torch_cuda, parity):
     torch = torch_cuda
     n, batch = 8192, 1024

@@ -55,6 +55,26 @@ static int log2u(uint64_t n) {
     return l;
 }
 
+hipError_t Handle::scratch_begin(hipStream_t st) {
+    if (!scratch_recorded || st == scratch_stream) return hipSuccess;  // one stream orders its own work
+    return hipStreamWaitEvent(st, scratch_ev, 0);
+}
+
+void Handle::scratch_end(hipStream_t st) {
+    if (!scratch_ev && hipEventCreateWithFlags(&scratch_ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        scratch_ev = nullptr;
+        return;
+    }
+    if (hipEventRecord(scratch_ev, st) != hipSuccess) {
+        (void)hipGetLastError();
+        scratch_recorded = false;
+        return;
+    }
+    scratch_stream = st;
+    scratch_recorded = true;
+}
+
 int Handle::ensure(DevBuf& b, uint64_t bytes) {
     if (b.bytes >= bytes && b.p) return WGFFT_OK;
     uint64_t cap = g_alloc_cap.load();
@@ -365,6 +385,8 @@ int Handle::run(Plan* p, Mode mode, const float2* in, float2* out, const float* 
     const int P = (int)p->pass_var.size();
     const bool inplace = (const void*)in == (const void*)out;
     int rc;
+    ScratchUse su(this, s);
+    if (su.e != hipSuccess) return hip_fail(su.e, "stream ordering of the scratch buffers");
     if ((rc = ensure(scratch, chunk * tbytes))) return rc;
     if ((P % 2 == 1 || inplace) && P > 1) {
         if ((rc = ensure(scratch2, chunk * tbytes))) return rc;
@@ -527,6 +549,7 @@ void wgfft_destroy(wgfft_t* h) {
         h->free_pipe();
         for (DevBuf* b : {&h->io_in, &h->io_out, &h->io_w, &h->io_aux, &h->scratch, &h->scratch2, &h->red, &h->red2, &h->lossbuf})
             h->release(*b);
+        if (h->scratch_ev) hipEventDestroy(h->scratch_ev);
         hipStreamDestroy(h->stream);
     }
     delete h;
@@ -572,6 +595,8 @@ int wgfft_exec_strided_device(wgfft_t* h, const float* d_in, float* d_out, uint6
     const uint64_t total = n * batch;
     if ((rc = h->ensure(h->io_aux, total * sizeof(float2)))) return rc;
     hipStream_t s = pick_stream(h, stream);
+    ScratchUse su(h, s);
+    if (su.e != hipSuccess) return hip_fail(su.e, "stream ordering of the scratch buffers");
     const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 8192);
     float2* aux = (float2*)h->io_aux.p;
     const uint32_t log2n = (uint32_t)log2u(n);

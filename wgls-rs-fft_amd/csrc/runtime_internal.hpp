@@ -82,6 +82,15 @@ struct Handle {
     static constexpr uint64_t kHostPipeMinBytes = 8ull << 20;
     uint64_t host_chunk_bytes = 64ull << 20;
     HostPipe* pipe = nullptr;
+    // Device calls enqueued on different streams share the scratch buffers
+    // above.  Each call that uses them makes its stream wait for an event
+    // recorded after the previous such call's work, so two calls never use
+    // them at the same time on the device.
+    hipEvent_t scratch_ev = nullptr;
+    hipStream_t scratch_stream = nullptr;
+    bool scratch_recorded = false;
+    hipError_t scratch_begin(hipStream_t s);
+    void scratch_end(hipStream_t s);
 
     int ensure(DevBuf& b, uint64_t bytes);
     void release(DevBuf& b);
@@ -99,6 +108,21 @@ struct Handle {
     void free_pipe();
 };
 
+}  // namespace wgfft
+
+namespace wgfft {
+// Orders one call's use of the handle's scratch buffers after the previous
+// call's (on whatever stream that ran) and records the point after it on
+// `s` when the call returns, on success and on error paths alike.
+struct ScratchUse {
+    Handle* h;
+    hipStream_t s;
+    hipError_t e;
+    ScratchUse(Handle* h_, hipStream_t s_) : h(h_), s(s_), e(h_->scratch_begin(s_)) {}
+    ~ScratchUse() { h->scratch_end(s); }
+    ScratchUse(const ScratchUse&) = delete;
+    ScratchUse& operator=(const ScratchUse&) = delete;
+};
 }  // namespace wgfft
 
 struct wgfft_handle : wgfft::Handle {};

@@ -267,6 +267,8 @@ extern "C" int wgfft_vjp_windowed_device(wgfft_t* h, const float* d_x, const flo
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipStream_t s = (hipStream_t)stream;  // NULL: HIP default stream
+    ScratchUse su(h, s);
+    if (su.e != hipSuccess) return hip_fail(su.e, "stream ordering of the scratch buffers");
     if (batch == 0) {
         if ((e = zero_fill(d_gw, n, s)) != hipSuccess) return hip_fail(e, "zero fill of gw");
         return WGFFT_OK;
@@ -297,6 +299,8 @@ extern "C" int wgfft_fwd_adjoint_masked_device(wgfft_t* h, const float* d_x, con
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipStream_t s = (hipStream_t)stream;  // NULL: HIP default stream
+    ScratchUse su(h, s);
+    if (su.e != hipSuccess) return hip_fail(su.e, "stream ordering of the scratch buffers");
     if (batch == 0) {
         if ((e = zero_fill(d_gw, n, s)) != hipSuccess || (e = zero_fill(d_loss, 1, s)) != hipSuccess)
             return hip_fail(e, "zero fill of gw and loss");

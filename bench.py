@@ -196,6 +196,38 @@ def bench_c5(torch, gpu, wgfft, steps: int, tune: bool) -> dict:
             "fused_fwd_adjoint_ms": round(fused, 4), "fused_ratio": round(fused / fwd, 3), "plan": gpu.describe(n)}
 
 
+def bench_fused(torch, gpu, x, y, kern_ms: float, steps: int, warmup: int) -> dict:
+    """On the headline shape: the fused masked forward + adjoint and the
+    spectral filter IFFT(FFT(x) * s) (with and without the pipeline's noise at
+    its load, and as separate forward, multiply and inverse launches), as
+    ratios to the plain forward time kern_ms."""
+    import numpy as np
+
+    n = x.shape[1]
+    w = torch.from_numpy(np.hanning(n).astype(np.float32)).cuda()
+    m = (torch.rand(n, device="cuda") < 0.3).float()
+    gx = torch.empty_like(x)
+    gw = torch.empty(n, dtype=torch.float32, device="cuda")
+    loss = torch.empty(1, dtype=torch.float32, device="cuda")
+    k, kw = max(10, steps // 4), max(3, warmup // 4)
+    fa_ms = timed(torch, lambda: gpu.fwd_adjoint_masked_device(x, w, m, None, gx, gw, loss), k, kw)
+    out = {"fwd_adjoint": {"ms": round(fa_ms, 5), "plain_fwd_ms": round(kern_ms, 5), "ratio": round(fa_ms / kern_ms, 3)}}
+    sp = torch.from_numpy(np.hanning(n + 1)[:n].astype(np.float32)).cuda()
+    filt_ms = timed(torch, lambda: gpu.filter_device(x, sp, y), k, kw)
+
+    def unfused():
+        gpu.exec_device(x, y)
+        y.mul_(sp)
+        gpu.exec_device(y, y, inverse=True)
+
+    unf_ms = timed(torch, unfused, k, kw)
+    noisy_ms = timed(torch, lambda: gpu.filter_device(x, sp, y, noise_sigma=0.1, seed=1), k, kw)
+    out["spectral_filter"] = {"ms": round(filt_ms, 5), "ratio_to_fwd": round(filt_ms / kern_ms, 3),
+                              "unfused_ms": round(unf_ms, 5), "speedup_vs_unfused": round(unf_ms / filt_ms, 3),
+                              "with_noise_ms": round(noisy_ms, 5)}
+    return out
+
+
 def bench_rocfft(torch, gpu, steps: int) -> dict:
     """torch.fft.fft (hipFFT -> rocFFT on ROCm) on the bench shapes, out of
     place, against this library's default plan on the same tensors."""
@@ -240,7 +272,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--headline-only", action="store_true", help="only the N=4096x4096 forward (profiling)")
-    ap.add_argument("--only", choices=("shapes", "mixed", "large", "c1", "c5", "rocfft"),
+    ap.add_argument("--only", choices=("shapes", "mixed", "large", "c1", "c5", "rocfft", "fused"),
                     help="profiling: skip the headline and run one secondary section")
     ap.add_argument("--no-tune", action="store_true", help="use the default kernel variants")
     args = ap.parse_args()
@@ -254,6 +286,15 @@ def main() -> int:
     import wgfft
 
     gpu = wgfft.GpuFft.new()
+    if args.only == "fused":  # the headline shape's fused kernels, timed against its default forward
+        if not args.no_tune:
+            gpu.autotune(N, BATCH)
+        x = torch.from_numpy(wgfft.synth_uniform((BATCH, N), 0x5EED0002)).cuda()
+        y = torch.empty_like(x)
+        kern_ms = timed(torch, lambda: gpu.exec_device(x, y), args.steps, args.warmup)
+        print(json.dumps(bench_fused(torch, gpu, x, y, kern_ms, args.steps, args.warmup)))
+        gpu.close()
+        return 0
     if args.only:
         section = {"shapes": lambda: bench_shapes(torch, gpu, args.steps, not args.no_tune),
                    "mixed": lambda: bench_mixed(torch, gpu, wgfft, args.steps),
@@ -304,31 +345,7 @@ def main() -> int:
                        "copy_gbs": round(2 * x.numel() * 8 / (copy_ms * 1e-3) / 1e9, 1)}
 
     if not args.headline_only:
-        w = torch.from_numpy(np.hanning(N).astype(np.float32)).cuda()
-        m = (torch.rand(N, device="cuda") < 0.3).float()
-        gx = torch.empty_like(x)
-        gw = torch.empty(N, dtype=torch.float32, device="cuda")
-        loss = torch.empty(1, dtype=torch.float32, device="cuda")
-        fa_ms = timed(torch, lambda: gpu.fwd_adjoint_masked_device(x, w, m, None, gx, gw, loss),
-                      max(10, args.steps // 4), max(3, args.warmup // 4))
-        out["fwd_adjoint"] = {"ms": round(fa_ms, 5), "plain_fwd_ms": round(kern_ms, 5),
-                              "ratio": round(fa_ms / kern_ms, 3)}
-        sp = torch.from_numpy(np.hanning(N + 1)[:N].astype(np.float32)).cuda()
-        k = max(10, args.steps // 4)
-        filt_ms = timed(torch, lambda: gpu.filter_device(x, sp, y), k, max(3, args.warmup // 4))
-
-        def unfused():
-            gpu.exec_device(x, y)
-            y.mul_(sp)
-            gpu.exec_device(y, y, inverse=True)
-
-        unf_ms = timed(torch, unfused, k, max(3, args.warmup // 4))
-        noisy_ms = timed(torch, lambda: gpu.filter_device(x, sp, y, noise_sigma=0.1, seed=1), k,
-                         max(3, args.warmup // 4))
-        out["spectral_filter"] = {"ms": round(filt_ms, 5), "ratio_to_fwd": round(filt_ms / kern_ms, 3),
-                                  "unfused_ms": round(unf_ms, 5), "speedup_vs_unfused": round(unf_ms / filt_ms, 3),
-                                  "with_noise_ms": round(noisy_ms, 5)}
-        del gx
+        out.update(bench_fused(torch, gpu, x, y, kern_ms, args.steps, args.warmup))
         x_keep, y_keep = x, y
         del x, y
         out["other_shapes"] = bench_shapes(torch, gpu, args.steps, not args.no_tune)

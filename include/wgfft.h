@@ -150,7 +150,10 @@ int wgfft_add_noise_device(wgfft_t* h, float* d_x, uint64_t n, uint64_t batch, u
 
 /* Plan `count` transforms of lengths 2^log2n[i] (6 <= log2n[i] <= 16) stored
  * contiguously at complex offsets offsets[i] (host arrays; offsets even).
- * The plan uploads a device work queue; it stays valid until destroyed. */
+ * The plan uploads a device work queue; it stays valid until destroyed with
+ * wgfft_ragged_destroy or until its handle is destroyed: wgfft_destroy also
+ * frees every plan still open on the handle, after which such a plan must
+ * not be used or passed to wgfft_ragged_destroy. */
 int wgfft_ragged_create(wgfft_t* h, const uint64_t* offsets, const uint32_t* log2n, uint64_t count,
                         wgfft_ragged_t** out);
 void wgfft_ragged_destroy(wgfft_ragged_t* p);

@@ -467,9 +467,105 @@ static void *fft64_worker(void *arg) {
     return NULL;
 }
 
-/* In-place batched FP64 FFT over `threads` pthreads. */
+/* One transform split over threads (for batches smaller than the thread
+ * count): every stage of fft64_one's butterflies, each computed exactly as
+ * there, is divided between the threads (by blocks, or by k within each block
+ * when there are fewer blocks than threads), with a barrier between stages,
+ * so the result is bit-identical to the one-thread transform.  Thread t also
+ * fills its share of the twiddle table and of the bit-reversal swaps (a pair
+ * i < rev(i) is swapped only by the owner of i). */
+typedef struct {
+    double *x;
+    uint64_t n;
+    double *c, *s;
+    int sign, t, nt;
+    pthread_barrier_t *bar;
+} fft64_par_t;
+
+static void *fft64_par_worker(void *arg) {
+    fft64_par_t *p = (fft64_par_t *)arg;
+    const uint64_t n = p->n, T = (uint64_t)p->nt, t = (uint64_t)p->t;
+    double *x = p->x, *c = p->c, *s = p->s;
+    int log2n = 0;
+    while ((1ull << log2n) < n) ++log2n;
+    for (uint64_t j = (n / 2 + 1) * t / T; j < (n / 2 + 1) * (t + 1) / T; ++j) {
+        double th = 2.0 * M_PI * (double)j / (double)n;
+        c[j] = cos(th);
+        s[j] = sin(th) * (double)p->sign;
+    }
+    for (uint64_t i = n * t / T; i < n * (t + 1) / T; ++i) {
+        uint64_t r = 0;
+        for (int b = 0; b < log2n; ++b) r |= ((i >> b) & 1ull) << (log2n - 1 - b);
+        if (i < r) {
+            double tr = x[2 * i], ti = x[2 * i + 1];
+            x[2 * i] = x[2 * r]; x[2 * i + 1] = x[2 * r + 1];
+            x[2 * r] = tr; x[2 * r + 1] = ti;
+        }
+    }
+    pthread_barrier_wait(p->bar);
+    for (uint64_t len = 2; len <= n; len <<= 1) {
+        const uint64_t half = len >> 1, step = n / len, blocks = n / len;
+        uint64_t b0 = 0, b1 = blocks, k0 = 0, k1 = half;
+        if (blocks >= T) {
+            b0 = blocks * t / T;
+            b1 = blocks * (t + 1) / T;
+        } else {
+            k0 = half * t / T;
+            k1 = half * (t + 1) / T;
+        }
+        for (uint64_t blk = b0; blk < b1; ++blk) {
+            const uint64_t i = blk * len;
+            for (uint64_t k = k0; k < k1; ++k) {
+                double wr = c[k * step], wi = s[k * step];
+                double *u = x + 2 * (i + k), *v = x + 2 * (i + k + half);
+                double tr = v[0] * wr - v[1] * wi;
+                double ti = v[0] * wi + v[1] * wr;
+                v[0] = u[0] - tr; v[1] = u[1] - ti;
+                u[0] += tr; u[1] += ti;
+            }
+        }
+        pthread_barrier_wait(p->bar);
+    }
+    return NULL;
+}
+
+static int fft64_one_parallel(double *x, uint64_t n, int sign, int threads) {
+    double *c = (double *)malloc(sizeof(double) * (n / 2 + 1));
+    double *s = (double *)malloc(sizeof(double) * (n / 2 + 1));
+    fft64_par_t *jobs = (fft64_par_t *)calloc((size_t)threads, sizeof(fft64_par_t));
+    pthread_t *th = (pthread_t *)calloc((size_t)threads, sizeof(pthread_t));
+    pthread_barrier_t bar;
+    if (!c || !s || !jobs || !th || pthread_barrier_init(&bar, NULL, (unsigned)threads) != 0) {
+        free(c); free(s); free(jobs); free(th);
+        return -2;
+    }
+    for (int t = 0; t < threads; ++t) {
+        jobs[t].x = x; jobs[t].n = n; jobs[t].c = c; jobs[t].s = s; jobs[t].sign = sign;
+        jobs[t].t = t; jobs[t].nt = threads; jobs[t].bar = &bar;
+    }
+    for (int t = 1; t < threads; ++t) pthread_create(&th[t], NULL, fft64_par_worker, &jobs[t]);
+    fft64_par_worker(&jobs[0]);
+    for (int t = 1; t < threads; ++t) pthread_join(th[t], NULL);
+    pthread_barrier_destroy(&bar);
+    free(th);
+    free(jobs);
+    free(c);
+    free(s);
+    return 0;
+}
+
+/* In-place batched FP64 FFT over `threads` pthreads: whole transforms per
+ * thread when the batch has at least as many transforms as threads (or the
+ * transforms are short), else each transform split over the threads. */
 int oracle_fft64_inplace(double *x, uint64_t n, uint64_t batch, int sign, int threads) {
     if (n == 0 || (n & (n - 1)) != 0) return -1;
+    if (threads > 1 && batch < (uint64_t)threads && n >= (1ull << 16)) {
+        for (uint64_t b = 0; b < batch; ++b) {
+            int rc = fft64_one_parallel(x + 2 * n * b, n, sign, threads);
+            if (rc) return rc;
+        }
+        return 0;
+    }
     double *c = (double *)malloc(sizeof(double) * (n / 2 + 1));
     double *s = (double *)malloc(sizeof(double) * (n / 2 + 1));
     for (uint64_t j = 0; j <= n / 2; ++j) {

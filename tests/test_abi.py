@@ -124,3 +124,50 @@ def test_python_mirror_validates_before_device_work():
         _flatten([np.zeros(4, np.complex64), np.zeros(8, np.complex64)])
     with pytest.raises(wgfft.WgfftError):
         _flatten([np.zeros(0, np.complex64)])
+
+
+def _unbound_executor():
+    """A GpuFft whose handle was never created: its device wrappers must
+    reject bad tensors before they reach the C-ABI (which would get NULL)."""
+    g = wgfft.GpuFft.__new__(wgfft.GpuFft)
+    g._h = None
+    g._L = _lib.load()
+    g._plans = []
+    return g
+
+
+def test_device_wrappers_reject_host_tensors():
+    import torch
+
+    g = _unbound_executor()
+    x = torch.zeros((2, 64), dtype=torch.complex64)  # host memory: never a kernel pointer
+    w = torch.ones(64, dtype=torch.float32)
+    cases = [
+        lambda: g.exec_device(x, x),
+        lambda: g.exec_windowed_device(x, w, x),
+        lambda: g.vjp_windowed_device(x, w, x, x, w),
+        lambda: g.fwd_adjoint_masked_device(x, w, w, None, x, w, torch.zeros(1)),
+        lambda: g.filter_device(x, w, x),
+        lambda: g.add_noise_device(x),
+        lambda: g.exec_strided_device(x, x),
+        lambda: g.bench_device(x, x),
+    ]
+    for i, f in enumerate(cases):
+        with pytest.raises(wgfft.WgfftError) as e:
+            f()
+        assert e.value.code == wgfft.WGFFT_ERR_INVALID_ARGUMENT, i
+        assert "GPU" in str(e.value), (i, str(e.value))
+
+
+def test_real_operand_checks():
+    import torch
+    from wgfft.gpu_fft import _check_real, _check_tensor
+
+    with pytest.raises(wgfft.WgfftError) as e:
+        _check_real(torch.ones(63, dtype=torch.float32), 64, "window")
+    assert "window" in str(e.value) and "64" in str(e.value)
+    for bad in (torch.ones(64, dtype=torch.float16), torch.ones(128)[::2], [1.0] * 64, None):
+        with pytest.raises(wgfft.WgfftError):
+            _check_real(bad, 64, "window")
+    with pytest.raises(wgfft.WgfftError):
+        _check_tensor(np.zeros((2, 64), np.complex64))

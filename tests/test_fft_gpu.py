@@ -21,16 +21,24 @@ def rel_bar(n):
     return 5e-6 if n <= (1 << 20) else 2e-5
 
 
-def check_fwd_inv(gpu, x):
+def fp64_refs(x):
+    """FP64 forward and 1/N-scaled inverse of x (the oracle's radix-2 FFT)."""
+    n = x.shape[-1]
+    return oracle.fft64(x, -1), oracle.fft64(x, +1) / n
+
+
+def check_fwd_inv(gpu, x, refs=None):
+    """Whole-vector forward and inverse of the batch against FP64; returns the
+    forward output and the two errors."""
     batch, n = x.shape
+    want_f, want_i = refs if refs is not None else fp64_refs(x)
     y = gpu.fft_array(x)
-    want = oracle.fft64(x, -1)
-    e_f = oracle.rel_l2(y, want)
+    e_f = oracle.rel_l2(y, want_f)
     assert e_f <= rel_bar(n), (n, batch, e_f)
     yi = gpu.fft_array(x, inverse=True)
-    e_i = oracle.rel_l2(yi, oracle.fft64(x, +1) / n)
+    e_i = oracle.rel_l2(yi, want_i)
     assert e_i <= rel_bar(n), (n, batch, e_i)
-    return y
+    return y, e_f, e_i
 
 
 # ----------------------------------------------------------- golden vectors
@@ -49,7 +57,7 @@ def test_all_lengths_vs_fp64_and_reference(gpu, log2n):
     n = 1 << log2n
     batch = max(1, min(37, (1 << 19) // n))  # odd batch exercises the group tail
     x = oracle.synth_uniform((batch, n), 0x5EED0100 + log2n)
-    y = check_fwd_inv(gpu, x)
+    y, _, _ = check_fwd_inv(gpu, x)
     ref = oracle.ref_fft_f32(x)
     assert np.abs(y - ref).max() < 1e-3  # reference restatement, README bar
     single = gpu.fft_array(x[-1:])
@@ -103,19 +111,21 @@ def test_multipass_lengths(gpu, log2n, batch):
     check_fwd_inv(gpu, x)
 
 
-@pytest.mark.parametrize("log2n,batch", [(17, 3), (20, 2), (22, 2), (23, 1)])
+@pytest.mark.parametrize("log2n,batch", [(17, 3), (20, 2), (22, 2), (23, 1), (24, 1), (25, 1)])
 def test_every_global_pass_variant(log2n, batch):
     """Every compiled candidate of every global pass (LDS-staged and direct
     I/O, with and without the register prefetch), pinned in turn with the
-    other passes on their defaults, matches FP64 forward and inverse."""
+    other passes on their defaults, matches FP64 forward and inverse (at
+    2^24 and 2^25 the three passes of the default plans)."""
     n = 1 << log2n
     x = oracle.synth_uniform((batch, n), 0x5EED0180 + log2n)
+    refs = fp64_refs(x)
     for p in range(wgfft.GpuFft.pass_count(n)):
         for i in range(wgfft.GpuFft.pass_variant_count(n, p)):
             g = wgfft.GpuFft.new()
             try:
                 desc = g.select_pass_variant(n, p, i)
-                check_fwd_inv(g, x)
+                check_fwd_inv(g, x, refs)
             except AssertionError as e:
                 raise AssertionError(f"pass {p} variant {i}: {desc}: {e}") from None
             finally:
@@ -377,37 +387,58 @@ def test_c3_large_n_2pow22(gpu, parity):
         g.close()
 
 
+@pytest.mark.parametrize("log2n", [24, 25])
+def test_three_pass_plans_vs_fp64(parity, log2n):
+    """The default three-pass plans of 2^24 and 2^25 and the plans the
+    autotuner keeps there, forward and inverse, as whole vectors against the
+    FP64 FFT on the large-N workload's signal (32 tones + noise)."""
+    n = 1 << log2n
+    x = oracle.synth_sinusoids(n, 1, 0x5EED0003)
+    refs = fp64_refs(x)
+    for tuned in (False, True):
+        g = wgfft.GpuFft.new()
+        try:
+            plan = g.autotune(n, 1) if tuned else g.describe(n)
+            assert plan.startswith("multipass"), plan
+            if not tuned:
+                assert wgfft.GpuFft.pass_count(n) == 3, plan
+            _, e_f, e_i = check_fwd_inv(g, x, refs)
+            parity[f"2^{log2n}x1 {'autotuned' if tuned else 'default'}"] = {
+                "fwd_rel_l2_vs_fp64": e_f, "inv_rel_l2_vs_fp64": e_i, "bar": 2e-5, "plan": plan}
+        finally:
+            g.close()
+
+
 def test_c3_large_n_2pow26(gpu, parity):
+    """N = 2^26, batch 1, on the workload's signal (32 random-frequency,
+    random-phase unit tones + Gaussian noise sigma 0.1): the whole forward and
+    inverse against the FP64 FFT, for the default three-pass plan and the plan
+    the autotuner keeps; plus bins against the FP64 direct DFT, Parseval and
+    the round trip."""
     n = 1 << 26
-    x = wgfft.synth_uniform((1, n), 0x5EED0013)
-    y = gpu.fft_array(x)
+    x = oracle.synth_sinusoids(n, 1, 0x5EED0003)
+    want_f, want_i = fp64_refs(x)
     bins = np.array([0, 1, 12345, n // 2, n - 1, 33554431 - 777], dtype=np.uint64)
-    want = oracle.dft_bins64(x[0], bins)
-    scale = np.sqrt(np.sum(np.abs(x.astype(np.complex128)) ** 2))  # |Y_k| ~ ||x||
-    e_bins = float(np.abs(y[0, bins.astype(np.int64)] - want).max() / scale)
-    assert e_bins < 2e-5
-    # Parseval and round trip over the whole transform
-    py = np.sum(np.abs(y.astype(np.complex128)) ** 2)
-    e_pars = float(abs(py / (n * scale**2) - 1))
-    assert e_pars < 1e-5
-    r = gpu.fft_array(y, inverse=True)
-    e_rt = oracle.rel_l2(r, x)
-    parity["C3 2^26x1"] = {"bins_max_err_over_norm_vs_fp64_dft": e_bins, "parseval_rel_err": e_pars,
-                           "round_trip_rel_l2": e_rt, "bar": 2e-5, "plan": gpu.describe(n)}
-    assert e_rt <= 2e-5
-    # the plan the bench times (autotuned radix split and pass variants)
-    g = wgfft.GpuFft.new()
-    try:
-        plan = g.autotune(n, 1)
-        y2 = g.fft_array(x)
-        e2_bins = float(np.abs(y2[0, bins.astype(np.int64)] - want).max() / scale)
-        e2_rt = oracle.rel_l2(g.fft_array(y2, inverse=True), x)
-        e2_vs_default = oracle.rel_l2(y2, y)
-        parity["C3 2^26x1 autotuned"] = {"bins_max_err_over_norm_vs_fp64_dft": e2_bins, "round_trip_rel_l2": e2_rt,
-                                         "rel_l2_vs_default_plan": e2_vs_default, "bar": 2e-5, "plan": plan}
-        assert e2_bins < 2e-5 and e2_rt <= 2e-5 and e2_vs_default < 2e-5
-    finally:
-        g.close()
+    want_bins = oracle.dft_bins64(x[0], bins)
+    scale = np.sqrt(np.sum(np.abs(x.astype(np.complex128)) ** 2))  # ||x||, the RMS size of a bin |Y_k|
+    assert np.abs(want_f[0, bins.astype(np.int64)] - want_bins).max() / scale < 1e-9  # the two FP64 oracles agree
+    for tuned in (False, True):
+        g = wgfft.GpuFft.new()
+        try:
+            plan = g.autotune(n, 1) if tuned else g.describe(n)
+            y, e_f, e_i = check_fwd_inv(g, x, (want_f, want_i))
+            e_bins = float(np.abs(y[0, bins.astype(np.int64)] - want_bins).max() / scale)
+            assert e_bins < 2e-5
+            py = np.sum(np.abs(y.astype(np.complex128)) ** 2)
+            e_pars = float(abs(py / (n * scale**2) - 1))
+            assert e_pars < 1e-5
+            e_rt = oracle.rel_l2(g.fft_array(y, inverse=True), x)
+            assert e_rt <= 2e-5
+            parity["C3 2^26x1" + (" autotuned" if tuned else "")] = {
+                "fwd_rel_l2_vs_fp64": e_f, "inv_rel_l2_vs_fp64": e_i, "bins_max_err_over_norm_vs_fp64_dft": e_bins,
+                "parseval_rel_err": e_pars, "round_trip_rel_l2": e_rt, "bar": 2e-5, "plan": plan}
+        finally:
+            g.close()
 
 
 # ------------------------------------------------------- device entry points
@@ -537,3 +568,31 @@ def test_windowed_device_c5_shape(gpu, torch_cuda, parity):
     err = oracle.rel_l2(ty.cpu().numpy(), oracle.windowed_fwd64(x, w))
     parity["C5 8192x1024 windowed forward"] = {"rel_l2_vs_fp64": err, "bar": 5e-6}
     assert err <= 5e-6
+
+
+def test_device_wrappers_reject_bad_operands(gpu, torch_cuda):
+    """Short, wrongly typed or host-resident windows, masks and gradients, and
+    outputs of another shape, are rejected before any kernel sees them."""
+    torch = torch_cuda
+    n, batch = 256, 2
+    x = torch.zeros((batch, n), dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    w = torch.ones(n, dtype=torch.float32, device="cuda")
+    bad_w = (torch.ones(n - 1, device="cuda"), torch.ones(n, dtype=torch.float16, device="cuda"), torch.ones(n),
+             torch.ones(2 * n, device="cuda")[::2])
+    for bw in bad_w:
+        with pytest.raises(wgfft.WgfftError):
+            gpu.exec_windowed_device(x, bw, y)
+        with pytest.raises(wgfft.WgfftError):
+            gpu.filter_device(x, bw, y)
+        with pytest.raises(wgfft.WgfftError):
+            gpu.vjp_windowed_device(x, bw, x, y, w)
+    loss = torch.zeros(1, device="cuda")
+    with pytest.raises(wgfft.WgfftError):
+        gpu.fwd_adjoint_masked_device(x, w, torch.ones(n - 1, device="cuda"), None, y, w.clone(), loss)
+    with pytest.raises(wgfft.WgfftError):
+        gpu.vjp_windowed_device(x, w, x[:1], y, w.clone())  # gy of another shape
+    with pytest.raises(wgfft.WgfftError):
+        gpu.exec_device(x, torch.empty((batch, n // 2), dtype=torch.complex64, device="cuda"))
+    gpu.exec_windowed_device(x, w, y)  # the well-formed call still runs
+    torch.cuda.synchronize()

@@ -290,3 +290,60 @@ def test_adjoint_identity():
     lhs = np.vdot(oracle.fft64(u, -1), v)
     rhs = np.vdot(u, oracle.fft64(v, +1))
     assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+# ------------------------------------------------ selected FP64 DFT bins
+@pytest.mark.parametrize("n", [1, 2, 8, 64, 1024, 4096])
+def test_dft_bins_match_pocketfft_golden(n):
+    # every bin of the golden signals: direct sums against pocketfft FP64
+    g = _golden()
+    x = g[f"x_{n}"]
+    bins = np.arange(n, dtype=np.uint64)
+    got = oracle.dft_bins64(x[0], bins)
+    want = g[f"fwd_{n}"][0]
+    assert oracle.rel_l2(got, want) < 1e-12
+    if n >= 8:
+        # planted mistakes: the inverse sign, and the neighbouring bin
+        wrong_sign = np.conj(oracle.dft_bins64(np.conj(x[0]), bins))  # sum x exp(+2 pi i k j / N)
+        assert oracle.rel_l2(wrong_sign, want) > 0.5
+        assert oracle.rel_l2(oracle.dft_bins64(x[0], (bins + 1) % n), want) > 0.5
+
+
+def test_dft_bins_match_fft64_at_2pow15():
+    # random bins of a 2^15 signal against the FP64 radix-2 FFT (itself pinned
+    # against the golden vectors above), and the index wrap k mod N
+    n = 1 << 15
+    x = oracle.synth_uniform((1, n), 0x5EED0410)
+    y = oracle.fft64(x)[0]
+    rng = np.random.default_rng(12)
+    bins = np.unique(np.concatenate([[0, 1, n // 2, n - 1], rng.integers(0, n, 28)])).astype(np.uint64)
+    got = oracle.dft_bins64(x[0], bins)
+    assert np.abs(got - y[bins.astype(np.int64)]).max() < 1e-9 * np.sqrt(n)
+    assert np.array_equal(oracle.dft_bins64(x[0], bins + np.uint64(n)), got)
+
+
+def test_dft_bins_closed_forms_at_large_n():
+    # a complex exponential at bin k0 and a shifted impulse, at N = 2^20
+    n = 1 << 20
+    k0 = 123457
+    j = np.arange(n, dtype=np.float64)
+    e = np.exp(2j * np.pi * ((k0 * j) % n) / n).astype(np.complex64)
+    got = oracle.dft_bins64(e, np.array([k0, k0 + 1, 0], dtype=np.uint64))
+    assert abs(got[0] - n) < 1e-3 * n ** 0.5 and abs(got[1]) < 1e-2 * n ** 0.5 and abs(got[2]) < 1e-2 * n ** 0.5
+    imp = np.zeros(n, np.complex64)
+    imp[777] = 1
+    ks = np.array([0, 1, 5000, n - 3], dtype=np.uint64)
+    want = np.exp(-2j * np.pi * ((ks.astype(np.float64) * 777) % n) / n)
+    assert np.abs(oracle.dft_bins64(imp, ks) - want).max() < 1e-12
+
+
+def test_fft64_threaded_single_transform_bit_identical():
+    # a transform split over threads computes every butterfly as the
+    # one-thread transform does
+    for n in (1 << 16, 1 << 18):
+        x = oracle.synth_uniform((2, n), 0x5EED0420)
+        one = oracle.fft64(x, threads=1)
+        for t in (3, 8):
+            assert np.array_equal(oracle.fft64(x, threads=t), one)
+            assert np.array_equal(oracle.fft64(x, +1, threads=t), oracle.fft64(x, +1, threads=1))
+    assert oracle.rel_l2(one, np.fft.fft(x.astype(np.complex128), axis=-1)) < 1e-14

@@ -116,3 +116,40 @@ def test_ragged_rejects_bad_tables(gpu):
         wgfft.RaggedPlan(gpu, [0, 32], [6, 6])  # overlap
     with pytest.raises(wgfft.WgfftError):
         wgfft.RaggedPlan(gpu, [1], [6])  # misaligned
+
+
+def test_plan_lifetime_follows_handle(torch_cuda):
+    """Closing the GpuFft frees its open mixed-length plans (wgfft_destroy),
+    after which the Python plan is closed too: no use after free whichever is
+    closed first."""
+    torch = torch_cuda
+    k = np.array([6, 10, 14], dtype=np.uint32)
+    offsets, total = wgfft.ragged_layout(k)
+    g = wgfft.GpuFft.new()
+    p1 = wgfft.RaggedPlan(g, offsets, k)
+    p2 = wgfft.RaggedPlan(g, offsets, k)
+    tx = torch.from_numpy(oracle.synth_uniform((total,), 0x5EED0305)).cuda()
+    ty = torch.empty_like(tx)
+    p1.exec_device(tx, ty)
+    p2.close()  # plan first, then the handle
+    g.close()  # frees p1 with the handle
+    p1.close()  # no-op
+    with pytest.raises(wgfft.WgfftError):
+        p1.exec_device(tx, ty)
+    with pytest.raises(wgfft.WgfftError):
+        wgfft.RaggedPlan(g, offsets, k)
+    torch.cuda.synchronize()
+
+
+def test_plan_rejects_short_buffers(gpu, torch_cuda):
+    torch = torch_cuda
+    k = np.array([8, 12], dtype=np.uint32)
+    offsets, total = wgfft.ragged_layout(k)
+    plan = wgfft.RaggedPlan(gpu, offsets, k)
+    ok = torch.zeros(total, dtype=torch.complex64, device="cuda")
+    short = torch.zeros(total - 2, dtype=torch.complex64, device="cuda")
+    with pytest.raises(wgfft.WgfftError):
+        plan.exec_device(short, ok)
+    with pytest.raises(wgfft.WgfftError):
+        plan.exec_device(ok, short)
+    plan.close()

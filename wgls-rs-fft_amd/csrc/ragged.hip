@@ -338,6 +338,14 @@ static void ragged_free(wgfft_ragged_t* p) {
     delete p;
 }
 
+namespace wgfft {
+void ragged_destroy_all(Handle* h) {
+    if (!h->ragged_plans.empty()) hipDeviceSynchronize();
+    for (wgfft_ragged_t* p : h->ragged_plans) ragged_free(p);
+    h->ragged_plans.clear();
+}
+}  // namespace wgfft
+
 extern "C" int wgfft_ragged_create(wgfft_t* h, const uint64_t* offsets, const uint32_t* log2n, uint64_t count,
                                    wgfft_ragged_t** out) {
     if (out) *out = nullptr;
@@ -432,6 +440,7 @@ extern "C" int wgfft_ragged_create(wgfft_t* h, const uint64_t* offsets, const ui
             return hip_fail(e, "ragged plan upload");
         }
     }
+    h->ragged_plans.insert(p);
     *out = p;
     return WGFFT_OK;
 }
@@ -441,6 +450,7 @@ extern "C" void wgfft_ragged_destroy(wgfft_ragged_t* p) {
     std::lock_guard<std::mutex> lk(p->h->mu);
     hipSetDevice(p->h->device);
     hipDeviceSynchronize();
+    p->h->ragged_plans.erase(p);
     ragged_free(p);
 }
 

@@ -544,6 +544,7 @@ void wgfft_destroy(wgfft_t* h) {
         std::lock_guard<std::mutex> lk(h->mu);
         hipSetDevice(h->device);
         hipStreamSynchronize(h->stream);
+        ragged_destroy_all(h);
         for (auto& t : h->tables) hipFree(t.p);
         h->tables.clear();
         h->free_pipe();

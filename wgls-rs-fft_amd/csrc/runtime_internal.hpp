@@ -7,10 +7,12 @@
 #include <functional>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
 #include <utility>
 #include <vector>
 
+#include "../../include/wgfft.h"
 #include "kernels.hpp"
 
 namespace wgfft {
@@ -46,6 +48,9 @@ struct Plan {
 
 struct Handle;
 struct HostPipe;
+// Free every mixed-length plan still open on h (called by wgfft_destroy with
+// h->mu held).
+void ragged_destroy_all(Handle* h);
 struct AdjointVariant;
 void load_tune_file(Handle* h);
 const AdjointVariant* find_adjoint(int log2n);
@@ -82,6 +87,9 @@ struct Handle {
     static constexpr uint64_t kHostPipeMinBytes = 8ull << 20;
     uint64_t host_chunk_bytes = 64ull << 20;
     HostPipe* pipe = nullptr;
+    // mixed-length plans created on this handle and not yet destroyed; they
+    // use its device and lock, so wgfft_destroy frees them with it
+    std::set<wgfft_ragged_t*> ragged_plans;
     // Device calls enqueued on different streams share the scratch buffers
     // above.  Each call that uses them makes its stream wait for an event
     // recorded after the previous such call's work, so two calls never use

@@ -90,12 +90,16 @@ struct Timer {
         if (a) hipEventDestroy(a);
         if (b) hipEventDestroy(b);
     }
+    // launch() returns a WGFFT status; a burst stops at the first failure
+    // and returns it, so a candidate that did not run is never timed
     template <class F>
     int burst(F&& launch, int iters, float* ms) {
         hipEventRecord(a, s);
-        for (int i = 0; i < iters; ++i) launch();
+        int rc = WGFFT_OK;
+        for (int i = 0; i < iters && rc == WGFFT_OK; ++i) rc = launch();
         hipEventRecord(b, s);
         const hipError_t e = hipEventSynchronize(b);
+        if (rc != WGFFT_OK) return rc;
         if (e != hipSuccess) return hip_fail(e, "autotune timing");
         hipEventElapsedTime(ms, a, b);
         return WGFFT_OK;
@@ -169,7 +173,9 @@ static std::string describe_multipass(const Plan* p) {
 
 // Time every radix split the planner offers (multipass_splits) and, for
 // each, every combination of global-pass variants (a few per pass) on a
-// scratch batch; keep the fastest split and variants.
+// scratch batch; keep the fastest split and variants.  A combination whose
+// launch fails is skipped; when none runs the plan keeps its previous split
+// and the first failure is returned.
 static int tune_multipass(Handle* h, Plan* p, uint64_t batch, char* desc, uint64_t desc_len) {
     const uint64_t bytes = p->n * batch * sizeof(float2);
     int rc;
@@ -177,12 +183,16 @@ static int tune_multipass(Handle* h, Plan* p, uint64_t batch, char* desc, uint64
     if ((rc = fill_noise(h, h->io_in.p, bytes))) return rc;
     Timer tm(h->stream);
     auto launch = [&] {
-        h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream);
+        return h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream);
     };
+    const std::vector<int> prev_split = p->pass_log2r;
+    const std::vector<const GlobalPassVariant*> prev_vars = p->pass_var;
     if ((rc = tm.settle(launch))) return rc;
     std::vector<int> best_split = p->pass_log2r;
     std::vector<const GlobalPassVariant*> best_vars = p->pass_var;
     float best_ms = 1e30f;
+    int first_err = WGFFT_OK;
+    std::string first_msg;
     for (const std::vector<int>& split : multipass_splits(p->log2n)) {
         if ((rc = h->set_split(p, split))) return rc;
         std::vector<std::vector<const GlobalPassVariant*>> cands;
@@ -195,8 +205,12 @@ static int tune_multipass(Handle* h, Plan* p, uint64_t batch, char* desc, uint64
         for (;;) {
             for (size_t i = 0; i < cands.size(); ++i) p->pass_var[i] = cands[i][idx[i]];
             float ms;
-            if ((rc = tm.time(launch, &ms))) return rc;
-            if (ms < best_ms) {
+            if ((rc = tm.time(launch, &ms))) {
+                if (first_err == WGFFT_OK) {
+                    first_err = rc;
+                    first_msg = g_last_error;
+                }
+            } else if (ms < best_ms) {
                 best_ms = ms;
                 best_split = split;
                 best_vars = p->pass_var;
@@ -205,6 +219,11 @@ static int tune_multipass(Handle* h, Plan* p, uint64_t batch, char* desc, uint64
             while (k < idx.size() && ++idx[k] == cands[k].size()) idx[k++] = 0;
             if (k == idx.size()) break;
         }
+    }
+    if (best_ms >= 1e30f) {
+        if ((rc = h->set_split(p, prev_split))) return rc;
+        p->pass_var = prev_vars;
+        return fail(first_err, "autotune: no candidate ran (" + first_msg + ")");
     }
     if ((rc = h->set_split(p, best_split))) return rc;
     p->pass_var = best_vars;
@@ -246,6 +265,10 @@ extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc
     int rc;
     if ((rc = h->get_plan(n, &p))) return rc;
     const uint64_t bytes = n * batch * sizeof(float2);
+    // the sweeps write the handle's scratch buffers: ordered after any device
+    // call still using them on another stream
+    ScratchUse su(h, h->stream);
+    if (su.e != hipSuccess) return hip_fail(su.e, "stream ordering of the scratch buffers");
     if (p->kind == Plan::kMultiPass) return tune_multipass(h, p, batch, desc, desc_len);
     if (p->kind != Plan::kBatched) {
         if (desc && desc_len) std::snprintf(desc, desc_len, "no alternatives for N=%llu", (unsigned long long)n);
@@ -258,19 +281,29 @@ extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc
     float best_ms = 1e30f;
     const BatchedVariant* cur = p->var;
     auto launch = [&] {
-        h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream, cur);
+        return h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream,
+                      cur);
     };
     if ((rc = tm.settle(launch))) return rc;
+    int first_err = WGFFT_OK;
+    std::string first_msg;
     for (const auto& v : batched_registry()) {
         if (v.log2n != p->log2n) continue;
         cur = &v;
         float ms;
-        if ((rc = tm.time(launch, &ms))) return rc;
+        if ((rc = tm.time(launch, &ms))) {  // a candidate that fails to launch is never chosen
+            if (first_err == WGFFT_OK) {
+                first_err = rc;
+                first_msg = g_last_error;
+            }
+            continue;
+        }
         if (ms < best_ms) {
             best_ms = ms;
             best = &v;
         }
     }
+    if (!best) return fail(first_err ? first_err : WGFFT_ERR_INVALID_LENGTH, "autotune: no variant ran (" + first_msg + ")");
     // fused forward + adjoint variants for this length
     std::string adj_desc;
     const auto adjs = adjoint_variants(p->log2n);
@@ -285,23 +318,23 @@ extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc
         float best_adj_ms = 1e30f;
         const AdjointVariant* av = adjs[0];
         auto alaunch = [&] {
-            fused_adjoint(h, av, true, (const float*)h->io_in.p, w, m, nullptr, nullptr, (float*)h->io_aux.p, gw, loss,
-                          n, batch, p->tw, h->stream);
+            return fused_adjoint(h, av, true, (const float*)h->io_in.p, w, m, nullptr, nullptr, (float*)h->io_aux.p, gw,
+                                 loss, n, batch, p->tw, h->stream);
         };
         // two interleaved sweeps, each variant keeping its faster time: one
         // sweep's order and clock drift picked a variant 8% slower than the
-        // default now and then
+        // default now and then; a variant that fails to launch keeps 1e30
         std::vector<float> adj_ms(adjs.size(), 1e30f);
         for (int sweep = 0; sweep < 2; ++sweep)
             for (size_t i = 0; i < adjs.size(); ++i) {
                 av = adjs[sweep ? adjs.size() - 1 - i : i];
                 float ms;
-                if ((rc = tm.time(alaunch, &ms))) return rc;
+                if (tm.time(alaunch, &ms) != WGFFT_OK) continue;
                 const size_t j = sweep ? adjs.size() - 1 - i : i;
                 adj_ms[j] = std::min(adj_ms[j], ms);
             }
         for (size_t j = 0; j < adjs.size(); ++j)
-            if (adj_ms[j] < best_adj_ms) {
+            if (adj_ms[j] < 1e30f && adj_ms[j] < best_adj_ms) {
                 best_adj_ms = adj_ms[j];
                 best_adj = adjs[j];
             }

@@ -19,6 +19,7 @@ Every transform runs in the HIP kernels of libwgfft.so through the C-ABI.
 from __future__ import annotations
 
 import ctypes
+import weakref
 from typing import Iterable, List, Sequence
 
 import numpy as np
@@ -76,6 +77,7 @@ class GpuFft(FftExecutor):
         check(L.wgfft_create(ctypes.byref(h), int(device)))
         self._h = h
         self._L = L
+        self._plans = []  # RaggedPlans made on this handle (weak references)
 
     @classmethod
     def new(cls, device: int = 0) -> "GpuFft":
@@ -84,6 +86,13 @@ class GpuFft(FftExecutor):
     is_gpu_available = staticmethod(is_gpu_available)
 
     def close(self) -> None:
+        """Destroy the handle; wgfft_destroy also frees every mixed-length plan
+        still open on it, so those RaggedPlans are marked closed first."""
+        for ref in getattr(self, "_plans", ()):
+            plan = ref()
+            if plan is not None:
+                plan._p = None
+        self._plans = []
         if getattr(self, "_h", None):
             self._L.wgfft_destroy(self._h)
             self._h = None
@@ -153,25 +162,43 @@ class GpuFft(FftExecutor):
     def exec_device(self, x, out, inverse: bool = False, stream=None) -> None:
         """Transform a (batch, n) complex64 CUDA tensor into `out`."""
         _check_tensor(x)
-        _check_tensor(out)
+        _check_tensor(out, x.shape)
         batch, n = x.shape
         check(self._L.wgfft_exec_device(self._h, x.data_ptr(), out.data_ptr(), n, batch,
                                         WGFFT_INVERSE if inverse else WGFFT_FORWARD, self._stream(stream)))
 
     def exec_windowed_device(self, x, w, out, stream=None) -> None:
+        """out = FFT(w * x) per row: x, out (batch, n) complex64, w n floats."""
         _check_tensor(x)
-        _check_tensor(out)
+        _check_tensor(out, x.shape)
         batch, n = x.shape
+        _check_real(w, n, "window")
         check(self._L.wgfft_exec_windowed_device(self._h, x.data_ptr(), w.data_ptr(), out.data_ptr(), n, batch,
                                                  self._stream(stream)))
 
     def vjp_windowed_device(self, x, w, gy, gx, gw, stream=None) -> None:
+        """gx = w * F^H gy and gw = sum over rows of Re(conj(F^H gy) x)."""
+        _check_tensor(x)
+        _check_tensor(gy, x.shape)
+        _check_tensor(gx, x.shape)
         batch, n = x.shape
+        _check_real(w, n, "window")
+        _check_real(gw, n, "gw")
         check(self._L.wgfft_vjp_windowed_device(self._h, x.data_ptr(), w.data_ptr(), gy.data_ptr(),
                                                 gx.data_ptr(), gw.data_ptr(), n, batch, self._stream(stream)))
 
     def fwd_adjoint_masked_device(self, x, w, mask, y, gx, gw, loss, stream=None) -> None:
+        """Fused forward + adjoint of L = sum m |FFT(w x)|^2: writes gx, gw,
+        the loss (one float) and, when y is a tensor, Y = FFT(w x)."""
+        _check_tensor(x)
+        _check_tensor(gx, x.shape)
+        if y is not None:
+            _check_tensor(y, x.shape)
         batch, n = x.shape
+        _check_real(w, n, "window")
+        _check_real(mask, n, "mask")
+        _check_real(gw, n, "gw")
+        _check_real(loss, 1, "loss")
         check(self._L.wgfft_fwd_adjoint_masked_device(
             self._h, x.data_ptr(), w.data_ptr(), mask.data_ptr(), 0 if y is None else y.data_ptr(),
             gx.data_ptr(), gw.data_ptr(), loss.data_ptr(), n, batch, self._stream(stream)))
@@ -181,7 +208,7 @@ class GpuFft(FftExecutor):
         any element strides (a slice, a transpose) into `out`, another such
         view; the tensors' own strides give the layouts."""
         if x.dtype != out.dtype or str(x.dtype) != "torch.complex64" or not x.is_cuda or not out.is_cuda:
-            raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT, "expected complex64 CUDA tensors")
+            raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT, "expected complex64 tensors on the GPU")
         if x.dim() != 2 or tuple(out.shape) != tuple(x.shape):
             raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT, "expected two (batch, n) views of the same shape")
         batch, n = x.shape
@@ -205,12 +232,17 @@ class GpuFft(FftExecutor):
         pipeline's Gaussian noise of add_noise_device when noise_sigma != 0;
         out may be x."""
         _check_tensor(x)
-        _check_tensor(out)
+        _check_tensor(out, x.shape)
         batch, n = x.shape
+        _check_real(s, n, "spectral window")
         check(self._L.wgfft_filter_device(self._h, x.data_ptr(), s.data_ptr(), out.data_ptr(), n, batch,
                                           seed & 0xFFFFFFFF, noise_sigma, self._stream(stream)))
 
     def bench_device(self, x, out, inverse: bool = False, warmup: int = 3, iters: int = 50) -> float:
+        """Seconds per exec_device call on resident buffers, timed with HIP
+        events on the handle's stream (wgfft_bench_device)."""
+        _check_tensor(x)
+        _check_tensor(out, x.shape)
         batch, n = x.shape
         sec = ctypes.c_double(0.0)
         check(self._L.wgfft_bench_device(self._h, x.data_ptr(), out.data_ptr(), n, batch,
@@ -278,33 +310,45 @@ class RaggedPlan:
         k = np.ascontiguousarray(log2n, dtype=np.uint32)
         if off.shape != k.shape:
             raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT, "offsets and log2n differ in length")
+        if not getattr(gpu, "_h", None):
+            raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT, "the GpuFft handle is closed")
         p = ctypes.c_void_p()
         check(gpu._L.wgfft_ragged_create(gpu._h, off.ctypes.data, k.ctypes.data, len(off), ctypes.byref(p)))
         self._p = p
         self._gpu = gpu
         self.samples = int(gpu._L.wgfft_ragged_samples(p))
+        # complex elements a buffer must hold: the end of the last signal
+        self.extent = int((off + (np.uint64(1) << k.astype(np.uint64))).max()) if len(off) else 0
+        gpu._plans.append(weakref.ref(self))
 
     def exec_device(self, x, out, inverse: bool = False, mode: int = 0, stream=None) -> None:
         """x, out: contiguous complex64 CUDA tensors holding the packed signals."""
         import torch
 
+        if not getattr(self, "_p", None):
+            raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT, "the mixed-length plan is closed")
         for t in (x, out):
-            if not (t.is_cuda and t.dtype == torch.complex64 and t.is_contiguous()):
-                raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT, "expected contiguous complex64 CUDA tensors")
+            if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.complex64 and t.is_contiguous()
+                    and t.numel() >= self.extent):
+                raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT,
+                                 f"expected contiguous complex64 tensors of at least {self.extent} elements on the GPU")
         check(self._gpu._L.wgfft_ragged_exec_device(self._p, x.data_ptr(), out.data_ptr(),
                                                     WGFFT_INVERSE if inverse else WGFFT_FORWARD, int(mode),
                                                     GpuFft._stream(stream)))
 
     def describe(self) -> str:
         """Kernel choice of the plan (wgfft_ragged_describe)."""
+        if not getattr(self, "_p", None):
+            raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT, "the mixed-length plan is closed")
         buf = ctypes.create_string_buffer(256)
         check(self._gpu._L.wgfft_ragged_describe(self._p, buf, 256))
         return buf.value.decode()
 
     def close(self) -> None:
-        if getattr(self, "_p", None):
+        # a plan whose handle was closed was freed with it (_p already None)
+        if getattr(self, "_p", None) and getattr(self._gpu, "_h", None):
             self._gpu._L.wgfft_ragged_destroy(self._p)
-            self._p = None
+        self._p = None
 
     def __del__(self):
         try:
@@ -322,12 +366,28 @@ def ragged_layout(log2n) -> tuple[np.ndarray, int]:
     return offsets, int(lens.sum())
 
 
-def _check_tensor(t) -> None:
+def _check_tensor(t, shape=None) -> None:
+    """A contiguous (batch, n) complex64 CUDA tensor (of `shape` if given):
+    anything else never reaches a kernel as a pointer."""
     import torch
 
-    if not (t.is_cuda and t.dtype == torch.complex64 and t.is_contiguous() and t.dim() == 2):
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.complex64 and t.is_contiguous()
+            and t.dim() == 2):
         raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT,
                          "expected a contiguous (batch, n) complex64 tensor on the GPU")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT,
+                         f"expected a tensor of shape {tuple(shape)}, got {tuple(t.shape)}")
+
+
+def _check_real(t, n: int, what: str) -> None:
+    """A contiguous float32 CUDA tensor of exactly n elements."""
+    import torch
+
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()
+            and t.numel() == n):
+        raise WgfftError(_lib.WGFFT_ERR_INVALID_ARGUMENT,
+                         f"{what} must be a contiguous float32 tensor of {n} elements on the GPU")
 
 
 def synth_uniform(shape: Iterable[int], seed: int) -> np.ndarray:

@@ -207,6 +207,13 @@ int wgfft_pass_variant_count(uint64_t n, int pass);
  * this handle; writes the plan description into desc (if non-NULL). */
 int wgfft_select_pass_variant(wgfft_t* h, uint64_t n, int pass, int index, char* desc, uint64_t desc_len);
 
+/* Give length n (> 16384) the radix split 2^log2r[0] x ... x 2^log2r[passes-1]
+ * (first pass first), each pass on its default kernel, as a tuning run would;
+ * the radices must multiply to n and each needs a compiled kernel at its
+ * stride.  Writes the plan description into desc (if non-NULL).  Lets
+ * callers pin a split and tests run three-pass chains at small lengths. */
+int wgfft_select_split(wgfft_t* h, uint64_t n, const int* log2r, int passes, char* desc, uint64_t desc_len);
+
 /* Number of compiled fused forward+adjoint / VJP variants for length n (0 when
  * the length is served by the unfused path), and: make variant `index` the
  * handle's choice for length n, as a tuning run would, writing its

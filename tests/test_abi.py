@@ -171,3 +171,9 @@ def test_real_operand_checks():
             _check_real(bad, 64, "window")
     with pytest.raises(wgfft.WgfftError):
         _check_tensor(np.zeros((2, 64), np.complex64))
+
+
+def test_select_split_validation():
+    L = _lib.load()
+    arr = (ctypes.c_int * 3)(7, 7, 6)
+    assert L.wgfft_select_split(None, 1 << 20, arr, 3, None, 0) == wgfft.WGFFT_ERR_INVALID_ARGUMENT

@@ -596,3 +596,23 @@ def test_device_wrappers_reject_bad_operands(gpu, torch_cuda):
         gpu.exec_device(x, torch.empty((batch, n // 2), dtype=torch.complex64, device="cuda"))
     gpu.exec_windowed_device(x, w, y)  # the well-formed call still runs
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("log2n,split", [(21, (7, 7, 7)), (22, (8, 7, 7)), (23, (8, 8, 7))])
+def test_pinned_splits_vs_fp64(log2n, split):
+    """Radix splits pinned with wgfft_select_split (three passes at
+    lengths where the default has two) match FP64 forward and inverse, in
+    place and out of place."""
+    n = 1 << log2n
+    x = oracle.synth_uniform((2, n), 0x5EED0270 + log2n)
+    refs = fp64_refs(x)
+    g = wgfft.GpuFft.new()
+    try:
+        desc = g.select_split(n, split)
+        assert desc.count("(") == len(split), desc
+        check_fwd_inv(g, x, refs)
+        buf = x.copy()
+        wgfft._lib.check(g._L.wgfft_exec_host(g._h, buf.ctypes.data, buf.ctypes.data, n, 2, wgfft.WGFFT_FORWARD))
+        assert oracle.rel_l2(buf, refs[0]) <= rel_bar(n), desc
+    finally:
+        g.close()

@@ -350,18 +350,21 @@ int Handle::get_plan(uint64_t n, Plan** out) {
     return WGFFT_OK;
 }
 
-int Handle::blocks_per_cu(const BatchedVariant* v, Mode m) {
+int Handle::blocks_per_cu(const BatchedVariant* v, Mode m, bool cache) {
     auto key = std::make_pair(v, (int)m);
     auto it = occ_cache.find(key);
     if (it != occ_cache.end()) return it->second;
     int b = v->blocks_per_cu(m);
-    occ_cache[key] = b;
+    if (cache) occ_cache[key] = b;
     return b;
 }
 
-// Enqueue one batched transform of `batch` length-n signals.
+// Enqueue one batched transform of `batch` length-n signals.  With fixed
+// scratch (fixed != nullptr, fixed_bytes of device memory) nothing is
+// allocated: the multipass intermediates are carved from it and the batch is
+// cut into chunks that fit; the caller has checked that one transform fits.
 int Handle::run(Plan* p, Mode mode, const float2* in, float2* out, const float* win, uint64_t batch,
-                float scale, hipStream_t s, const BatchedVariant* force) {
+                float scale, hipStream_t s, const BatchedVariant* force, float2* fixed, uint64_t fixed_bytes) {
     if (batch == 0) return WGFFT_OK;
     hipError_t e = hipSuccess;
     if (p->kind == Plan::kSmall) {
@@ -372,47 +375,60 @@ int Handle::run(Plan* p, Mode mode, const float2* in, float2* out, const float* 
     }
     if (p->kind == Plan::kBatched) {
         const BatchedVariant* v = force ? force : p->var;
-        int grid = blocks_per_cu(v, mode) * num_cus;
+        int grid = blocks_per_cu(v, mode, fixed == nullptr) * num_cus;
         BatchedArgs a{in, out, p->tw, win, batch, scale, grid, s};
         e = v->launch(mode, a);
         if (e != hipSuccess) return hip_fail(e, "batched kernel launch");
         return WGFFT_OK;
     }
-    // multi-pass: chunk the batch so the scratch intermediate stays bounded
+    // multi-pass: chunk the batch so the scratch intermediates stay bounded
     const uint64_t tbytes = p->n * sizeof(float2);
-    uint64_t chunk = std::max<uint64_t>(1, mp_chunk_bytes / tbytes);
-    chunk = std::min(chunk, batch);
     const int P = (int)p->pass_var.size();
-    const bool inplace = (const void*)in == (const void*)out;
+    const bool two = multipass_needs_two_buffers(P);
+    float2 *buf_a, *buf_b;
+    uint64_t chunk;
+    if (fixed) {
+        const uint64_t per = two ? fixed_bytes / 2 : fixed_bytes;
+        chunk = std::min<uint64_t>(per / tbytes, batch);
+        if (chunk == 0) return fail(WGFFT_ERR_INVALID_ARGUMENT, "the transform's intermediates exceed the fixed scratch");
+        buf_a = fixed;
+        buf_b = two ? fixed + (per / sizeof(float2)) : nullptr;
+        return run_passes(p, mode, in, out, win, batch, scale, s, buf_a, buf_b, chunk);
+    }
+    chunk = std::min<uint64_t>(std::max<uint64_t>(1, mp_chunk_bytes / tbytes), batch);
     int rc;
     ScratchUse su(this, s);
     if (su.e != hipSuccess) return hip_fail(su.e, "stream ordering of the scratch buffers");
     if ((rc = ensure(scratch, chunk * tbytes))) return rc;
-    if ((P % 2 == 1 || inplace) && P > 1) {
-        if ((rc = ensure(scratch2, chunk * tbytes))) return rc;
-    }
+    if (two && (rc = ensure(scratch2, chunk * tbytes))) return rc;
+    buf_a = (float2*)scratch.p;
+    buf_b = two ? (float2*)scratch2.p : nullptr;
+    return run_passes(p, mode, in, out, win, batch, scale, s, buf_a, buf_b, chunk);
+}
+
+// The intermediates of a chain of P passes: pass i < P - 1 writes buf_a when
+// i is even and buf_b when odd, so three passes or more need both (two
+// passes, in place or not, read the input and write the output directly).
+bool multipass_needs_two_buffers(int passes) { return passes >= 3; }
+
+// The global passes of a multipass plan over `batch` transforms, `chunk`
+// transforms at a time through the intermediates buf_a (and buf_b).
+int Handle::run_passes(Plan* p, Mode mode, const float2* in, float2* out, const float* win, uint64_t batch,
+                       float scale, hipStream_t s, float2* buf_a, float2* buf_b, uint64_t chunk) {
+    const int P = (int)p->pass_var.size();
     for (uint64_t b0 = 0; b0 < batch; b0 += chunk) {
-        uint64_t nb = std::min(chunk, batch - b0);
+        const uint64_t nb = std::min(chunk, batch - b0);
         const float2* src = in + b0 * p->n;
         float2* dst_final = out + b0 * p->n;
         uint64_t L = 1;
         for (int i = 0; i < P; ++i) {
-            float2* dst;
-            if (i == P - 1) {
-                dst = dst_final;
-            } else {
-                // alternate scratch buffers; never write the buffer being read
-                float2* a = (float2*)scratch.p;
-                float2* b = (float2*)(scratch2.p ? scratch2.p : scratch.p);
-                dst = (i % 2 == 0) ? a : b;
-                if ((const void*)dst == (const void*)src) dst = (dst == a) ? b : a;
-            }
+            float2* dst = (i == P - 1) ? dst_final : (i % 2 == 0) ? buf_a : buf_b;
             const GlobalPassVariant* v = p->pass_var[i];
             GlobalPassArgs a{src, dst, p->tw_r[p->pass_log2r[i]], p->tw_hi, p->tw_lo, win, (uint32_t)p->log2n, p->h, L,
                              nb, (i == P - 1) ? scale : 1.0f, num_cus, s};
             Mode m = mode;
             if (mode == kWindowed && i > 0) m = kForward;
-            e = v->launch(m, a);
+            const hipError_t e = v->launch(m, a);
             if (e != hipSuccess) return hip_fail(e, "global pass launch");
             src = dst;
             L <<= p->pass_log2r[i];

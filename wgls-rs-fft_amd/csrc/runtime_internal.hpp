@@ -64,6 +64,7 @@ const BatchedVariant* default_variant(int log2n);
 hipError_t launch_add_noise(const float2* src, float2* dst, uint64_t n, uint64_t batch, uint32_t seed,
                             float sigma, hipStream_t s);
 std::vector<int> multipass_radices(int log2n);
+bool multipass_needs_two_buffers(int passes);
 std::vector<std::vector<int>> multipass_splits(int log2n);
 const BatchedVariant* find_variant(int log2n, const char* plan, int E, int G, int flags);
 std::vector<const GlobalPassVariant*> global_pass_candidates(int log2r, uint64_t L, uint64_t nr);
@@ -105,9 +106,12 @@ struct Handle {
     int upload_table(const std::vector<float2>& host, float2** dev);
     int get_plan(uint64_t n, Plan** out);
     int set_split(Plan* p, const std::vector<int>& log2r);
-    int blocks_per_cu(const BatchedVariant* v, Mode m);
+    // workgroups per CU of a variant (kept in occ_cache when cache is set)
+    int blocks_per_cu(const BatchedVariant* v, Mode m, bool cache = true);
     int run(Plan* p, Mode mode, const float2* in, float2* out, const float* win, uint64_t batch, float scale,
-            hipStream_t s, const BatchedVariant* force = nullptr);
+            hipStream_t s, const BatchedVariant* force = nullptr, float2* fixed = nullptr, uint64_t fixed_bytes = 0);
+    int run_passes(Plan* p, Mode mode, const float2* in, float2* out, const float* win, uint64_t batch, float scale,
+                   hipStream_t s, float2* buf_a, float2* buf_b, uint64_t chunk);
     bool host_pipelined(uint64_t n, uint64_t batch) const;
     // Host in -> host out through the staging pipeline; run_chunk(d_in, d_out,
     // transforms) enqueues one chunk's transform on `stream`.

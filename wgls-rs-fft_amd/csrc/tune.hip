@@ -437,3 +437,35 @@ extern "C" int wgfft_select_pass_variant(wgfft_t* h, uint64_t n, int pass, int i
     if (desc && desc_len) std::snprintf(desc, desc_len, "%s", describe_multipass(p).c_str());
     return WGFFT_OK;
 }
+
+extern "C" int wgfft_select_split(wgfft_t* h, uint64_t n, const int* log2r, int passes, char* desc,
+                                  uint64_t desc_len) {
+    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
+    const int l = log2_exact(n);
+    if (l < 0) return fail(WGFFT_ERR_INVALID_LENGTH, kMsgLength);
+    if (!log2r || passes < 2) return fail(WGFFT_ERR_INVALID_ARGUMENT, "a split needs at least two radices");
+    std::vector<int> r(log2r, log2r + passes);
+    int sum = 0;
+    uint64_t L = 1;
+    for (int ri : r) {
+        if (ri < 1 || ri > 13) return fail(WGFFT_ERR_INVALID_ARGUMENT, "radices must be 2^1 .. 2^13");
+        sum += ri;
+    }
+    if (sum != l) return fail(WGFFT_ERR_INVALID_ARGUMENT, "the radices must multiply to n");
+    for (int ri : r) {
+        if (global_pass_candidates(ri, L, n >> ri).empty())
+            return fail(WGFFT_ERR_INVALID_ARGUMENT, "no global-pass kernel for radix 2^" + std::to_string(ri) +
+                                                        " at stride " + std::to_string(L));
+        L <<= ri;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    Plan* p;
+    int rc;
+    if ((rc = h->get_plan(n, &p))) return rc;
+    if (p->kind != Plan::kMultiPass) return fail(WGFFT_ERR_INVALID_ARGUMENT, "length has no global passes");
+    if ((rc = h->set_split(p, r))) return rc;
+    if (desc && desc_len) std::snprintf(desc, desc_len, "%s", describe_multipass(p).c_str());
+    return WGFFT_OK;
+}

@@ -30,7 +30,7 @@ EXPORTS = (
     "wgfft_ragged_create", "wgfft_ragged_destroy", "wgfft_ragged_exec_device", "wgfft_ragged_samples", "wgfft_ragged_describe",
     "wgfft_bench_device",
     "wgfft_autotune", "wgfft_plan_describe", "wgfft_variant_count", "wgfft_select_variant",
-    "wgfft_pass_count", "wgfft_pass_variant_count", "wgfft_select_pass_variant",
+    "wgfft_pass_count", "wgfft_pass_variant_count", "wgfft_select_pass_variant", "wgfft_select_split",
     "wgfft_adjoint_variant_count", "wgfft_select_adjoint_variant",
     "wgfft_synth_uniform", "wgfft_set_alloc_cap",
 )
@@ -100,6 +100,7 @@ def load() -> ctypes.CDLL:
     L.wgfft_pass_count.argtypes = [_u64]
     L.wgfft_pass_variant_count.argtypes = [_u64, _i]
     L.wgfft_select_pass_variant.argtypes = [_vp, _u64, _i, _i, ctypes.c_char_p, _u64]
+    L.wgfft_select_split.argtypes = [_vp, _u64, ctypes.POINTER(_i), _i, ctypes.c_char_p, _u64]
     L.wgfft_adjoint_variant_count.argtypes = [_u64]
     L.wgfft_select_adjoint_variant.argtypes = [_vp, _u64, _i, ctypes.c_char_p, _u64]
     L.wgfft_synth_uniform.argtypes = [_vp, _u64, _u64]

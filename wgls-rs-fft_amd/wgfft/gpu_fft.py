@@ -281,6 +281,15 @@ class GpuFft(FftExecutor):
         check(self._L.wgfft_select_pass_variant(self._h, n, p, index, buf, 256))
         return buf.value.decode()
 
+    def select_split(self, n: int, log2r) -> str:
+        """Pin the radix split 2^log2r[0] x 2^log2r[1] x ... of length n
+        (wgfft_select_split); returns the plan description."""
+        r = [int(v) for v in log2r]
+        arr = (ctypes.c_int * max(1, len(r)))(*r)
+        buf = ctypes.create_string_buffer(256)
+        check(self._L.wgfft_select_split(self._h, n, arr, len(r), buf, 256))
+        return buf.value.decode()
+
     def select_variant(self, n: int, index: int) -> str:
         """Pin compiled variant `index` for length n (wgfft_select_variant)."""
         buf = ctypes.create_string_buffer(256)

@@ -95,6 +95,23 @@ int wgfft_exec_host(wgfft_t* h, const float* in, float* out, uint64_t n, uint64_
 int wgfft_exec_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, uint64_t batch, int dir,
                       void* stream);
 
+/* The same transform allocating nothing: no hipMalloc, hipFree or other
+ * allocating call, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it).  No counterpart in
+ * the reference.  The intermediates of lengths above 16384 live in a static
+ * 64 MiB device array that every handle on a GPU shares (each GPU the library
+ * runs on holds it until the process ends): at most 2^23 for a two-pass plan,
+ * 2^22 for three passes or more; larger lengths fail with
+ * WGFFT_ERR_INVALID_ARGUMENT, and larger batches run in chunks that fit.
+ * The handle's plan for n must exist (a plan allocates its twiddle tables):
+ * make it first with wgfft_plan_describe or any other call of length n, else
+ * WGFFT_ERR_INVALID_ARGUMENT.  Calls of this entry point that overlap on one
+ * GPU (other streams, other handles) race on the shared array: order them.
+ * The array keeps the last call's intermediates; nothing in it needs to
+ * start at zero. */
+int wgfft_exec_device_noalloc(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, uint64_t batch, int dir,
+                              void* stream);
+
 /* Batched FFT of strided device layouts: element j of signal b is at complex
  * offset b*idist + j*istride of d_in (b*odist + j*ostride of d_out); 8-byte
  * aligned buffers; the output layout must not overlap itself.  The reference

@@ -458,6 +458,45 @@ __global__ void __launch_bounds__(256) k_scatter(const float2* __restrict__ src,
 
 using namespace wgfft;
 
+// Device scratch of wgfft_exec_device_noalloc: the multipass intermediates of
+// callers that cannot allocate.  Each GPU the library runs on holds one copy
+// (64 MiB) until the process ends.  Host code never passes this name to a
+// kernel or a copy (on the host it is only a stand-in for the symbol): it
+// uses the device address hipGetSymbolAddress returns for the current device,
+// looked up once per device when a handle is created there.
+constexpr uint64_t kNoAllocScratchBytes = 64ull << 20;
+static __device__ __align__(256) float2 scratch[kNoAllocScratchBytes / sizeof(float2)];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static float2* g_noalloc_ptr[kMaxDevices];        // device address of `scratch` per device
+static std::string g_noalloc_err[kMaxDevices];    // why the lookup failed, if it did
+
+// With `device` current: find this device's copy of the static scratch.
+static void noalloc_lookup(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        g_noalloc_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return;
+    }
+    g_noalloc_ptr[device] = (float2*)ptr;
+}
+
+static float2* noalloc_ptr(int device, std::string* why) {
+    if (device < 0 || device >= kMaxDevices) {
+        *why = "device index beyond the static scratch table";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[device]) *why = g_noalloc_err[device];
+    return g_noalloc_ptr[device];
+}
+
 static bool g_device_probed = false;
 static int g_device_count = 0;
 static std::mutex g_probe_mu;
@@ -546,6 +585,7 @@ int wgfft_create(wgfft_t** out, int device) {
         delete h;
         return hip_fail(e, "hipStreamCreateWithFlags");
     }
+    noalloc_lookup(device);  // with `device` current
     if (const char* env = std::getenv("WGFFT_MP_CHUNK_BYTES")) h->mp_chunk_bytes = std::strtoull(env, nullptr, 10);
     if (const char* env = std::getenv("WGFFT_HOST_CHUNK_BYTES"))
         h->host_chunk_bytes = std::max<uint64_t>(1ull << 20, std::strtoull(env, nullptr, 10));
@@ -588,6 +628,42 @@ int wgfft_exec_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, u
     bool inv = dir == WGFFT_INVERSE;
     return h->run(p, inv ? kInverse : kForward, (const float2*)d_in, (float2*)d_out, nullptr, batch,
                   inv ? 1.0f / (float)n : 1.0f, pick_stream(h, stream));
+}
+
+int wgfft_exec_device_noalloc(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, uint64_t batch, int dir,
+                              void* stream) {
+    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
+    int rc;
+    if ((rc = check_len(n)) || (rc = check_dir(dir)) || (rc = check_size(n, batch))) return rc;
+    if (batch == 0) return WGFFT_OK;
+    if (!d_in || !d_out) return fail(WGFFT_ERR_INVALID_ARGUMENT, "device buffer is NULL");
+    if ((rc = check_aligned(d_in, "d_in")) || (rc = check_aligned(d_out, "d_out"))) return rc;
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    std::string why;
+    float2* sp = noalloc_ptr(h->device, &why);
+    if (!sp) return fail(WGFFT_ERR_HIP, "the static scratch is not available on this device: " + why);
+    // a plan allocates its twiddle tables, so it must exist already
+    auto it = h->plans.find(n);
+    if (it == h->plans.end())
+        return fail(WGFFT_ERR_INVALID_ARGUMENT,
+                    "no plan for this length on the handle: make it first with a call that may allocate "
+                    "(wgfft_plan_describe, or any transform of this length)");
+    Plan* p = &it->second;
+    if (p->kind == Plan::kMultiPass) {
+        const uint64_t need =
+            p->n * sizeof(float2) * (multipass_needs_two_buffers((int)p->pass_var.size()) ? 2 : 1);
+        if (need > kNoAllocScratchBytes)
+            return fail(WGFFT_ERR_INVALID_ARGUMENT,
+                        "length " + std::to_string(n) + " needs " + std::to_string(need) +
+                            " bytes of intermediates per transform, more than the " +
+                            std::to_string(kNoAllocScratchBytes) +
+                            "-byte static scratch (at most 2^23 for a two-pass plan, 2^22 for three passes)");
+    }
+    const bool inv = dir == WGFFT_INVERSE;
+    return h->run(p, inv ? kInverse : kForward, (const float2*)d_in, (float2*)d_out, nullptr, batch,
+                  inv ? 1.0f / (float)n : 1.0f, pick_stream(h, stream), nullptr, sp, kNoAllocScratchBytes);
 }
 
 int wgfft_exec_strided_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, uint64_t batch,

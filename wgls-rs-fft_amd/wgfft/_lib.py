@@ -24,7 +24,8 @@ WGFFT_INVERSE = 1
 # Every symbol include/wgfft.h declares.
 EXPORTS = (
     "wgfft_device_available", "wgfft_create", "wgfft_destroy", "wgfft_last_error", "wgfft_name",
-    "wgfft_validate_lengths", "wgfft_exec_host", "wgfft_exec_device", "wgfft_exec_windowed_host",
+    "wgfft_validate_lengths", "wgfft_exec_host", "wgfft_exec_device", "wgfft_exec_device_noalloc",
+    "wgfft_exec_windowed_host",
     "wgfft_exec_windowed_device", "wgfft_vjp_windowed_device", "wgfft_fwd_adjoint_masked_device",
     "wgfft_filter_device", "wgfft_exec_strided_device", "wgfft_add_noise_device",
     "wgfft_ragged_create", "wgfft_ragged_destroy", "wgfft_ragged_exec_device", "wgfft_ragged_samples", "wgfft_ragged_describe",
@@ -78,6 +79,7 @@ def load() -> ctypes.CDLL:
     L.wgfft_validate_lengths.argtypes = [ctypes.POINTER(_u64), _u64, ctypes.POINTER(_u64)]
     L.wgfft_exec_host.argtypes = [_vp, _vp, _vp, _u64, _u64, _i]
     L.wgfft_exec_device.argtypes = [_vp, _vp, _vp, _u64, _u64, _i, _vp]
+    L.wgfft_exec_device_noalloc.argtypes = [_vp, _vp, _vp, _u64, _u64, _i, _vp]
     L.wgfft_exec_windowed_host.argtypes = [_vp, _vp, _vp, _vp, _u64, _u64]
     L.wgfft_exec_windowed_device.argtypes = [_vp, _vp, _vp, _vp, _u64, _u64, _vp]
     L.wgfft_vjp_windowed_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _u64, _u64, _vp]

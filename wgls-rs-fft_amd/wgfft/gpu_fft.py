@@ -167,6 +167,15 @@ class GpuFft(FftExecutor):
         check(self._L.wgfft_exec_device(self._h, x.data_ptr(), out.data_ptr(), n, batch,
                                         WGFFT_INVERSE if inverse else WGFFT_FORWARD, self._stream(stream)))
 
+    def exec_device_noalloc(self, x, out, inverse: bool = False, stream=None) -> None:
+        """exec_device without any allocation (wgfft_exec_device_noalloc): the
+        plan for n must exist (describe(n) makes it) and n <= 2^23."""
+        _check_tensor(x)
+        _check_tensor(out, x.shape)
+        batch, n = x.shape
+        check(self._L.wgfft_exec_device_noalloc(self._h, x.data_ptr(), out.data_ptr(), n, batch,
+                                                WGFFT_INVERSE if inverse else WGFFT_FORWARD, self._stream(stream)))
+
     def exec_windowed_device(self, x, w, out, stream=None) -> None:
         """out = FFT(w * x) per row: x, out (batch, n) complex64, w n floats."""
         _check_tensor(x)

@@ -70,7 +70,13 @@ def lib():
 
 
 def _threads() -> int:
-    return max(1, min(16, os.cpu_count() or 1))
+    """Threads for the checker: the CPUs this process may run on, capped by
+    OMP_NUM_THREADS when set (a GPU box declares its per-GPU CPU share there)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
 
 
 def _p(a: np.ndarray, t):

@@ -616,3 +616,38 @@ def test_pinned_splits_vs_fp64(log2n, split):
         assert oracle.rel_l2(buf, refs[0]) <= rel_bar(n), desc
     finally:
         g.close()
+
+
+def test_bench_device_matches_event_timing(gpu, torch_cuda):
+    """wgfft_bench_device (GpuFftTrait::benchmark_gpu_only, src/lib.rs:109-143:
+    warm-up calls with waits, then back-to-back calls between HIP events on
+    the handle's stream) agrees with HIP-event timing of the same device calls
+    on the caller's stream, and leaves the transform in the output."""
+    torch = torch_cuda
+    import time
+
+    n, batch = 4096, 2048
+    xs = wgfft.synth_uniform((batch, n), 0x5EED0B00)
+    x = torch.from_numpy(xs).cuda()
+    y = torch.empty_like(x)
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 0.3:  # settle the shader clock
+        for _ in range(20):
+            gpu.exec_device(x, y)
+        torch.cuda.synchronize()
+    ev = []
+    lib = []
+    for _ in range(3):
+        lib.append(gpu.bench_device(x, y, warmup=3, iters=50) * 1e3)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(50):
+            gpu.exec_device(x, y)
+        e1.record()
+        torch.cuda.synchronize()
+        ev.append(e0.elapsed_time(e1) / 50)
+    a, b = min(lib), min(ev)
+    assert a > 0 and abs(a - b) / b < 0.12, (lib, ev)
+    assert oracle.rel_l2(y[:4].cpu().numpy(), oracle.fft64(xs[:4])) <= 5e-6
+    with pytest.raises(wgfft.WgfftError):
+        gpu.bench_device(x, y, iters=0)

@@ -116,9 +116,12 @@ int wgfft_exec_device_noalloc(wgfft_t* h, const float* d_in, float* d_out, uint6
  * offset b*idist + j*istride of d_in (b*odist + j*ostride of d_out); 8-byte
  * aligned buffers; the output layout must not overlap itself.  The reference
  * lists configurable strides and distances as future work (REQUIREMENTS.md:14,
- * "Stride Management").  Unit stride with distance n is the plain device call;
- * other layouts are gathered into a contiguous scratch buffer, transformed in
- * place there and scattered, on the caller's stream. */
+ * "Stride Management").  Unit stride with distance n is the plain device call.
+ * Padded rows -- unit strides, even distances of at least n, 16-byte aligned
+ * buffers, input and output disjoint (or the same rows, in place) -- are
+ * transformed in one pass for n <= 16384.  Other layouts are gathered into a
+ * contiguous scratch buffer, transformed in place there and scattered, on the
+ * caller's stream. */
 int wgfft_exec_strided_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, uint64_t batch,
                               uint64_t istride, uint64_t idist, uint64_t ostride, uint64_t odist, int dir,
                               void* stream);

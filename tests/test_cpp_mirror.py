@@ -34,10 +34,10 @@ def test_cpp_mirror_error_contract_without_device():
     if wgfft.is_gpu_available():
         pytest.skip("a device is present")
     out = _run("cpu")
-    assert out.count("PASS") == 4
+    assert out.count("PASS") >= 20
 
 
 @pytest.mark.gpu
 def test_cpp_mirror_reference_tests_on_gpu():
     out = _run("gpu")
-    assert out.count("PASS") >= 10
+    assert out.count("PASS") >= 35

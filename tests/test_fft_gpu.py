@@ -651,3 +651,71 @@ def test_bench_device_matches_event_timing(gpu, torch_cuda):
     assert oracle.rel_l2(y[:4].cpu().numpy(), oracle.fft64(xs[:4])) <= 5e-6
     with pytest.raises(wgfft.WgfftError):
         gpu.bench_device(x, y, iters=0)
+
+
+@pytest.mark.parametrize("n,batch", [(8, 100), (256, 33), (4096, 64), (16384, 6)])
+def test_padded_rows_one_pass(torch_cuda, n, batch):
+    """Padded rows (unit stride, even distances >= n) are transformed in one
+    pass with no scratch: the call still works under an allocation cap that
+    the gather/scatter path would exceed, and matches the contiguous call bit
+    for bit; odd distances and rows overlapping in another layout take the
+    gather/scatter path and match too."""
+    torch = torch_cuda
+    x = oracle.synth_uniform((batch, n), 0x5EED0920 + n)
+    t = torch.from_numpy(x).cuda()
+    g = wgfft.GpuFft.new()
+    ref_fwd = None
+    try:
+        for inverse in (False, True):
+            ref = torch.empty_like(t)
+            g.exec_device(t, ref, inverse=inverse)
+            if not inverse:
+                ref_fwd = ref
+            src = torch.zeros((batch, n + 64), dtype=torch.complex64, device="cuda")
+            src[:, :n] = t
+            dst = torch.zeros((batch, n + 32), dtype=torch.complex64, device="cuda")
+            torch.cuda.synchronize()
+            wgfft.set_alloc_cap(1 << 20 if n * batch * 8 > (1 << 20) else 0)
+            try:
+                g.exec_strided_device(src[:, :n], dst[:, :n], inverse=inverse)
+            finally:
+                wgfft.set_alloc_cap(0)
+            torch.cuda.synchronize()
+            assert torch.equal(dst[:, :n], ref), inverse
+            assert not dst[:, n:].abs().any()  # the padding is untouched
+            # in place on padded rows
+            inpl = src.clone()
+            g.exec_strided_device(inpl[:, :n], inpl[:, :n], inverse=inverse)
+            torch.cuda.synchronize()
+            assert torch.equal(inpl[:, :n], ref) and torch.equal(inpl[:, n:], src[:, n:]), inverse
+            # odd input distance: gathered
+            odd = torch.zeros((batch, n + 1), dtype=torch.complex64, device="cuda")
+            odd[:, :n] = t
+            out = torch.empty_like(t)
+            g.exec_strided_device(odd[:, :n], out, inverse=inverse)
+            torch.cuda.synchronize()
+            assert torch.equal(out, ref), inverse
+        assert oracle.rel_l2(ref.cpu().numpy(), oracle.fft64(x, +1) / n) <= 5e-6
+    finally:
+        g.close()
+    if n * batch * 8 > (1 << 20):
+        # on a fresh handle (no scratch yet) under a 1 MiB cap: padded rows
+        # run, an element stride of 2 needs the scratch copy and is refused
+        g = wgfft.GpuFft.new()
+        try:
+            g.describe(n)
+            strided = torch.zeros((batch, 2 * n), dtype=torch.complex64, device="cuda")[:, ::2]
+            out = torch.zeros((batch, n + 32), dtype=torch.complex64, device="cuda")
+            torch.cuda.synchronize()
+            wgfft.set_alloc_cap(1 << 20)
+            try:
+                g.exec_strided_device(src[:, :n], out[:, :n])
+                with pytest.raises(wgfft.WgfftError) as e:
+                    g.exec_strided_device(strided, out[:, :n])
+                assert e.value.code == wgfft.WGFFT_ERR_OUT_OF_MEMORY
+            finally:
+                wgfft.set_alloc_cap(0)
+            torch.cuda.synchronize()
+            assert torch.equal(out[:, :n], ref_fwd)
+        finally:
+            g.close()

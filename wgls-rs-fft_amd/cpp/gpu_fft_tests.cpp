@@ -14,6 +14,7 @@
 #include "gpu_fft.hpp"
 
 using wgls_rs_fft::Batch;
+using wgls_rs_fft::check;
 using wgls_rs_fft::Complex;
 using wgls_rs_fft::FftError;
 using wgls_rs_fft::GpuFft;
@@ -69,7 +70,54 @@ static float max_diff(const std::vector<Complex>& a, const std::vector<Complex>&
     return m;
 }
 
+// Host logic that needs no device: validation, argument checks on every
+// entry point, input synthesis, registry queries (the sanitizer build runs
+// these on the CPU).
+static void run_host_logic() {
+    uint64_t n = 7;
+    CHECK(wgfft_validate_lengths(nullptr, 0, &n) == WGFFT_OK && n == 0, "empty batch validates to n = 0");
+    uint64_t ok[3] = {1024, 1024, 1024};
+    CHECK(wgfft_validate_lengths(ok, 3, &n) == WGFFT_OK && n == 1024, "equal power-of-two lengths");
+    CHECK(wgfft_validate_lengths(nullptr, 2, &n) == WGFFT_ERR_INVALID_ARGUMENT, "NULL length table");
+    float buf[16];
+    CHECK(wgfft_exec_host(nullptr, buf, buf, 8, 1, WGFFT_FORWARD) == WGFFT_ERR_INVALID_ARGUMENT, "exec_host NULL handle");
+    CHECK(wgfft_exec_device(nullptr, buf, buf, 8, 1, WGFFT_FORWARD, nullptr) == WGFFT_ERR_INVALID_ARGUMENT,
+          "exec_device NULL handle");
+    CHECK(wgfft_exec_device_noalloc(nullptr, buf, buf, 8, 1, WGFFT_FORWARD, nullptr) == WGFFT_ERR_INVALID_ARGUMENT,
+          "exec_device_noalloc NULL handle");
+    CHECK(wgfft_exec_strided_device(nullptr, buf, buf, 8, 1, 1, 8, 1, 8, WGFFT_FORWARD, nullptr) ==
+              WGFFT_ERR_INVALID_ARGUMENT,
+          "exec_strided NULL handle");
+    CHECK(wgfft_vjp_windowed_device(nullptr, buf, buf, buf, buf, buf, 8, 1, nullptr) == WGFFT_ERR_INVALID_ARGUMENT,
+          "vjp NULL handle");
+    CHECK(wgfft_fwd_adjoint_masked_device(nullptr, buf, buf, buf, nullptr, buf, buf, buf, 8, 1, nullptr) ==
+              WGFFT_ERR_INVALID_ARGUMENT,
+          "fused adjoint NULL handle");
+    CHECK(wgfft_ragged_create(nullptr, nullptr, nullptr, 0, nullptr) == WGFFT_ERR_INVALID_ARGUMENT,
+          "ragged_create NULL handle");
+    CHECK(wgfft_ragged_samples(nullptr) == 0, "ragged_samples NULL plan");
+    wgfft_ragged_destroy(nullptr);
+    wgfft_destroy(nullptr);
+    int split[3] = {7, 7, 7};
+    CHECK(wgfft_select_split(nullptr, 1u << 21, split, 3, nullptr, 0) == WGFFT_ERR_INVALID_ARGUMENT,
+          "select_split NULL handle");
+    double sec = 0;
+    CHECK(wgfft_bench_device(nullptr, buf, buf, 8, 1, WGFFT_FORWARD, 0, 1, &sec) == WGFFT_ERR_INVALID_ARGUMENT,
+          "bench NULL handle");
+    std::vector<float> a(1000), b(1000);
+    wgfft_synth_uniform(a.data(), a.size(), 42);
+    wgfft_synth_uniform(b.data(), b.size(), 42);
+    bool in_range = true;
+    for (float v : a) in_range &= (v >= -1.f && v < 1.f);
+    CHECK(a == b && in_range, "synth_uniform deterministic in [-1, 1)");
+    CHECK(wgfft_variant_count(4096) >= 1 && wgfft_variant_count(12) == 0 && wgfft_pass_count(1u << 22) == 2 &&
+              wgfft_pass_variant_count(1u << 22, 5) == 0 && wgfft_adjoint_variant_count(1u << 20) == 0,
+          "registry queries");
+    CHECK(std::string(wgfft_name()).find("Stockham") != std::string::npos, "name");
+}
+
 static int run_cpu() {
+    run_host_logic();
     CHECK(!GpuFft::is_gpu_available(), "no device in this process");
     try {
         GpuFft::create();
@@ -138,6 +186,41 @@ static int run_gpu() {
         Batch fused = fft.fft_windowed({sig}, w);
         CHECK(max_diff(fused[0], spec[0]) < 1e-3f, "fft_windowed equals window-then-fft");
     }
+    // host paths of the runtime under load: the pinned-staging pipeline
+    // (>= 8 MB), a global-pass length, a pinned split, the autotuner, the
+    // mixed-length plan (freed with its handle), padded rows and the
+    // no-allocation entry point
+    {
+        const size_t n = 4096, batch = 1024;  // 32 MB: staged through pinned chunks
+        Batch in(batch, make_input(n));
+        Batch out = fft.fft(in);
+        Batch back = fft.ifft(out);
+        CHECK(out.size() == batch && max_diff(back[batch - 1], in[batch - 1]) < 1e-3f, "host pipeline 4096 x 1024");
+        Batch big(2, make_input(1u << 20));
+        Batch bo = fft.ifft(fft.fft(big));
+        CHECK(max_diff(bo[1], big[1]) < 1e-3f, "multipass 2^20 round trip");
+        char desc[256];
+        int split[3] = {7, 7, 7};
+        CHECK(wgfft_select_split(fft.handle(), 1u << 21, split, 3, desc, sizeof desc) == WGFFT_OK, "select_split");
+        Batch s3(1, make_input(1u << 21));
+        CHECK(max_diff(fft.ifft(fft.fft(s3))[0], s3[0]) < 1e-3f, "three-pass split round trip");
+        CHECK(wgfft_autotune(fft.handle(), 1024, 512, desc, sizeof desc) == WGFFT_OK, "autotune 1024");
+        CHECK(wgfft_autotune(fft.handle(), 1u << 17, 4, desc, sizeof desc) == WGFFT_OK, "autotune 2^17");
+    }
+    {
+        wgfft_t* h = nullptr;
+        check(wgfft_create(&h, 0));
+        uint64_t offs[3] = {0, 64, 64 + 1024};
+        uint32_t lg[3] = {6, 10, 16};
+        wgfft_ragged_t* p = nullptr;
+        wgfft_ragged_t* q = nullptr;
+        CHECK(wgfft_ragged_create(h, offs, lg, 3, &p) == WGFFT_OK && wgfft_ragged_samples(p) == 64 + 1024 + 65536,
+              "ragged plan");
+        CHECK(wgfft_ragged_create(h, offs, lg, 3, &q) == WGFFT_OK, "second ragged plan");
+        wgfft_ragged_destroy(q);
+        wgfft_destroy(h);  // frees p with the handle
+        CHECK(true, "handle destroyed with an open plan");
+    }
     // error contract on a live executor
     CHECK(fft.fft({}).empty(), "empty batch -> empty result");
     try {
@@ -157,5 +240,9 @@ static int run_gpu() {
 
 int main(int argc, char** argv) {
     const std::string mode = argc > 1 ? argv[1] : "cpu";
-    return mode == "gpu" ? run_gpu() : run_cpu();
+    if (mode == "gpu") {
+        run_host_logic();
+        return run_gpu();
+    }
+    return run_cpu();
 }

@@ -34,6 +34,8 @@ struct BatchedArgs {
     float scale;        // applied at store (1/N for the inverse)
     int grid;           // workgroups to launch (0: one per group)
     hipStream_t stream;
+    uint64_t idist = 0;  // complex elements between consecutive transforms of in (0: N)
+    uint64_t odist = 0;  // ... of out (0: N)
 };
 
 // BatchedVariant::flags
@@ -228,7 +230,7 @@ struct Io {
 template <int N, int E, int G, int FL, int MODE, int... Rs>
 __global__ void __launch_bounds__(G*(N / E)) __attribute__((amdgpu_waves_per_eu(flag_min_waves(FL) > 0 ? flag_min_waves(FL) : 1)))
 k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
-          const float* __restrict__ win, uint64_t batch, float scale) {
+          const float* __restrict__ win, uint64_t batch, float scale, uint64_t idist, uint64_t odist) {
     constexpr int T = N / E;
     constexpr bool INV = MODE == kInverse;
     constexpr bool NT = (FL & kFlagNtStore) != 0;
@@ -258,7 +260,7 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
     uint64_t grp = blockIdx.x;
     float2 nxt[PREFETCH ? E : 1];
     if constexpr (PREFETCH) {
-        if (grp < ngroups && grp * G + g < batch) IO::load(nxt, in + (grp * G + g) * N, t);
+        if (grp < ngroups && grp * G + g < batch) IO::load(nxt, in + (grp * G + g) * idist, t);
     }
 
     for (; grp < ngroups; grp += gridDim.x) {
@@ -274,10 +276,10 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
             for (int i = 0; i < E; ++i) d[i] = nxt[i];
             const uint64_t ngrp = grp + gridDim.x;
             if constexpr ((FL & kFlagPrio) != 0) __builtin_amdgcn_s_setprio(2);
-            if (ngrp < ngroups && ngrp * G + g < batch) IO::load(nxt, in + (ngrp * G + g) * N, t);
+            if (ngrp < ngroups && ngrp * G + g < batch) IO::load(nxt, in + (ngrp * G + g) * idist, t);
             if constexpr ((FL & kFlagPrio) != 0) __builtin_amdgcn_s_setprio(0);
         } else {
-            if (active) IO::load(d, in + tr * N, t);
+            if (active) IO::load(d, in + tr * idist, t);
         }
         if constexpr (MODE == kWindowed) {
 #pragma unroll
@@ -290,7 +292,7 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
         else
             EN::run(d, lds, tw, t, Radices<Rs...>{});
         if constexpr ((FL & kFlagPrio) != 0) __builtin_amdgcn_s_setprio(2);
-        if (active) IO::template store<INV, NT>(out + tr * N, d, t, scale);
+        if (active) IO::template store<INV, NT>(out + tr * odist, d, t, scale);
         if constexpr ((FL & kFlagPrio) != 0) __builtin_amdgcn_s_setprio(0);
         WGFFT_LDS_SYNC();  // the next group's first pass overwrites the exchange buffer
     }
@@ -310,7 +312,8 @@ struct BatchedKernel {
         if (grid > ngroups) grid = ngroups;
         if (grid == 0) return hipSuccess;
         hipLaunchKernelGGL((k_batched<N, E, G, FL, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0,
-                           a.stream, a.in, a.out, a.tw, a.win, a.batch, a.scale);
+                           a.stream, a.in, a.out, a.tw, a.win, a.batch, a.scale, a.idist ? a.idist : (uint64_t)N,
+                           a.odist ? a.odist : (uint64_t)N);
         return hipGetLastError();
     }
     static hipError_t launch(Mode m, const BatchedArgs& a) {
@@ -1016,7 +1019,7 @@ struct GlobalPassWideKernel {
 template <int N, int E, int MODE, int... Rs>
 __global__ void __launch_bounds__((N / 2) / E) __attribute__((amdgpu_waves_per_eu(4)))
 k_split2(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
-         const float* __restrict__ win, uint64_t batch, float scale) {
+         const float* __restrict__ win, uint64_t batch, float scale, uint64_t idist, uint64_t odist) {
     constexpr int H = N / 2;
     constexpr int T = H / E;
     constexpr bool INV = MODE == kInverse;
@@ -1030,7 +1033,7 @@ k_split2(const float2* __restrict__ in, float2* __restrict__ out, const float2* 
     typedef float f2v __attribute__((ext_vector_type(2)));
     for (uint64_t tr = blockIdx.x; tr < batch; tr += gridDim.x) {
         const uint32_t t = launder_lane(threadIdx.x);
-        const float2* x = in + tr * N;
+        const float2* x = in + tr * idist;
         float2 y0[E], y1[E];
         __builtin_amdgcn_s_setprio(2);
 #pragma unroll
@@ -1063,7 +1066,7 @@ k_split2(const float2* __restrict__ in, float2* __restrict__ out, const float2* 
         // opaque after the barrier: nothing of the second half is computed early
         const uint32_t t1 = launder_lane(t);
         EN::run_single(y1, lds, twh, t1, Radices<Rs...>{});
-        float2* o = out + tr * N;
+        float2* o = out + tr * odist;
         __builtin_amdgcn_s_setprio(2);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -1088,7 +1091,8 @@ struct Split2Kernel {
         if (grid > a.batch) grid = a.batch;
         if (grid == 0) return hipSuccess;
         hipLaunchKernelGGL((k_split2<N, E, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0, a.stream, a.in,
-                           a.out, a.tw, a.win, a.batch, a.scale);
+                           a.out, a.tw, a.win, a.batch, a.scale, a.idist ? a.idist : (uint64_t)N,
+                           a.odist ? a.odist : (uint64_t)N);
         return hipGetLastError();
     }
     static hipError_t launch(Mode m, const BatchedArgs& a) {

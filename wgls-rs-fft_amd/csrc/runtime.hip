@@ -363,12 +363,17 @@ int Handle::blocks_per_cu(const BatchedVariant* v, Mode m, bool cache) {
 // scratch (fixed != nullptr, fixed_bytes of device memory) nothing is
 // allocated: the multipass intermediates are carved from it and the batch is
 // cut into chunks that fit; the caller has checked that one transform fits.
+// idist / odist (complex elements between transforms, 0 = n) are honoured
+// by the single-kernel lengths only (n <= 16384).
 int Handle::run(Plan* p, Mode mode, const float2* in, float2* out, const float* win, uint64_t batch,
-                float scale, hipStream_t s, const BatchedVariant* force, float2* fixed, uint64_t fixed_bytes) {
+                float scale, hipStream_t s, const BatchedVariant* force, float2* fixed, uint64_t fixed_bytes,
+                uint64_t idist, uint64_t odist) {
     if (batch == 0) return WGFFT_OK;
     hipError_t e = hipSuccess;
+    if (p->kind == Plan::kMultiPass && ((idist && idist != p->n) || (odist && odist != p->n)))
+        return fail(WGFFT_ERR_INVALID_ARGUMENT, "signal distances other than n need n <= 16384");
     if (p->kind == Plan::kSmall) {
-        BatchedArgs a{in, out, nullptr, win, batch, scale, 0, s};
+        BatchedArgs a{in, out, nullptr, win, batch, scale, 0, s, idist, odist};
         e = launch_small(mode, p->log2n, a);
         if (e != hipSuccess) return hip_fail(e, "small-length kernel launch");
         return WGFFT_OK;
@@ -376,7 +381,7 @@ int Handle::run(Plan* p, Mode mode, const float2* in, float2* out, const float* 
     if (p->kind == Plan::kBatched) {
         const BatchedVariant* v = force ? force : p->var;
         int grid = blocks_per_cu(v, mode, fixed == nullptr) * num_cus;
-        BatchedArgs a{in, out, p->tw, win, batch, scale, grid, s};
+        BatchedArgs a{in, out, p->tw, win, batch, scale, grid, s, idist, odist};
         e = v->launch(mode, a);
         if (e != hipSuccess) return hip_fail(e, "batched kernel launch");
         return WGFFT_OK;
@@ -685,9 +690,26 @@ int wgfft_exec_strided_device(wgfft_t* h, const float* d_in, float* d_out, uint6
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     Plan* p;
     if ((rc = h->get_plan(n, &p))) return rc;
+    hipStream_t s = pick_stream(h, stream);
+    const bool inv = dir == WGFFT_INVERSE;
+    // Padded rows (unit element stride, distances of at least n): the
+    // single-kernel lengths read and write them in place of the contiguous
+    // rows, one pass over memory.  Needs 16-byte aligned rows (even distances)
+    // and either disjoint input and output ranges or the same rows in place.
+    const bool rows_ok = batch == 1 || (idist >= n && odist >= n && ((idist | odist) & 1) == 0);
+    if (istride == 1 && ostride == 1 && p->kind != Plan::kMultiPass && rows_ok &&
+        (((uintptr_t)d_in | (uintptr_t)d_out) & 15u) == 0) {
+        const uint64_t in_end = (uint64_t)(uintptr_t)d_in + ((batch - 1) * idist + n) * sizeof(float2);
+        const uint64_t out_end = (uint64_t)(uintptr_t)d_out + ((batch - 1) * odist + n) * sizeof(float2);
+        const bool disjoint = in_end <= (uint64_t)(uintptr_t)d_out || out_end <= (uint64_t)(uintptr_t)d_in;
+        const bool same = (const void*)d_in == (const void*)d_out && idist == odist;
+        if (disjoint || same)
+            return h->run(p, inv ? kInverse : kForward, (const float2*)d_in, (float2*)d_out, nullptr, batch,
+                          inv ? 1.0f / (float)n : 1.0f, s, nullptr, nullptr, 0, batch == 1 ? n : idist,
+                          batch == 1 ? n : odist);
+    }
     const uint64_t total = n * batch;
     if ((rc = h->ensure(h->io_aux, total * sizeof(float2)))) return rc;
-    hipStream_t s = pick_stream(h, stream);
     ScratchUse su(h, s);
     if (su.e != hipSuccess) return hip_fail(su.e, "stream ordering of the scratch buffers");
     const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 8192);
@@ -695,7 +717,6 @@ int wgfft_exec_strided_device(wgfft_t* h, const float* d_in, float* d_out, uint6
     const uint32_t log2n = (uint32_t)log2u(n);
     hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), 0, s, (const float2*)d_in, aux, log2n, total, istride, idist);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "gather launch");
-    const bool inv = dir == WGFFT_INVERSE;
     if ((rc = h->run(p, inv ? kInverse : kForward, aux, aux, nullptr, batch, inv ? 1.0f / (float)n : 1.0f, s)))
         return rc;
     hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, (const float2*)aux, (float2*)d_out, log2n, total,

@@ -109,7 +109,8 @@ struct Handle {
     // workgroups per CU of a variant (kept in occ_cache when cache is set)
     int blocks_per_cu(const BatchedVariant* v, Mode m, bool cache = true);
     int run(Plan* p, Mode mode, const float2* in, float2* out, const float* win, uint64_t batch, float scale,
-            hipStream_t s, const BatchedVariant* force = nullptr, float2* fixed = nullptr, uint64_t fixed_bytes = 0);
+            hipStream_t s, const BatchedVariant* force = nullptr, float2* fixed = nullptr, uint64_t fixed_bytes = 0,
+            uint64_t idist = 0, uint64_t odist = 0);
     int run_passes(Plan* p, Mode mode, const float2* in, float2* out, const float* win, uint64_t batch, float scale,
                    hipStream_t s, float2* buf_a, float2* buf_b, uint64_t chunk);
     bool host_pipelined(uint64_t n, uint64_t batch) const;

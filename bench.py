@@ -148,6 +148,9 @@ def cpu_standin_pocketfft() -> dict:
 
 
 SETTLE_S = 0.25
+# calls timed per secondary measurement whatever --steps is: their kernels
+# take 20-100 us, so fewer calls would time well under a millisecond
+SECONDARY_STEPS = 200
 
 
 def settle(torch, fn, seconds: float = SETTLE_S) -> None:
@@ -251,7 +254,7 @@ def bench_shapes(torch, gpu, steps: int, tune: bool) -> dict:
         y = torch.empty_like(x)
         if tune:
             gpu.autotune(n, batch)
-        ms = timed(torch, lambda: gpu.exec_device(x, y), max(10, steps // 4), 5)
+        ms = timed(torch, lambda: gpu.exec_device(x, y), max(SECONDARY_STEPS, steps // 4), 5)
         s = n * batch
         gbs = 16.0 * s / (ms * 1e-3) / 1e9
         out[f"{n}x{batch}"] = {"ms": round(ms, 4), "gsamples_s": round(s / (ms * 1e-3) / 1e9, 2),
@@ -299,7 +302,7 @@ def bench_c5(torch, gpu, wgfft, steps: int, tune: bool) -> dict:
     loss = torch.empty(1, dtype=torch.float32, device="cuda")
     if tune:
         gpu.autotune(n, batch)
-    k = max(10, steps // 4)
+    k = max(SECONDARY_STEPS, steps // 4)
     fwd = timed(torch, lambda: gpu.exec_device(x, y), k, 5)
     win = timed(torch, lambda: gpu.exec_windowed_device(x, w, y), k, 5)
     vjp = timed(torch, lambda: gpu.vjp_windowed_device(x, w, gy, gx, gw), k, 5)
@@ -323,7 +326,7 @@ def bench_fused(torch, gpu, x, y, kern_ms: float, steps: int, warmup: int) -> di
     gx = torch.empty_like(x)
     gw = torch.empty(n, dtype=torch.float32, device="cuda")
     loss = torch.empty(1, dtype=torch.float32, device="cuda")
-    k, kw = max(10, steps // 4), max(3, warmup // 4)
+    k, kw = max(SECONDARY_STEPS, steps // 4), max(3, warmup // 4)
     fa_ms = timed(torch, lambda: gpu.fwd_adjoint_masked_device(x, w, m, None, gx, gw, loss), k, kw)
     out = {"fwd_adjoint": {"ms": round(fa_ms, 5), "plain_fwd_ms": round(kern_ms, 5), "ratio": round(fa_ms / kern_ms, 3)}}
     sp = torch.from_numpy(np.hanning(n + 1)[:n].astype(np.float32)).cuda()

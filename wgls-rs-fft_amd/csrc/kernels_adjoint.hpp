@@ -42,7 +42,7 @@ __device__ __forceinline__ void load_real_pairs(float (&r)[E], const float* __re
     }
 }
 
-// Options (OPT bits):
+// Options (OPT bits; kAdjSplit2 marks the split-form kernels below):
 //  kAdjKeepX     hold x in registers from the load to the gw accumulation
 //                (one HBM read, more registers) instead of re-reading it after
 //                the adjoint transform (the workgroup read it microseconds
@@ -51,7 +51,10 @@ __device__ __forceinline__ void load_real_pairs(float (&r)[E], const float* __re
 //                transform waits on a global twiddle load.
 // (A register prefetch of the next transform's input on top of kAdjTwLds was
 // measured slower at N=4096: the extra registers cost a workgroup per CU.)
-constexpr int kAdjKeepX = 1, kAdjTwLds = 2;
+constexpr int kAdjKeepX = 1, kAdjTwLds = 2, kAdjSplit2 = 4;
+// AdjointVariant::roles: the calls a variant is the untuned default for
+// (the first registered variant of a length serves any role none claims).
+constexpr int kRoleFused = 1, kRoleVjp = 2, kRoleFilter = 4;
 
 template <int N, int E, bool FWD, int OPT, int... Rs>
 __global__ void __launch_bounds__(N / E)
@@ -227,6 +230,263 @@ k_filter(FilterArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same two kernels in split form for N = 2H (the structure of k_split2):
+// the forward is a decimation-in-frequency radix-2 stage in registers,
+// y0[n] = a + b, y1[n] = (a - b) W_N^n (a = w x[n], b = w x[n + H]), then
+// the H-point transforms of y0 and y1 in turn through one H-point LDS
+// footprint, giving Y[2k] and Y[2k + 1]; the adjoint is its transpose: the
+// conjugate-twiddle H-point transforms of gY[2k] and gY[2k + 1] give g0, g1,
+// and gu[n] = g0 + conj(W_N^n) g1, gu[n + H] = g0 - conj(W_N^n) g1.  Half
+// the LDS of k_adjoint and H/E threads, so three or four workgroups share a
+// CU.  The gw partials are held in 2E registers across the transforms a
+// workgroup handles, which costs the fused kernel its occupancy (two
+// workgroups per CU at 191 VGPRs); the VJP and the filter keep more.
+// Thread t holds the single-layout slots of both halves: slot i RF + m is
+// index t + T i + m H/RF (RF the first radix of the H-point plan).
+// WAVES: the minimum waves per SIMD the register allocation must allow.
+template <int N, int E, bool FWD, int WAVES, int... Rs>
+__global__ void __launch_bounds__((N / 2) / E) __attribute__((amdgpu_waves_per_eu(WAVES)))
+k_adjoint_split2(AdjointArgs a) {
+    constexpr int H = N / 2;
+    constexpr int T = H / E;
+    using MP = GdMap<H, E, 1, Rs...>;
+    __shared__ __align__(16) float2 lds[LdsLayoutSingle<H, E, Rs...>::len];
+    __shared__ __align__(16) float2 lds_tw[twlds_len(ilog2(N))];
+    __shared__ float red[T];
+    const TwLds twn = twlds_fill<N>(lds_tw, a.tw, threadIdx.x, T);
+    __syncthreads();
+    const TwLds twh = twn.shifted(1);  // W_H for the halves
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    float acc0[E], acc1[E];  // gw at indices n and n + H
+#pragma unroll
+    for (int e = 0; e < E; ++e) acc0[e] = acc1[e] = 0.f;
+    float lacc = 0.f;
+    for (uint64_t tr = blockIdx.x; tr < a.batch; tr += gridDim.x) {
+        const uint32_t t = launder_lane(threadIdx.x);
+        float2 y0[E], y1[E];
+        if constexpr (FWD) {
+            const float2* x = a.x + tr * N;
+            __builtin_amdgcn_s_setprio(2);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t n = MP::row(e, t);
+                const f2v va = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(x + n));
+                const f2v vb = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(x + n + H));
+                y0[e] = make_float2(va.x, va.y);
+                y1[e] = make_float2(vb.x, vb.y);
+            }
+            __builtin_amdgcn_s_setprio(0);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t n = MP::row(e, t);
+                const float2 w = cmul(twn.hi[n >> twn.h], twn.lo[n & ((1u << twn.h) - 1)]);
+                const float2 p = cscale(y0[e], a.w[n]), q = cscale(y1[e], a.w[n + H]);
+                y0[e] = cadd(p, q);
+                y1[e] = cmul(csub(p, q), w);
+            }
+            Engine<H, E, T, false>::run_single(y0, lds, twh, t, Radices<Rs...>{});
+            WGFFT_LDS_SYNC();
+            const uint32_t t1 = launder_lane(t);
+            Engine<H, E, T, false>::run_single(y1, lds, twh, t1, Radices<Rs...>{});
+            __builtin_amdgcn_sched_barrier(0);
+            // Y[2k] = y0, Y[2k + 1] = y1 at k = row(e); gY = 2 m Y, loss += m |Y|^2
+            const float2* m2 = reinterpret_cast<const float2*>(launder(a.mask));
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t k = MP::row(e, t1);
+                if (a.y)
+                    st16<false>(reinterpret_cast<float4*>(a.y + tr * N + 2 * k),
+                                make_float4(y0[e].x, y0[e].y, y1[e].x, y1[e].y));
+                const float2 mv = m2[k];
+                lacc += mv.x * (y0[e].x * y0[e].x + y0[e].y * y0[e].y) + mv.y * (y1[e].x * y1[e].x + y1[e].y * y1[e].y);
+                y0[e] = cscale(y0[e], 2.f * mv.x);
+                y1[e] = cscale(y1[e], 2.f * mv.y);
+            }
+            WGFFT_LDS_SYNC();  // the second half's last exchange read before the adjoint writes
+            __builtin_amdgcn_sched_barrier(0);
+        } else {
+            const float4* g4 = reinterpret_cast<const float4*>(a.gy + tr * N);
+            __builtin_amdgcn_s_setprio(2);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const float4 v = ld16<true>(g4 + MP::row(e, t));
+                y0[e] = make_float2(v.x, v.y);
+                y1[e] = make_float2(v.z, v.w);
+            }
+            __builtin_amdgcn_s_setprio(0);
+        }
+        const uint32_t ta = launder_lane(t);
+        Engine<H, E, T, true>::run_single(y0, lds, twh, ta, Radices<Rs...>{});
+        WGFFT_LDS_SYNC();
+        const uint32_t tb = launder_lane(ta);
+        Engine<H, E, T, true>::run_single(y1, lds, twh, tb, Radices<Rs...>{});
+        __builtin_amdgcn_sched_barrier(0);
+        // gu[n] = g0 + conj(W^n) g1, gu[n + H] = g0 - conj(W^n) g1; gx = w gu;
+        // gw += Re(conj(gu) x), x re-read (it was loaded microseconds ago)
+        const float2* x = launder(a.x) + tr * N;
+        const float* wv = launder(a.w);
+        float2* gx = a.gx + tr * N;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t n = MP::row(e, tb);
+            const float2 w = cmul(twn.hi[n >> twn.h], twn.lo[n & ((1u << twn.h) - 1)]);
+            const float2 g1 = twmul<true>(y1[e], w);
+            const float2 ga = cadd(y0[e], g1), gb = csub(y0[e], g1);
+            const float2 xa = x[n], xb = x[n + H];
+            acc0[e] += ga.x * xa.x + ga.y * xa.y;
+            acc1[e] += gb.x * xb.x + gb.y * xb.y;
+            y0[e] = cscale(ga, wv[n]);
+            y1[e] = cscale(gb, wv[n + H]);
+        }
+        __builtin_amdgcn_s_setprio(2);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t n = MP::row(e, tb);
+            const f2v va = {y0[e].x, y0[e].y}, vb = {y1[e].x, y1[e].y};
+            __builtin_nontemporal_store(va, reinterpret_cast<f2v*>(gx + n));
+            __builtin_nontemporal_store(vb, reinterpret_cast<f2v*>(gx + n + H));
+        }
+        __builtin_amdgcn_s_setprio(0);
+        WGFFT_LDS_SYNC();  // the next transform's first half overwrites the exchange buffer
+    }
+    const uint32_t t = threadIdx.x;
+    float* row = a.part + (uint64_t)blockIdx.x * a.ld;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t n = MP::row(e, t);
+        row[n] = acc0[e];
+        row[n + H] = acc1[e];
+    }
+    if constexpr (FWD) {
+        red[t] = lacc;
+        __syncthreads();
+        if (t == 0) {
+            float s = 0.f;
+            for (int i = 0; i < T; ++i) s += red[i];
+            row[N] = s;
+        }
+    }
+}
+
+// Spectral filter in the same split form: noise (optional), the forward
+// radix-2 stage and halves, s at Y[2k] and Y[2k + 1], the conjugate halves
+// and the transposed radix-2 stage, 1/N at the store.
+template <int N, int E, bool NOISE, int... Rs>
+__global__ void __launch_bounds__((N / 2) / E) __attribute__((amdgpu_waves_per_eu(4)))
+k_filter_split2(FilterArgs a) {
+    constexpr int H = N / 2;
+    constexpr int T = H / E;
+    using MP = GdMap<H, E, 1, Rs...>;
+    __shared__ __align__(16) float2 lds[LdsLayoutSingle<H, E, Rs...>::len];
+    __shared__ __align__(16) float2 lds_tw[twlds_len(ilog2(N))];
+    const TwLds twn = twlds_fill<N>(lds_tw, a.tw, threadIdx.x, T);
+    __syncthreads();
+    const TwLds twh = twn.shifted(1);
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    const float sc = 1.0f / (float)N;
+    for (uint64_t tr = blockIdx.x; tr < a.batch; tr += gridDim.x) {
+        const uint32_t t = launder_lane(threadIdx.x);
+        const float2* x = a.x + tr * N;
+        float2 y0[E], y1[E];
+        __builtin_amdgcn_s_setprio(2);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t n = MP::row(e, t);
+            const f2v va = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(x + n));
+            const f2v vb = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(x + n + H));
+            y0[e] = make_float2(va.x, va.y);
+            y1[e] = make_float2(vb.x, vb.y);
+        }
+        __builtin_amdgcn_s_setprio(0);
+        if constexpr (NOISE) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t n = MP::row(e, t);
+                y0[e] = cadd(y0[e], pipeline_noise(n, (uint32_t)tr, a.seed, a.sigma));
+                y1[e] = cadd(y1[e], pipeline_noise(n + H, (uint32_t)tr, a.seed, a.sigma));
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t n = MP::row(e, t);
+            const float2 w = cmul(twn.hi[n >> twn.h], twn.lo[n & ((1u << twn.h) - 1)]);
+            const float2 p = y0[e], q = y1[e];
+            y0[e] = cadd(p, q);
+            y1[e] = cmul(csub(p, q), w);
+        }
+        Engine<H, E, T, false>::run_single(y0, lds, twh, t, Radices<Rs...>{});
+        WGFFT_LDS_SYNC();
+        const uint32_t t1 = launder_lane(t);
+        Engine<H, E, T, false>::run_single(y1, lds, twh, t1, Radices<Rs...>{});
+        __builtin_amdgcn_sched_barrier(0);
+        const float2* s2 = reinterpret_cast<const float2*>(launder(a.s));
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const float2 sv = s2[MP::row(e, t1)];
+            y0[e] = cscale(y0[e], sv.x);
+            y1[e] = cscale(y1[e], sv.y);
+        }
+        WGFFT_LDS_SYNC();
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t ta = launder_lane(t1);
+        Engine<H, E, T, true>::run_single(y0, lds, twh, ta, Radices<Rs...>{});
+        WGFFT_LDS_SYNC();
+        const uint32_t tb = launder_lane(ta);
+        Engine<H, E, T, true>::run_single(y1, lds, twh, tb, Radices<Rs...>{});
+        float2* o = a.y + tr * N;
+        __builtin_amdgcn_s_setprio(2);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t n = MP::row(e, tb);
+            const float2 w = cmul(twn.hi[n >> twn.h], twn.lo[n & ((1u << twn.h) - 1)]);
+            const float2 g1 = twmul<true>(y1[e], w);
+            const float2 ga = cscale(cadd(y0[e], g1), sc), gb = cscale(csub(y0[e], g1), sc);
+            const f2v va = {ga.x, ga.y}, vb = {gb.x, gb.y};
+            __builtin_nontemporal_store(va, reinterpret_cast<f2v*>(o + n));
+            __builtin_nontemporal_store(vb, reinterpret_cast<f2v*>(o + n + H));
+        }
+        __builtin_amdgcn_s_setprio(0);
+        WGFFT_LDS_SYNC();
+    }
+}
+
+template <int N, int E, int WAVES, int... Rs>
+struct AdjointSplit2Kernel {
+    static constexpr int THREADS = (N / 2) / E;
+    template <bool FWD>
+    static hipError_t go(const AdjointArgs& a) {
+        hipLaunchKernelGGL((k_adjoint_split2<N, E, FWD, WAVES, Rs...>), dim3((unsigned)a.grid), dim3(THREADS),
+                           0, a.stream, a);
+        return hipGetLastError();
+    }
+    static hipError_t launch(bool fwd, const AdjointArgs& a) { return fwd ? go<true>(a) : go<false>(a); }
+    template <bool FWD>
+    static int occ() {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &blocks, (const void*)k_adjoint_split2<N, E, FWD, WAVES, Rs...>, THREADS, 0) != hipSuccess)
+            return 1;
+        return blocks > 0 ? blocks : 1;
+    }
+    static int blocks_per_cu(bool fwd) { return fwd ? occ<true>() : occ<false>(); }
+    static hipError_t filter(const FilterArgs& a) {
+        if (a.sigma != 0.f)
+            hipLaunchKernelGGL((k_filter_split2<N, E, true, Rs...>), dim3((unsigned)a.grid), dim3(THREADS), 0,
+                               a.stream, a);
+        else
+            hipLaunchKernelGGL((k_filter_split2<N, E, false, Rs...>), dim3((unsigned)a.grid), dim3(THREADS), 0,
+                               a.stream, a);
+        return hipGetLastError();
+    }
+    static int filter_blocks_per_cu(bool noise) {
+        int blocks = 0;
+        const void* f = noise ? (const void*)k_filter_split2<N, E, true, Rs...> : (const void*)k_filter_split2<N, E, false, Rs...>;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, THREADS, 0) != hipSuccess) return 1;
+        return blocks > 0 ? blocks : 1;
+    }
+};
+
 template <int N, int E, int OPT, int... Rs>
 struct AdjointKernel {
     static constexpr int THREADS = N / E;
@@ -272,6 +532,7 @@ struct AdjointVariant {
     int (*blocks_per_cu)(bool);
     hipError_t (*filter)(const FilterArgs&);  // the spectral filter with the same plan
     int (*filter_blocks_per_cu)(bool noise);
+    int roles;  // kRole* bits: untuned default for these calls
 };
 
 std::vector<AdjointVariant>& adjoint_registry();
@@ -285,6 +546,16 @@ struct AdjointRegistrar {
         ::wgfft::ilog2(N), E, OPT, #PLAN, &::wgfft::AdjointKernel<N, E, OPT, __VA_ARGS__>::launch,   \
         &::wgfft::AdjointKernel<N, E, OPT, __VA_ARGS__>::blocks_per_cu,                              \
         &::wgfft::AdjointKernel<N, E, OPT, __VA_ARGS__>::filter,                                     \
-        &::wgfft::AdjointKernel<N, E, OPT, __VA_ARGS__>::filter_blocks_per_cu})
+        &::wgfft::AdjointKernel<N, E, OPT, __VA_ARGS__>::filter_blocks_per_cu, 0})
+
+// Split-form variant of length N; PLAN is the radix split of N/2 (first
+// radix = last radix, the single layout).
+#define WGFFT_REGISTER_ADJOINT_SPLIT2(N, E, WAVES, ROLES, PLAN, ...)                                        \
+    static ::wgfft::AdjointRegistrar _wgfft_adjs2_##N##_##E##_##WAVES##_##PLAN(::wgfft::AdjointVariant{       \
+        ::wgfft::ilog2(N), E, ::wgfft::kAdjSplit2 | ((WAVES) << 8), "2x" #PLAN,                               \
+        &::wgfft::AdjointSplit2Kernel<N, E, WAVES, __VA_ARGS__>::launch,                                     \
+        &::wgfft::AdjointSplit2Kernel<N, E, WAVES, __VA_ARGS__>::blocks_per_cu,                              \
+        &::wgfft::AdjointSplit2Kernel<N, E, WAVES, __VA_ARGS__>::filter,                                     \
+        &::wgfft::AdjointSplit2Kernel<N, E, WAVES, __VA_ARGS__>::filter_blocks_per_cu, ROLES})
 
 }  // namespace wgfft

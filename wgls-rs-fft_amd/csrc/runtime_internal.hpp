@@ -53,7 +53,10 @@ struct HostPipe;
 void ragged_destroy_all(Handle* h);
 struct AdjointVariant;
 void load_tune_file(Handle* h);
-const AdjointVariant* find_adjoint(int log2n);
+const AdjointVariant* find_adjoint(int log2n, int role);
+// The handle's fused kernel for `role` (kRole*): its tuned or pinned choice,
+// else the registry default.
+const AdjointVariant* pick_adjoint(Handle* h, int log2n, int role);
 const std::vector<const AdjointVariant*> adjoint_variants(int log2n);
 int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x, const float* d_w,
                   const float* d_mask, const float* d_gy, float* d_y, float* d_gx, float* d_gw, float* d_loss,
@@ -76,7 +79,9 @@ struct Handle {
     std::mutex mu;
     std::map<uint64_t, Plan> plans;
     std::map<int, const BatchedVariant*> tuned;  // log2n -> autotuned variant
-    std::map<int, const AdjointVariant*> tuned_adj;
+    std::map<int, const AdjointVariant*> tuned_adj;     // fused forward + adjoint
+    std::map<int, const AdjointVariant*> tuned_vjp;     // VJP
+    std::map<int, const AdjointVariant*> tuned_filter;  // spectral filter
     std::map<std::pair<const BatchedVariant*, int>, int> occ_cache;
     std::vector<DevBuf> tables;
     DevBuf io_in, io_out, io_w, io_aux, scratch, scratch2, red, red2, lossbuf;

@@ -304,7 +304,8 @@ extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc
         }
     }
     if (!best) return fail(first_err ? first_err : WGFFT_ERR_INVALID_LENGTH, "autotune: no variant ran (" + first_msg + ")");
-    // fused forward + adjoint variants for this length
+    // fused kernels for this length, timed per role: the fused forward +
+    // adjoint, the VJP and the spectral filter each keep their own fastest
     std::string adj_desc;
     const auto adjs = adjoint_variants(p->log2n);
     if (adjs.size() > 1) {
@@ -314,34 +315,59 @@ extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc
         float* m = w + n;
         float* loss = m + n;
         float* gw = (float*)h->io_out.p;  // scratch: gx goes to io_aux
-        const AdjointVariant* best_adj = nullptr;
-        float best_adj_ms = 1e30f;
         const AdjointVariant* av = adjs[0];
-        auto alaunch = [&] {
+        auto fused_launch = [&] {
             return fused_adjoint(h, av, true, (const float*)h->io_in.p, w, m, nullptr, nullptr, (float*)h->io_aux.p, gw,
                                  loss, n, batch, p->tw, h->stream);
         };
-        // two interleaved sweeps, each variant keeping its faster time: one
-        // sweep's order and clock drift picked a variant 8% slower than the
-        // default now and then; a variant that fails to launch keeps 1e30
-        std::vector<float> adj_ms(adjs.size(), 1e30f);
-        for (int sweep = 0; sweep < 2; ++sweep)
-            for (size_t i = 0; i < adjs.size(); ++i) {
-                av = adjs[sweep ? adjs.size() - 1 - i : i];
-                float ms;
-                if (tm.time(alaunch, &ms) != WGFFT_OK) continue;
-                const size_t j = sweep ? adjs.size() - 1 - i : i;
-                adj_ms[j] = std::min(adj_ms[j], ms);
-            }
-        for (size_t j = 0; j < adjs.size(); ++j)
-            if (adj_ms[j] < 1e30f && adj_ms[j] < best_adj_ms) {
-                best_adj_ms = adj_ms[j];
-                best_adj = adjs[j];
-            }
-        if (best_adj) {
-            h->tuned_adj[p->log2n] = best_adj;
-            char buf[128];
-            adj_desc = std::string("; ") + adjoint_describe(best_adj, buf, sizeof buf);
+        auto vjp_launch = [&] {  // gY: the noise batch itself
+            return fused_adjoint(h, av, false, (const float*)h->io_in.p, w, nullptr, (const float*)h->io_in.p, nullptr,
+                                 (float*)h->io_aux.p, gw, nullptr, n, batch, p->tw, h->stream);
+        };
+        auto filter_launch = [&] {
+            int grid = av->filter_blocks_per_cu(false) * h->num_cus;
+            if ((uint64_t)grid > batch) grid = (int)batch;
+            FilterArgs fa{(const float2*)h->io_in.p, w, (float2*)h->io_aux.p, p->tw, batch, grid, h->stream, 0u, 0.f};
+            const hipError_t e = av->filter(fa);
+            return e == hipSuccess ? WGFFT_OK : hip_fail(e, "filter kernel launch");
+        };
+        const int roles[3] = {kRoleFused, kRoleVjp, kRoleFilter};
+        const AdjointVariant* best_role[3] = {nullptr, nullptr, nullptr};
+        for (int r = 0; r < 3; ++r) {
+            // two interleaved sweeps, each variant keeping its faster time: one
+            // sweep's order and clock drift picked a variant 8% slower than the
+            // default now and then; a variant that fails to launch keeps 1e30
+            std::vector<float> adj_ms(adjs.size(), 1e30f);
+            for (int sweep = 0; sweep < 2; ++sweep)
+                for (size_t i = 0; i < adjs.size(); ++i) {
+                    const size_t j = sweep ? adjs.size() - 1 - i : i;
+                    av = adjs[j];
+                    float ms;
+                    const int trc = roles[r] == kRoleFused ? tm.time(fused_launch, &ms)
+                                    : roles[r] == kRoleVjp ? tm.time(vjp_launch, &ms)
+                                                           : tm.time(filter_launch, &ms);
+                    if (trc != WGFFT_OK) continue;
+                    adj_ms[j] = std::min(adj_ms[j], ms);
+                }
+            float best_ms_r = 1e30f;
+            for (size_t j = 0; j < adjs.size(); ++j)
+                if (adj_ms[j] < best_ms_r) {
+                    best_ms_r = adj_ms[j];
+                    best_role[r] = adjs[j];
+                }
+        }
+        char buf[128];
+        if (best_role[0]) {
+            h->tuned_adj[p->log2n] = best_role[0];
+            adj_desc = std::string("; ") + adjoint_describe(best_role[0], buf, sizeof buf);
+        }
+        if (best_role[1]) {
+            h->tuned_vjp[p->log2n] = best_role[1];
+            adj_desc += std::string("; vjp ") + best_role[1]->plan + ((best_role[1]->opt & kAdjSplit2) ? " split2" : "");
+        }
+        if (best_role[2]) {
+            h->tuned_filter[p->log2n] = best_role[2];
+            adj_desc += std::string("; filter ") + best_role[2]->plan + ((best_role[2]->opt & kAdjSplit2) ? " split2" : "");
         }
     }
     if (best) {

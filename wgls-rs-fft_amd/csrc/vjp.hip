@@ -24,10 +24,20 @@ std::vector<AdjointVariant>& adjoint_registry() {
     return v;
 }
 
-const AdjointVariant* find_adjoint(int log2n) {
+// The registry default of a length for a role: the first variant claiming
+// the role, else the first registered.
+const AdjointVariant* find_adjoint(int log2n, int role) {
+    for (const auto& v : adjoint_registry())
+        if (v.log2n == log2n && (v.roles & role)) return &v;
     for (const auto& v : adjoint_registry())
         if (v.log2n == log2n) return &v;
     return nullptr;
+}
+
+const AdjointVariant* pick_adjoint(Handle* h, int log2n, int role) {
+    auto& m = role == kRoleVjp ? h->tuned_vjp : role == kRoleFilter ? h->tuned_filter : h->tuned_adj;
+    auto it = m.find(log2n);
+    return it != m.end() ? it->second : find_adjoint(log2n, role);
 }
 
 const std::vector<const AdjointVariant*> adjoint_variants(int log2n) {
@@ -39,8 +49,10 @@ const std::vector<const AdjointVariant*> adjoint_variants(int log2n) {
 
 const char* adjoint_describe(const AdjointVariant* v, char* buf, uint64_t len) {
     if (buf && len)
-        std::snprintf(buf, len, "fused N=%d plan=%s E=%d keep_x=%d%s", 1 << v->log2n, v->plan, v->E,
-                      (v->opt & kAdjKeepX) ? 1 : 0, (v->opt & kAdjTwLds) ? " tw-lds" : "");
+        std::snprintf(buf, len, "fused N=%d plan=%s E=%d keep_x=%d%s%s%s", 1 << v->log2n, v->plan, v->E,
+                      (v->opt & kAdjKeepX) ? 1 : 0, (v->opt & kAdjTwLds) ? " tw-lds" : "",
+                      (v->opt & kAdjSplit2) ? " split2" : "",
+                      (v->opt >> 8) ? (" waves>=" + std::to_string(v->opt >> 8)).c_str() : "");
     return buf;
 }
 
@@ -202,20 +214,21 @@ static int vjp_finish(Handle* h, const float* d_x, const float* d_w, float* d_gx
 int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x, const float* d_w,
                          const float* d_mask, const float* d_gy, float* d_y, float* d_gx, float* d_gw,
                          float* d_loss, uint64_t n, uint64_t batch, const float2* tw, hipStream_t s) {
+    // partial matrix: one row of n + 4 floats per workgroup (gw partials,
+    // then the loss partial in column n; rows kept 16-byte aligned)
+    const uint64_t ld = n + 4;
     int grid = v->blocks_per_cu(fwd) * h->num_cus;
     if ((uint64_t)grid > batch) grid = (int)batch;
+    const uint64_t rows = (uint64_t)grid;
     int rc;
-    // partial matrix: grid rows of n + 4 floats (gw partials, then the loss
-    // partial in column n; rows kept 16-byte aligned)
-    const uint64_t ld = n + 4;
-    if ((rc = h->ensure(h->red, (uint64_t)grid * ld * sizeof(float)))) return rc;
+    if ((rc = h->ensure(h->red, rows * ld * sizeof(float)))) return rc;
     AdjointArgs a{(const float2*)d_x, d_w, d_mask, (const float2*)d_gy, (float2*)d_y, (float2*)d_gx, tw,
                   (float*)h->red.p, ld, batch, grid, s};
     hipError_t e = v->launch(fwd, a);
     if (e != hipSuccess) return hip_fail(e, "adjoint kernel launch");
     const uint64_t ncols = fwd ? n + 1 : n;
     hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)((ncols + 63) / 64)), dim3(64 * kRowLanes), 0, s,
-                       (const float*)h->red.p, ld, (uint64_t)grid, n, d_gw, fwd ? d_loss : nullptr);
+                       (const float*)h->red.p, ld, rows, n, d_gw, fwd ? d_loss : nullptr);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "adjoint reduction launch");
     return WGFFT_OK;
@@ -276,7 +289,7 @@ extern "C" int wgfft_vjp_windowed_device(wgfft_t* h, const float* d_x, const flo
     Plan* p;
     if ((rc = h->get_plan(n, &p))) return rc;
     if (p->kind == Plan::kBatched) {
-        const AdjointVariant* v = h->tuned_adj.count(p->log2n) ? h->tuned_adj[p->log2n] : find_adjoint(p->log2n);
+        const AdjointVariant* v = pick_adjoint(h, p->log2n, kRoleVjp);
         if (v)
             return fused_adjoint(h, v, false, d_x, d_w, nullptr, d_gy, nullptr, d_gx, d_gw, nullptr, n, batch, p->tw,
                                  s);
@@ -309,7 +322,7 @@ extern "C" int wgfft_fwd_adjoint_masked_device(wgfft_t* h, const float* d_x, con
     Plan* p;
     if ((rc = h->get_plan(n, &p))) return rc;
     if (p->kind == Plan::kBatched) {
-        const AdjointVariant* v = h->tuned_adj.count(p->log2n) ? h->tuned_adj[p->log2n] : find_adjoint(p->log2n);
+        const AdjointVariant* v = pick_adjoint(h, p->log2n, kRoleFused);
         if (v)
             return fused_adjoint(h, v, true, d_x, d_w, d_mask, nullptr, d_y, d_gx, d_gw, d_loss, n, batch, p->tw, s);
     }
@@ -346,7 +359,7 @@ extern "C" int wgfft_select_adjoint_variant(wgfft_t* h, uint64_t n, int index, c
     const auto vs = adjoint_variants(log2n);
     if (index < 0 || index >= (int)vs.size()) return fail(WGFFT_ERR_INVALID_ARGUMENT, "variant index out of range");
     std::lock_guard<std::mutex> lk(h->mu);
-    h->tuned_adj[log2n] = vs[(size_t)index];
+    h->tuned_adj[log2n] = h->tuned_vjp[log2n] = h->tuned_filter[log2n] = vs[(size_t)index];
     adjoint_describe(vs[(size_t)index], desc, desc_len);
     return WGFFT_OK;
 }
@@ -371,7 +384,7 @@ extern "C" int wgfft_filter_device(wgfft_t* h, const float* d_x, const float* d_
     Plan* p;
     if ((rc = h->get_plan(n, &p))) return rc;
     if (p->kind == Plan::kBatched) {
-        const AdjointVariant* v = h->tuned_adj.count(p->log2n) ? h->tuned_adj[p->log2n] : find_adjoint(p->log2n);
+        const AdjointVariant* v = pick_adjoint(h, p->log2n, kRoleFilter);
         if (v) {
             int grid = v->filter_blocks_per_cu(sigma != 0.f) * h->num_cus;
             if ((uint64_t)grid > batch) grid = (int)batch;

@@ -156,6 +156,16 @@ def test_every_fused_variant(torch_cuda, log2n):
             e_gx = oracle.rel_l2(gx.cpu().numpy(), gx64)
             e_gw = np.linalg.norm(gw.cpu().numpy() - gw64) / np.linalg.norm(gw64)
             assert e_l <= 1e-5 and e_y <= 5e-6 and e_gx <= 1e-5 and e_gw <= 1e-5, (desc, e_l, e_y, e_gx, e_gw)
+            # without Y (its own instantiation in the split-form kernels)
+            gx.zero_()
+            gw.zero_()
+            loss.zero_()
+            g.fwd_adjoint_masked_device(tx, tw_, tm, None, gx, gw, loss)
+            torch.cuda.synchronize()
+            e_l = abs(float(loss.item()) - loss64) / loss64
+            e_gx = oracle.rel_l2(gx.cpu().numpy(), gx64)
+            e_gw = np.linalg.norm(gw.cpu().numpy() - gw64) / np.linalg.norm(gw64)
+            assert e_l <= 1e-5 and e_gx <= 1e-5 and e_gw <= 1e-5, (desc, "no y", e_l, e_gx, e_gw)
             gx.zero_()
             gw.zero_()
             g.vjp_windowed_device(tx, tw_, tgy, gx, gw)
@@ -165,3 +175,4 @@ def test_every_fused_variant(torch_cuda, log2n):
             assert e_gx <= 5e-6 and e_gw <= 1e-5, (desc, "vjp", e_gx, e_gw)
         finally:
             g.close()
+

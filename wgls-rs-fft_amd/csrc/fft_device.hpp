@@ -19,6 +19,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+#include <utility>
+
 namespace wgfft {
 
 // LDS barrier that does not wait for outstanding global loads: only LDS
@@ -46,6 +49,47 @@ __device__ __forceinline__ P* launder(P* p) {
 __device__ __forceinline__ uint32_t launder_lane(uint32_t v) {
     asm volatile("" : "+v"(v));
     return v;
+}
+
+// Compile-time unrolled loop: f(std::integral_constant<int, I>{}) for I < N,
+// so the body can use I in constant expressions.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// exp(-2 pi i j / M) rounded to f32, evaluated at compile time (Taylor
+// series in double after reducing the angle to [-pi, pi]), so multiplies by
+// fixed roots of unity take immediate operands instead of table loads.
+struct CplxF {
+    float x, y;
+};
+constexpr double kPiD = 3.14159265358979323846264338327950288;
+constexpr double cx_sin(double x) {
+    double t = x, s = x;
+    for (int k = 1; k < 26; ++k) {
+        t *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+        s += t;
+    }
+    return s;
+}
+constexpr double cx_cos(double x) {
+    double t = 1.0, s = 1.0;
+    for (int k = 1; k < 26; ++k) {
+        t *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+        s += t;
+    }
+    return s;
+}
+constexpr CplxF cx_root(long long M, long long j) {
+    const long long r = ((j % M) + M) % M;
+    double a = -2.0 * kPiD * (double)r / (double)M;
+    if (a < -kPiD) a += 2.0 * kPiD;
+    return CplxF{(float)cx_cos(a), (float)cx_sin(a)};
 }
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }

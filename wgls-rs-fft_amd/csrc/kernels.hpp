@@ -539,6 +539,8 @@ struct GdMap {
     __device__ __forceinline__ static uint32_t row(int s, uint32_t t) {
         return t + T * (s / RF) + (s % RF) * (R / RF);
     }
+    // row(s, t) - t: the fixed part of the index of slot s
+    static constexpr int offset(int s) { return T * (s / RF) + (s % RF) * (R / RF); }
 };
 
 // LDS slot (complex elements) per column of a direct global pass: the

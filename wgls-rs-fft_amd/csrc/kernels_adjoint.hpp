@@ -277,14 +277,16 @@ k_adjoint_split2(AdjointArgs a) {
                 y1[e] = make_float2(vb.x, vb.y);
             }
             __builtin_amdgcn_s_setprio(0);
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
+            // W_N^n with n = t + offset(e): W_N^t times a compile-time root
+            const float2 wt = cmul(twn.hi[t >> twn.h], twn.lo[t & ((1u << twn.h) - 1)]);
+            static_for<E>([&](auto ec) {
+                constexpr int e = decltype(ec)::value;
+                constexpr CplxF c = cx_root(N, MP::offset(e));
                 const uint32_t n = MP::row(e, t);
-                const float2 w = cmul(twn.hi[n >> twn.h], twn.lo[n & ((1u << twn.h) - 1)]);
                 const float2 p = cscale(y0[e], a.w[n]), q = cscale(y1[e], a.w[n + H]);
                 y0[e] = cadd(p, q);
-                y1[e] = cmul(csub(p, q), w);
-            }
+                y1[e] = cmul(csub(p, q), cmul(wt, make_float2(c.x, c.y)));
+            });
             Engine<H, E, T, false>::run_single(y0, lds, twh, t, Radices<Rs...>{});
             WGFFT_LDS_SYNC();
             const uint32_t t1 = launder_lane(t);
@@ -327,18 +329,19 @@ k_adjoint_split2(AdjointArgs a) {
         const float2* x = launder(a.x) + tr * N;
         const float* wv = launder(a.w);
         float2* gx = a.gx + tr * N;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
+        const float2 wtb = cmul(twn.hi[tb >> twn.h], twn.lo[tb & ((1u << twn.h) - 1)]);
+        static_for<E>([&](auto ec) {
+            constexpr int e = decltype(ec)::value;
+            constexpr CplxF c = cx_root(N, MP::offset(e));
             const uint32_t n = MP::row(e, tb);
-            const float2 w = cmul(twn.hi[n >> twn.h], twn.lo[n & ((1u << twn.h) - 1)]);
-            const float2 g1 = twmul<true>(y1[e], w);
+            const float2 g1 = twmul<true>(y1[e], cmul(wtb, make_float2(c.x, c.y)));
             const float2 ga = cadd(y0[e], g1), gb = csub(y0[e], g1);
             const float2 xa = x[n], xb = x[n + H];
             acc0[e] += ga.x * xa.x + ga.y * xa.y;
             acc1[e] += gb.x * xb.x + gb.y * xb.y;
             y0[e] = cscale(ga, wv[n]);
             y1[e] = cscale(gb, wv[n + H]);
-        }
+        });
         __builtin_amdgcn_s_setprio(2);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -407,14 +410,14 @@ k_filter_split2(FilterArgs a) {
                 y1[e] = cadd(y1[e], pipeline_noise(n + H, (uint32_t)tr, a.seed, a.sigma));
             }
         }
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const uint32_t n = MP::row(e, t);
-            const float2 w = cmul(twn.hi[n >> twn.h], twn.lo[n & ((1u << twn.h) - 1)]);
+        const float2 wt = cmul(twn.hi[t >> twn.h], twn.lo[t & ((1u << twn.h) - 1)]);
+        static_for<E>([&](auto ec) {
+            constexpr int e = decltype(ec)::value;
+            constexpr CplxF c = cx_root(N, MP::offset(e));
             const float2 p = y0[e], q = y1[e];
             y0[e] = cadd(p, q);
-            y1[e] = cmul(csub(p, q), w);
-        }
+            y1[e] = cmul(csub(p, q), cmul(wt, make_float2(c.x, c.y)));
+        });
         Engine<H, E, T, false>::run_single(y0, lds, twh, t, Radices<Rs...>{});
         WGFFT_LDS_SYNC();
         const uint32_t t1 = launder_lane(t);
@@ -436,16 +439,17 @@ k_filter_split2(FilterArgs a) {
         Engine<H, E, T, true>::run_single(y1, lds, twh, tb, Radices<Rs...>{});
         float2* o = a.y + tr * N;
         __builtin_amdgcn_s_setprio(2);
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
+        const float2 wtb = cmul(twn.hi[tb >> twn.h], twn.lo[tb & ((1u << twn.h) - 1)]);
+        static_for<E>([&](auto ec) {
+            constexpr int e = decltype(ec)::value;
+            constexpr CplxF c = cx_root(N, MP::offset(e));
             const uint32_t n = MP::row(e, tb);
-            const float2 w = cmul(twn.hi[n >> twn.h], twn.lo[n & ((1u << twn.h) - 1)]);
-            const float2 g1 = twmul<true>(y1[e], w);
+            const float2 g1 = twmul<true>(y1[e], cmul(wtb, make_float2(c.x, c.y)));
             const float2 ga = cscale(cadd(y0[e], g1), sc), gb = cscale(csub(y0[e], g1), sc);
             const f2v va = {ga.x, ga.y}, vb = {gb.x, gb.y};
             __builtin_nontemporal_store(va, reinterpret_cast<f2v*>(o + n));
             __builtin_nontemporal_store(vb, reinterpret_cast<f2v*>(o + n + H));
-        }
+        });
         __builtin_amdgcn_s_setprio(0);
         WGFFT_LDS_SYNC();
     }
@@ -456,8 +460,8 @@ struct AdjointSplit2Kernel {
     static constexpr int THREADS = (N / 2) / E;
     template <bool FWD>
     static hipError_t go(const AdjointArgs& a) {
-        hipLaunchKernelGGL((k_adjoint_split2<N, E, FWD, WAVES, Rs...>), dim3((unsigned)a.grid), dim3(THREADS),
-                           0, a.stream, a);
+        hipLaunchKernelGGL((k_adjoint_split2<N, E, FWD, WAVES, Rs...>), dim3((unsigned)a.grid), dim3(THREADS), 0,
+                           a.stream, a);
         return hipGetLastError();
     }
     static hipError_t launch(bool fwd, const AdjointArgs& a) { return fwd ? go<true>(a) : go<false>(a); }

@@ -30,8 +30,10 @@ WGFFT_REGISTER_ADJOINT(16384, 32, 0, 16x16x16x4, 16, 16, 16, 4);
 // Split form (a radix-2 stage and two half-length transforms through a
 // half-length LDS footprint), the untuned default of the spectral filter at
 // both lengths (8192 x 1024: 31.9 against 47.5 us; 16384 x 512: 39.6
-// against 48.7 us) and of the VJP at 8192 (44.1 against 49.9 us).  As the
-// fused forward + adjoint it holds 191 VGPRs for the gw partials (two
-// workgroups per CU) and measured no faster than the first registration.
+// against 48.7 us) and of the VJP at 8192 (kernel 35.6 us).  As the fused
+// forward + adjoint it holds 188 VGPRs for the gw partials (two workgroups
+// per CU) and measured no faster than the first registration; three or four
+// workgroups per CU (capped registers, or per-transform partial rows
+// instead of the gw registers) measured slower still (77-125 us).
 WGFFT_REGISTER_ADJOINT_SPLIT2(8192, 16, 2, ::wgfft::kRoleVjp | ::wgfft::kRoleFilter, 16x16x16, 16, 16, 16);
 WGFFT_REGISTER_ADJOINT_SPLIT2(16384, 16, 2, ::wgfft::kRoleFilter, 16x8x4x16, 16, 8, 4, 16);

@@ -141,27 +141,30 @@ __global__ void __launch_bounds__(256) k_sum_all(const float* __restrict__ part,
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
-// Column sums of the rows x ld partial matrix in one launch: a block is 64
-// columns x 16 row lanes, each lane striding the rows with eight loads in
-// flight; lanes combine in a fixed order through LDS (no cross-block
-// ordering, so no device-scope fences).  Columns < n go to gw; when
-// loss != NULL, column n holds per-row loss partials summed into loss[0].
-constexpr int kRowLanes = 16;
+// Column sums of the rows x ld partial matrix in one launch: a block is
+// kRedCols columns x kRowLanes row lanes (128-byte row segments, about 256
+// blocks at N = 8192), each lane striding the rows with 16 loads in flight;
+// lanes combine in a fixed order through LDS (no cross-block ordering, so no
+// device-scope fences).  Columns < n go to gw; when loss != NULL, column n
+// holds per-row loss partials summed into loss[0].
+constexpr int kRedCols = 32, kRowLanes = 32;
 
-__global__ void __launch_bounds__(64 * kRowLanes)
+__global__ void __launch_bounds__(kRedCols * kRowLanes)
 k_reduce_cols(const float* __restrict__ part, uint64_t ld, uint64_t rows, uint64_t n, float* __restrict__ gw,
               float* __restrict__ loss) {
-    __shared__ float red[kRowLanes][64];
-    const uint32_t lane = threadIdx.x & 63, rl = threadIdx.x >> 6;
-    const uint64_t col = (uint64_t)blockIdx.x * 64 + lane;
+    __shared__ float red[kRowLanes][kRedCols];
+    const uint32_t lane = threadIdx.x % kRedCols, rl = threadIdx.x / kRedCols;
+    const uint64_t col = (uint64_t)blockIdx.x * kRedCols + lane;
     const uint64_t ncols = loss ? n + 1 : n;
     float acc = 0.f;
     if (col < ncols) {
         uint64_t r = rl;
-        for (; r + 7 * kRowLanes < rows; r += 8 * kRowLanes) {
-            float v[8];
+        for (; r + 15 * kRowLanes < rows; r += 16 * kRowLanes) {
+            float v[16];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) v[i] = part[(r + (uint64_t)kRowLanes * i) * ld + col];
+            for (int i = 0; i < 16; ++i) v[i] = part[(r + (uint64_t)kRowLanes * i) * ld + col];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] += v[i + 8];
             acc += ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
         }
         for (; r < rows; r += kRowLanes) acc += part[r * ld + col];
@@ -227,8 +230,9 @@ int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x
     hipError_t e = v->launch(fwd, a);
     if (e != hipSuccess) return hip_fail(e, "adjoint kernel launch");
     const uint64_t ncols = fwd ? n + 1 : n;
-    hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)((ncols + 63) / 64)), dim3(64 * kRowLanes), 0, s,
-                       (const float*)h->red.p, ld, rows, n, d_gw, fwd ? d_loss : nullptr);
+    hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)((ncols + kRedCols - 1) / kRedCols)),
+                       dim3(kRedCols * kRowLanes), 0, s, (const float*)h->red.p, ld, rows, n, d_gw,
+                       fwd ? d_loss : nullptr);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "adjoint reduction launch");
     return WGFFT_OK;

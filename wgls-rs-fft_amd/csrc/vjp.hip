@@ -142,36 +142,36 @@ __global__ void __launch_bounds__(256) k_sum_all(const float* __restrict__ part,
 }
 
 // Column sums of the rows x ld partial matrix in one launch: a block is
-// kRedCols columns x kRowLanes row lanes (128-byte row segments, about 256
-// blocks at N = 8192), each lane striding the rows with 16 loads in flight;
-// lanes combine in a fixed order through LDS (no cross-block ordering, so no
-// device-scope fences).  Columns < n go to gw; when loss != NULL, column n
-// holds per-row loss partials summed into loss[0].
-constexpr int kRedCols = 32, kRowLanes = 32;
-
-__global__ void __launch_bounds__(kRedCols * kRowLanes)
+// COLS columns x 1024 / COLS row lanes, each lane striding the rows with
+// eight loads in flight; lanes combine in a fixed order through LDS (no
+// cross-block ordering, so no device-scope fences).  Columns < n go to gw;
+// when loss != NULL, column n holds per-row loss partials summed into
+// loss[0].  Measured on partial matrices of 256-1024 rows: 32 columns (257
+// blocks at N = 8192) for N <= 4096 (768 x 4100: 3.1 against 3.8 us with
+// 64), 64 columns above (512 x 8196: 4.1 against 5.1 us with 32).
+template <int COLS>
+__global__ void __launch_bounds__(1024)
 k_reduce_cols(const float* __restrict__ part, uint64_t ld, uint64_t rows, uint64_t n, float* __restrict__ gw,
               float* __restrict__ loss) {
-    __shared__ float red[kRowLanes][kRedCols];
-    const uint32_t lane = threadIdx.x % kRedCols, rl = threadIdx.x / kRedCols;
-    const uint64_t col = (uint64_t)blockIdx.x * kRedCols + lane;
+    constexpr int LANES = 1024 / COLS;
+    __shared__ float red[LANES][COLS];
+    const uint32_t lane = threadIdx.x % COLS, rl = threadIdx.x / COLS;
+    const uint64_t col = (uint64_t)blockIdx.x * COLS + lane;
     const uint64_t ncols = loss ? n + 1 : n;
     float acc = 0.f;
     if (col < ncols) {
         uint64_t r = rl;
-        for (; r + 15 * kRowLanes < rows; r += 16 * kRowLanes) {
-            float v[16];
+        for (; r + 7 * LANES < rows; r += 8 * LANES) {
+            float v[8];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) v[i] = part[(r + (uint64_t)kRowLanes * i) * ld + col];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) v[i] += v[i + 8];
+            for (int i = 0; i < 8; ++i) v[i] = part[(r + (uint64_t)LANES * i) * ld + col];
             acc += ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
         }
-        for (; r < rows; r += kRowLanes) acc += part[r * ld + col];
+        for (; r < rows; r += LANES) acc += part[r * ld + col];
     }
     red[rl][lane] = acc;
     __syncthreads();
-    for (int w = kRowLanes / 2; w > 0; w >>= 1) {
+    for (int w = LANES / 2; w > 0; w >>= 1) {
         if ((int)rl < w) red[rl][lane] += red[rl + w][lane];
         __syncthreads();
     }
@@ -230,9 +230,12 @@ int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x
     hipError_t e = v->launch(fwd, a);
     if (e != hipSuccess) return hip_fail(e, "adjoint kernel launch");
     const uint64_t ncols = fwd ? n + 1 : n;
-    hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)((ncols + kRedCols - 1) / kRedCols)),
-                       dim3(kRedCols * kRowLanes), 0, s, (const float*)h->red.p, ld, rows, n, d_gw,
-                       fwd ? d_loss : nullptr);
+    if (n <= 4096)
+        hipLaunchKernelGGL(k_reduce_cols<32>, dim3((unsigned)((ncols + 31) / 32)), dim3(1024), 0, s,
+                           (const float*)h->red.p, ld, rows, n, d_gw, fwd ? d_loss : nullptr);
+    else
+        hipLaunchKernelGGL(k_reduce_cols<64>, dim3((unsigned)((ncols + 63) / 64)), dim3(1024), 0, s,
+                           (const float*)h->red.p, ld, rows, n, d_gw, fwd ? d_loss : nullptr);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "adjoint reduction launch");
     return WGFFT_OK;

@@ -20,9 +20,14 @@
  *   internal mutex); distinct handles are independent.  Device calls of one
  *   handle on different streams may run concurrently, except that those using
  *   the handle's scratch buffers (global passes for n > 16384, strided
- *   layouts, VJP reductions) are ordered after one another by an event.  A
- *   mixed-length plan has one scratch area of its own: its calls on
- *   different streams must be ordered by the caller.
+ *   layouts, VJP reductions) are ordered after one another: while they all
+ *   use one stream, by that stream; the first such call on a second stream
+ *   waits once for the device (hipDeviceSynchronize; inside a stream capture
+ *   it fails instead, so make one call on a new stream before capturing on
+ *   it) and from then on each such call records an event that the next one
+ *   on another stream waits for.  A mixed-length plan has one scratch area
+ *   of its own: its calls on different streams must be ordered by the
+ *   caller.
  */
 #ifndef WGFFT_H
 #define WGFFT_H

@@ -56,23 +56,53 @@ static int log2u(uint64_t n) {
 }
 
 hipError_t Handle::scratch_begin(hipStream_t st) {
-    if (!scratch_recorded || st == scratch_stream) return hipSuccess;  // one stream orders its own work
-    return hipStreamWaitEvent(st, scratch_ev, 0);
+    if (!scratch_pending || st == scratch_stream) return hipSuccess;  // one stream orders its own work
+    if (scratch_recorded) return hipStreamWaitEvent(st, scratch_ev, 0);
+    // the other stream's last use has no event after it: wait for the device
+    // once (not possible inside a capture), then record after every use
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+        return hipErrorStreamCaptureUnsupported;
+    const hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return e;
+    scratch_pending = false;
+    scratch_events = true;
+    return hipSuccess;
 }
 
+// The event only orders device work between streams of this device, so it
+// releases at device scope (the default system-scope fence adds ~1 us of GPU
+// idle time per record).
 void Handle::scratch_end(hipStream_t st) {
-    if (!scratch_ev && hipEventCreateWithFlags(&scratch_ev, hipEventDisableTiming) != hipSuccess) {
+    scratch_stream = st;
+    scratch_pending = true;
+    scratch_recorded = false;
+    if (!scratch_events) return;
+    if (!scratch_ev &&
+        hipEventCreateWithFlags(&scratch_ev, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess) {
         (void)hipGetLastError();
         scratch_ev = nullptr;
         return;
     }
     if (hipEventRecord(scratch_ev, st) != hipSuccess) {
         (void)hipGetLastError();
-        scratch_recorded = false;
         return;
     }
-    scratch_stream = st;
     scratch_recorded = true;
+}
+
+hipError_t Handle::scratch_quiesce() {
+    if (!scratch_pending || scratch_stream == stream || scratch_recorded) return hipSuccess;
+    const hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) scratch_pending = false;
+    return e;
+}
+
+void Handle::scratch_settled(hipStream_t st) {
+    if (scratch_pending && st == scratch_stream) {
+        scratch_pending = false;
+        scratch_recorded = false;
+    }
 }
 
 int Handle::ensure(DevBuf& b, uint64_t bytes) {
@@ -734,15 +764,19 @@ int wgfft_exec_host(wgfft_t* h, const float* in, float* out, uint64_t n, uint64_
     std::lock_guard<std::mutex> lk(h->mu);
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    if ((e = h->scratch_quiesce()) != hipSuccess) return hip_fail(e, "stream ordering of the scratch buffers");
     const uint64_t bytes = n * batch * sizeof(float2);
     Plan* p;
     if ((rc = h->get_plan(n, &p))) return rc;
     bool inv = dir == WGFFT_INVERSE;
-    if (h->host_pipelined(n, batch))
-        return h->run_host(in, out, n, batch, [&](const float2* d_in, float2* d_out, uint64_t b) {
+    if (h->host_pipelined(n, batch)) {
+        rc = h->run_host(in, out, n, batch, [&](const float2* d_in, float2* d_out, uint64_t b) {
             return h->run(p, inv ? kInverse : kForward, d_in, d_out, nullptr, b, inv ? 1.0f / (float)n : 1.0f,
                           h->stream);
         });
+        if (rc == WGFFT_OK) h->scratch_settled(h->stream);
+        return rc;
+    }
     if ((rc = h->ensure(h->io_in, bytes)) || (rc = h->ensure(h->io_out, bytes))) return rc;
     if ((e = hipMemcpyAsync(h->io_in.p, in, bytes, hipMemcpyHostToDevice, h->stream)) != hipSuccess)
         return hip_fail(e, "hipMemcpyAsync(H2D)");
@@ -752,6 +786,7 @@ int wgfft_exec_host(wgfft_t* h, const float* in, float* out, uint64_t n, uint64_
     if ((e = hipMemcpyAsync(out, h->io_out.p, bytes, hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
         return hip_fail(e, "hipMemcpyAsync(D2H)");
     if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    h->scratch_settled(h->stream);
     return WGFFT_OK;
 }
 
@@ -781,16 +816,20 @@ int wgfft_exec_windowed_host(wgfft_t* h, const float* x, const float* w, float* 
     std::lock_guard<std::mutex> lk(h->mu);
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    if ((e = h->scratch_quiesce()) != hipSuccess) return hip_fail(e, "stream ordering of the scratch buffers");
     const uint64_t bytes = n * batch * sizeof(float2);
     if ((rc = h->ensure(h->io_w, n * sizeof(float) + 16))) return rc;
     Plan* p;
     if ((rc = h->get_plan(n, &p))) return rc;
     if ((e = hipMemcpyAsync(h->io_w.p, w, n * sizeof(float), hipMemcpyHostToDevice, h->stream)) != hipSuccess)
         return hip_fail(e, "hipMemcpyAsync(H2D)");
-    if (h->host_pipelined(n, batch))
-        return h->run_host(x, y, n, batch, [&](const float2* d_in, float2* d_out, uint64_t b) {
+    if (h->host_pipelined(n, batch)) {
+        rc = h->run_host(x, y, n, batch, [&](const float2* d_in, float2* d_out, uint64_t b) {
             return h->run(p, kWindowed, d_in, d_out, (const float*)h->io_w.p, b, 1.0f, h->stream);
         });
+        if (rc == WGFFT_OK) h->scratch_settled(h->stream);
+        return rc;
+    }
     if ((rc = h->ensure(h->io_in, bytes)) || (rc = h->ensure(h->io_out, bytes))) return rc;
     if ((e = hipMemcpyAsync(h->io_in.p, x, bytes, hipMemcpyHostToDevice, h->stream)) != hipSuccess)
         return hip_fail(e, "hipMemcpyAsync(H2D)");
@@ -800,6 +839,7 @@ int wgfft_exec_windowed_host(wgfft_t* h, const float* x, const float* w, float* 
     if ((e = hipMemcpyAsync(y, h->io_out.p, bytes, hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
         return hip_fail(e, "hipMemcpyAsync(D2H)");
     if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    h->scratch_settled(h->stream);
     return WGFFT_OK;
 }
 
@@ -809,6 +849,13 @@ int wgfft_bench_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, 
     if (iters < 1) return fail(WGFFT_ERR_INVALID_ARGUMENT, "iters must be >= 1");
     int rc;
     hipError_t e;
+    {
+        // host-blocking: its calls on the handle's stream are ordered after
+        // any pending use of the scratch on another stream here
+        std::lock_guard<std::mutex> lk(h->mu);
+        if ((e = hipSetDevice(h->device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+        if ((e = h->scratch_quiesce()) != hipSuccess) return hip_fail(e, "stream ordering of the scratch buffers");
+    }
     for (int i = 0; i < warmup; ++i) {
         if ((rc = wgfft_exec_device(h, d_in, d_out, n, batch, dir, (void*)h->stream))) return rc;
         if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
@@ -831,6 +878,8 @@ int wgfft_bench_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, 
             float ms = 0.f;
             hipEventElapsedTime(&ms, a, b);
             *sec_per_iter = (double)ms * 1e-3 / iters;
+            std::lock_guard<std::mutex> lk(h->mu);
+            h->scratch_settled(h->stream);
         }
     }
     hipEventDestroy(a);

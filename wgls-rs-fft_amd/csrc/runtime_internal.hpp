@@ -83,6 +83,7 @@ struct Handle {
     std::map<int, const AdjointVariant*> tuned_vjp;     // VJP
     std::map<int, const AdjointVariant*> tuned_filter;  // spectral filter
     std::map<std::pair<const BatchedVariant*, int>, int> occ_cache;
+    std::map<std::pair<const AdjointVariant*, int>, int> adj_occ_cache;
     std::vector<DevBuf> tables;
     DevBuf io_in, io_out, io_w, io_aux, scratch, scratch2, red, red2, lossbuf;
     uint64_t held = 0;
@@ -97,14 +98,24 @@ struct Handle {
     // use its device and lock, so wgfft_destroy frees them with it
     std::set<wgfft_ragged_t*> ragged_plans;
     // Device calls enqueued on different streams share the scratch buffers
-    // above.  Each call that uses them makes its stream wait for an event
-    // recorded after the previous such call's work, so two calls never use
-    // them at the same time on the device.
+    // above, and two calls must never use them at the same time on the
+    // device.  While every use is on one stream, that stream orders them and
+    // nothing is recorded (an event record costs the GPU ~4.5 us of idle
+    // time before the next launch).  The first stream-ordered use on a
+    // second stream while another stream's use may still be pending waits
+    // for the device once; from then on every use records scratch_ev, and a
+    // use on another stream waits for it.  Host-blocking calls wait for the
+    // device up front when needed (scratch_quiesce) and settle their own
+    // stream when they have synchronized it.
     hipEvent_t scratch_ev = nullptr;
-    hipStream_t scratch_stream = nullptr;
-    bool scratch_recorded = false;
+    hipStream_t scratch_stream = nullptr;  // last use; may be pending if scratch_pending
+    bool scratch_pending = false;
+    bool scratch_recorded = false;  // scratch_ev follows the last use
+    bool scratch_events = false;    // record after every use
     hipError_t scratch_begin(hipStream_t s);
     void scratch_end(hipStream_t s);
+    hipError_t scratch_quiesce();
+    void scratch_settled(hipStream_t s);
 
     int ensure(DevBuf& b, uint64_t bytes);
     void release(DevBuf& b);
@@ -113,6 +124,10 @@ struct Handle {
     int set_split(Plan* p, const std::vector<int>& log2r);
     // workgroups per CU of a variant (kept in occ_cache when cache is set)
     int blocks_per_cu(const BatchedVariant* v, Mode m, bool cache = true);
+    // workgroups per CU of a fused kernel, kept in adj_occ_cache (the
+    // occupancy query costs microseconds per call): kind 0 the VJP, 1 the
+    // fused forward + adjoint, 2 the filter, 3 the filter with noise
+    int adjoint_blocks_per_cu(const AdjointVariant* v, int kind);
     int run(Plan* p, Mode mode, const float2* in, float2* out, const float* win, uint64_t batch, float scale,
             hipStream_t s, const BatchedVariant* force = nullptr, float2* fixed = nullptr, uint64_t fixed_bytes = 0,
             uint64_t idist = 0, uint64_t odist = 0);
@@ -130,8 +145,9 @@ struct Handle {
 
 namespace wgfft {
 // Orders one call's use of the handle's scratch buffers after the previous
-// call's (on whatever stream that ran) and records the point after it on
-// `s` when the call returns, on success and on error paths alike.
+// call's (on whatever stream that ran) and notes it as the last use when the
+// call returns, on success and on error paths alike (recording an event
+// after it once more than one stream has used the scratch).
 struct ScratchUse {
     Handle* h;
     hipStream_t s;

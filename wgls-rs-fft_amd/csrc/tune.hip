@@ -254,13 +254,8 @@ extern "C" int wgfft_plan_describe(wgfft_t* h, uint64_t n, char* desc, uint64_t 
     return WGFFT_OK;
 }
 
-extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc, uint64_t desc_len) {
-    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
-    if (n == 0 || (n & (n - 1))) return fail(WGFFT_ERR_INVALID_LENGTH, kMsgLength);
-    if (batch == 0) return fail(WGFFT_ERR_INVALID_ARGUMENT, "batch must be >= 1");
-    std::lock_guard<std::mutex> lk(h->mu);
-    hipError_t e = hipSetDevice(h->device);
-    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+// The sweeps of wgfft_autotune, with h->mu held on the handle's device.
+static int autotune_locked(Handle* h, uint64_t n, uint64_t batch, char* desc, uint64_t desc_len) {
     Plan* p;
     int rc;
     if ((rc = h->get_plan(n, &p))) return rc;
@@ -325,7 +320,7 @@ extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc
                                  (float*)h->io_aux.p, gw, nullptr, n, batch, p->tw, h->stream);
         };
         auto filter_launch = [&] {
-            int grid = av->filter_blocks_per_cu(false) * h->num_cus;
+            int grid = h->adjoint_blocks_per_cu(av, 2) * h->num_cus;
             if ((uint64_t)grid > batch) grid = (int)batch;
             FilterArgs fa{(const float2*)h->io_in.p, w, (float2*)h->io_aux.p, p->tw, batch, grid, h->stream, 0u, 0.f};
             const hipError_t e = av->filter(fa);
@@ -385,6 +380,22 @@ extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc
             }
         }
     }
+    return WGFFT_OK;
+}
+
+extern "C" int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc, uint64_t desc_len) {
+    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (n == 0 || (n & (n - 1))) return fail(WGFFT_ERR_INVALID_LENGTH, kMsgLength);
+    if (batch == 0) return fail(WGFFT_ERR_INVALID_ARGUMENT, "batch must be >= 1");
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    // host-blocking: its sweeps have finished when it returns
+    if ((e = h->scratch_quiesce()) != hipSuccess) return hip_fail(e, "stream ordering of the scratch buffers");
+    const int rc = autotune_locked(h, n, batch, desc, desc_len);
+    if (rc != WGFFT_OK) return rc;
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    h->scratch_settled(h->stream);
     return WGFFT_OK;
 }
 

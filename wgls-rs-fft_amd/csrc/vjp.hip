@@ -40,6 +40,15 @@ const AdjointVariant* pick_adjoint(Handle* h, int log2n, int role) {
     return it != m.end() ? it->second : find_adjoint(log2n, role);
 }
 
+int Handle::adjoint_blocks_per_cu(const AdjointVariant* v, int kind) {
+    auto key = std::make_pair(v, kind);
+    auto it = adj_occ_cache.find(key);
+    if (it != adj_occ_cache.end()) return it->second;
+    const int b = kind < 2 ? v->blocks_per_cu(kind == 1) : v->filter_blocks_per_cu(kind == 3);
+    adj_occ_cache[key] = b;
+    return b;
+}
+
 const std::vector<const AdjointVariant*> adjoint_variants(int log2n) {
     std::vector<const AdjointVariant*> out;
     for (const auto& v : adjoint_registry())
@@ -220,7 +229,7 @@ int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x
     // partial matrix: one row of n + 4 floats per workgroup (gw partials,
     // then the loss partial in column n; rows kept 16-byte aligned)
     const uint64_t ld = n + 4;
-    int grid = v->blocks_per_cu(fwd) * h->num_cus;
+    int grid = h->adjoint_blocks_per_cu(v, fwd ? 1 : 0) * h->num_cus;
     if ((uint64_t)grid > batch) grid = (int)batch;
     const uint64_t rows = (uint64_t)grid;
     int rc;
@@ -393,7 +402,7 @@ extern "C" int wgfft_filter_device(wgfft_t* h, const float* d_x, const float* d_
     if (p->kind == Plan::kBatched) {
         const AdjointVariant* v = pick_adjoint(h, p->log2n, kRoleFilter);
         if (v) {
-            int grid = v->filter_blocks_per_cu(sigma != 0.f) * h->num_cus;
+            int grid = h->adjoint_blocks_per_cu(v, sigma != 0.f ? 3 : 2) * h->num_cus;
             if ((uint64_t)grid > batch) grid = (int)batch;
             FilterArgs a{(const float2*)d_x, d_s, (float2*)d_y, p->tw, batch, grid, s, seed, sigma};
             if ((e = v->filter(a)) != hipSuccess) return hip_fail(e, "filter kernel launch");

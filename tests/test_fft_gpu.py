@@ -556,6 +556,54 @@ def test_scratch_shared_across_streams(gpu, torch_cuda):
     assert torch.equal(gx, gx_ref) and torch.equal(gw, gw_ref)
 
 
+def test_scratch_ordering_host_calls_and_stream_switches(torch_cuda):
+    """A fresh handle used from one stream records nothing; its scratch use is
+    then ordered by a host-blocking call (host transform of the same
+    two-pass length while a stream call is still in flight), by the first
+    switch to another stream (a wait for the device), and from then on by
+    events.  Every result equals the same calls made one after another on a
+    separate handle."""
+    torch = torch_cuda
+    n, batch = 1 << 20, 8
+    xa = wgfft.synth_uniform((batch, n), 0x5EED0920)
+    xb = wgfft.synth_uniform((batch, n), 0x5EED0921)
+    m, mb = 4096, 64
+    xs = torch.from_numpy(wgfft.synth_uniform((mb, m), 0x5EED0922)).cuda()
+    gy = torch.from_numpy(wgfft.synth_uniform((mb, m), 0x5EED0923)).cuda()
+    w = torch.from_numpy(np.hanning(m).astype(np.float32)).cuda()
+    ta, tb = torch.from_numpy(xa).cuda(), torch.from_numpy(xb).cuda()
+
+    ref = wgfft.GpuFft.new()
+    try:
+        ra, rb = torch.empty_like(ta), torch.empty_like(tb)
+        ref.exec_device(ta, ra)
+        ref.exec_device(tb, rb, inverse=True)
+        gx_ref, gw_ref = torch.empty_like(xs), torch.empty(m, dtype=torch.float32, device="cuda")
+        ref.vjp_windowed_device(xs, w, gy, gx_ref, gw_ref)
+        torch.cuda.synchronize()
+        host_ref = ref.fft_array(xb)
+    finally:
+        ref.close()
+
+    g = wgfft.GpuFft.new()
+    try:
+        s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+        ya, yb, ya2 = torch.empty_like(ta), torch.empty_like(tb), torch.empty_like(ta)
+        gx, gw = torch.empty_like(xs), torch.empty_like(gw_ref)
+        torch.cuda.synchronize()
+        g.exec_device(ta, ya, stream=s1.cuda_stream)  # left in flight
+        host = g.fft_array(xb)                        # host-blocking, same scratch
+        g.exec_device(tb, yb, inverse=True, stream=s1.cuda_stream)
+        g.vjp_windowed_device(xs, w, gy, gx, gw, stream=s2.cuda_stream)  # first switch
+        g.exec_device(ta, ya2, stream=s1.cuda_stream)  # ordered by an event from here on
+        torch.cuda.synchronize()
+        assert np.array_equal(host, host_ref)
+        assert torch.equal(ya, ra) and torch.equal(ya2, ra) and torch.equal(yb, rb)
+        assert torch.equal(gx, gx_ref) and torch.equal(gw, gw_ref)
+    finally:
+        g.close()
+
+
 def test_windowed_device_c5_shape(gpu, torch_cuda, parity):
     torch = torch_cuda
     n, batch = 8192, 1024

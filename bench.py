@@ -222,7 +222,7 @@ def bench_mixed(torch, gpu, wgfft, steps: int) -> dict:
     y = torch.empty_like(x)
     out = {"transforms": 20000, "samples": int(total), "bytes": 16 * int(total)}
     for mode, name in ((0, "persistent"), (1, "graph"), (2, "per_length")):
-        ms = timed(torch, lambda: plan.exec_device(x, y, mode=mode), max(5, steps // 10), 3)
+        ms = timed(torch, lambda: plan.exec_device(x, y, mode=mode), max(20, steps // 10), 3)
         gbs = 16.0 * total / (ms * 1e-3) / 1e9
         out[name] = {"ms": round(ms, 4), "gsamples_s": round(total / (ms * 1e-3) / 1e9, 2),
                      "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
@@ -352,7 +352,7 @@ def bench_rocfft(torch, gpu, steps: int) -> dict:
     for n, batch in ((4096, 4096), (256, 65536), (16384, 1024), (8192, 1024), (1 << 22, 16)):
         x = (torch.rand(2 * n * batch, device="cuda") * 2 - 1).view(torch.complex64).view(batch, n)
         y = torch.empty_like(x)
-        k = max(10, steps // 10)
+        k = max(50, steps // 10)
         ours = timed(torch, lambda: gpu.exec_device(x, y), k, 5)
         theirs = timed(torch, lambda: torch.fft.fft(x, dim=1), k, 5)
         out[f"{n}x{batch}"] = {"rocfft_ms": round(theirs, 4), "wgfft_ms": round(ours, 4),
@@ -370,8 +370,8 @@ def bench_large(torch, gpu, steps: int, tune: bool) -> dict:
         y = torch.empty_like(x)
         if tune:
             gpu.autotune(n, batch)
-        ms = timed(torch, lambda: gpu.exec_device(x, y), max(5, steps // 20), 2)
-        ms_inv = timed(torch, lambda: gpu.exec_device(y, x, inverse=True), max(5, steps // 20), 2)
+        ms = timed(torch, lambda: gpu.exec_device(x, y), max(20, steps // 20), 2)
+        ms_inv = timed(torch, lambda: gpu.exec_device(y, x, inverse=True), max(20, steps // 20), 2)
         s = n * batch
         gbs = 16.0 * s / (ms * 1e-3) / 1e9
         out[f"2^{log2n}x{batch}"] = {"fwd_ms": round(ms, 4), "inv_ms": round(ms_inv, 4),

@@ -187,7 +187,7 @@ pub fn validate(result: &[Vec<Complex<f32>>], reference: &[Vec<Complex<f32>>]) -
             return ValidationOutcome::Fail { max_error: f32::INFINITY };
         }
         for (a, b) in r.iter().zip(w.iter()) {
-            max_err = max_err.max((a - b).norm());
+            max_err = max_err.max((*a - *b).norm());
         }
     }
     if max_err <= VALIDATION_TOLERANCE {

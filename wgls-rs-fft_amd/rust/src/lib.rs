@@ -228,15 +228,17 @@ impl GpuFft {
     /// Time the compiled kernel variants for length `n` on a scratch batch
     /// and keep the fastest for this executor; returns its description.
     pub fn autotune(&self, n: usize, batch: usize) -> Result<String, Box<dyn Error>> {
-        let mut buf = [0u8; 256];
-        check(unsafe { sys::wgfft_autotune(self.h, n as u64, batch as u64, buf.as_mut_ptr() as *mut c_char, 256) })?;
+        let mut buf = [0u8; 512];
+        check(unsafe {
+            sys::wgfft_autotune(self.h, n as u64, batch as u64, buf.as_mut_ptr() as *mut c_char, buf.len() as u64)
+        })?;
         Ok(desc_string(&buf))
     }
 
     /// The kernel plan this executor uses for length `n`.
     pub fn describe(&self, n: usize) -> Result<String, Box<dyn Error>> {
-        let mut buf = [0u8; 256];
-        check(unsafe { sys::wgfft_plan_describe(self.h, n as u64, buf.as_mut_ptr() as *mut c_char, 256) })?;
+        let mut buf = [0u8; 512];
+        check(unsafe { sys::wgfft_plan_describe(self.h, n as u64, buf.as_mut_ptr() as *mut c_char, buf.len() as u64) })?;
         Ok(desc_string(&buf))
     }
 

@@ -12,7 +12,7 @@ fn make_input(n: usize) -> Vec<Complex<f32>> {
 }
 
 fn max_err(a: &[Complex<f32>], b: &[Complex<f32>]) -> f32 {
-    a.iter().zip(b.iter()).map(|(x, y)| (x - y).norm()).fold(0.0, f32::max)
+    a.iter().zip(b.iter()).map(|(x, y)| (*x - *y).norm()).fold(0.0, f32::max)
 }
 
 #[test]
@@ -29,7 +29,7 @@ fn gpu_fft_matches_rustfft() {
 #[test]
 fn ifft_roundtrip_and_batch() {
     let fft = GpuFft::new().expect("GPU required");
-    let batch: Vec<Vec<Complex<f32>>> = (0..8).map(|k| make_input(4096).iter().map(|v| v * (k as f32 + 1.0)).collect()).collect();
+    let batch: Vec<Vec<Complex<f32>>> = (0..8).map(|k| make_input(4096).iter().map(|v| *v * (k as f32 + 1.0)).collect()).collect();
     let spec = fft.fft(&batch).expect("fft failed");
     let back = fft.ifft(&spec).expect("ifft failed");
     for (a, b) in batch.iter().zip(back.iter()) {
@@ -48,7 +48,7 @@ fn windowed_entry_equals_window_then_fft() {
     let w: Vec<f32> = (0..n)
         .map(|i| 0.5 * (1.0 - (2.0 * std::f32::consts::PI * i as f32 / (n as f32 - 1.0)).cos()))
         .collect();
-    let xw: Vec<Complex<f32>> = x.iter().zip(w.iter()).map(|(v, s)| v * *s).collect();
+    let xw: Vec<Complex<f32>> = x.iter().zip(w.iter()).map(|(v, s)| *v * *s).collect();
     let fused = fft.fft_windowed(&[x], &w).unwrap();
     let plain = fft.fft(&[xw]).unwrap();
     assert!(max_err(&fused[0], &plain[0]) < 1e-3);

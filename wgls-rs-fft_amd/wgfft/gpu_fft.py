@@ -260,8 +260,8 @@ class GpuFft(FftExecutor):
         return sec.value
 
     def autotune(self, n: int, batch: int) -> str:
-        buf = ctypes.create_string_buffer(256)
-        check(self._L.wgfft_autotune(self._h, n, batch, buf, 256))
+        buf = ctypes.create_string_buffer(512)
+        check(self._L.wgfft_autotune(self._h, n, batch, buf, 512))
         return buf.value.decode()
 
     def describe(self, n: int) -> str:

@@ -362,6 +362,39 @@ def bench_rocfft(torch, gpu, steps: int) -> dict:
     return out
 
 
+def bench_strided(torch, gpu, steps: int) -> dict:
+    """Strided layouts on the headline shape (4096 x 4096): rows padded to
+    4160 complex (unit element stride, signal distance 4160: one pass) and the
+    column layout of a (4096 x 4096) matrix (element stride 4096, distance
+    1: gathered, transformed and scattered), against the plain forward of the
+    same data; algorithmic bytes 16 B/sample for each."""
+    n, batch, pad = N, BATCH, N + 64
+    k = max(SECONDARY_STEPS, steps // 4)
+    x = torch.from_numpy(synth_rows(batch, n)).cuda()
+    y = torch.empty_like(x)
+    plain = timed(torch, lambda: gpu.exec_device(x, y), k, 5)
+    xp = torch.empty((batch, pad), dtype=torch.complex64, device="cuda")
+    yp = torch.empty_like(xp)
+    xp[:, :n].copy_(x)
+    padded = timed(torch, lambda: gpu.exec_strided_device(xp[:, :n], yp[:, :n]), k, 5)
+    xc = x.t().contiguous().t()  # element stride 4096, distance 1
+    yc = torch.empty((n, batch), dtype=torch.complex64, device="cuda").t()
+    column = timed(torch, lambda: gpu.exec_strided_device(xc, yc), k, 5)
+    out = {"plain_ms": round(plain, 4)}
+    for name, ms in (("padded_rows", padded), ("column_layout", column)):
+        gbs = 16.0 * n * batch / (ms * 1e-3) / 1e9
+        out[name] = {"ms": round(ms, 4), "frac": round(gbs / HBM_PEAK_GBS, 4), "ratio_to_plain": round(ms / plain, 3)}
+    out["padded_rows"]["layout"] = f"distance {pad}, element stride 1 (one pass)"
+    out["column_layout"]["layout"] = f"element stride {batch}, distance 1 (gather, transform, scatter)"
+    return out
+
+
+def synth_rows(batch: int, n: int):
+    import wgfft
+
+    return wgfft.synth_uniform((batch, n), 0x5EED0002)
+
+
 def bench_large(torch, gpu, steps: int, tune: bool) -> dict:
     out = {}
     for log2n, batch in ((22, 16), (26, 1)):
@@ -389,7 +422,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--headline-only", action="store_true", help="only the N=4096x4096 forward (profiling)")
-    ap.add_argument("--only", choices=("shapes", "mixed", "large", "c1", "c5", "rocfft", "fused"),
+    ap.add_argument("--only", choices=("shapes", "mixed", "large", "c1", "c5", "rocfft", "fused", "strided"),
                     help="profiling: skip the headline and run one secondary section")
     ap.add_argument("--no-tune", action="store_true", help="use the default kernel variants")
     args = ap.parse_args()
@@ -418,7 +451,8 @@ def main() -> int:
                    "large": lambda: bench_large(torch, gpu, args.steps, not args.no_tune),
                    "c1": lambda: bench_c1_and_host(torch, gpu, wgfft),
                    "c5": lambda: bench_c5(torch, gpu, wgfft, args.steps, not args.no_tune),
-                   "rocfft": lambda: bench_rocfft(torch, gpu, args.steps)}[args.only]
+                   "rocfft": lambda: bench_rocfft(torch, gpu, args.steps),
+                   "strided": lambda: bench_strided(torch, gpu, args.steps)}[args.only]
         print(json.dumps({args.only: section()}))
         gpu.close()
         return 0
@@ -483,6 +517,7 @@ def main() -> int:
         out["c5_windowed"] = bench_c5(torch, gpu, wgfft, args.steps, not args.no_tune)
         out["c1_and_host_api"] = bench_c1_and_host(torch, gpu, wgfft)
         out["rocfft"] = bench_rocfft(torch, gpu, args.steps)
+        out["strided_layouts"] = bench_strided(torch, gpu, args.steps)
         del x_keep, y_keep
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

@@ -520,6 +520,37 @@ def test_strided_layouts(gpu, torch_cuda, n, batch):
                                                           -1, None))
 
 
+@pytest.mark.parametrize("n,batch,pitch", [(8, 50, 50), (4096, 3, 7), (256, 33, 40), (1 << 17, 2, 2)])
+def test_column_layouts(gpu, torch_cuda, n, batch, pitch):
+    """Signals stored as the columns of a matrix (adjacent signals: distance
+    1, element stride = the row pitch >= batch) in and out, through the tiled
+    gather and scatter, with edges where batch and n are not multiples of the
+    32 x 32 tile: bit-identical to the contiguous device call, forward and
+    inverse, and against FP64."""
+    torch = torch_cuda
+    x = oracle.synth_uniform((batch, n), 0x5EED0940 + n + batch)
+    t = torch.from_numpy(x).cuda()
+    base = torch.zeros((n, pitch), dtype=torch.complex64, device="cuda")
+    base[:, :batch] = t.t()
+    src = base[:, :batch].t()  # strides (1, pitch)
+    assert src.stride() == (1, pitch)
+    for inverse in (False, True):
+        ref = torch.empty_like(t)
+        gpu.exec_device(t, ref, inverse=inverse)
+        out_base = torch.zeros((n, pitch + 1), dtype=torch.complex64, device="cuda")
+        dst = out_base[:, :batch].t()  # strides (1, pitch + 1)
+        gpu.exec_strided_device(src, dst, inverse=inverse)
+        rows = torch.empty_like(t)
+        gpu.exec_strided_device(src, rows, inverse=inverse)  # column layout in, rows out
+        torch.cuda.synchronize()
+        want = ref.cpu().numpy()
+        assert np.array_equal(dst.cpu().numpy(), want), inverse
+        assert np.array_equal(rows.cpu().numpy(), want), inverse
+        assert not out_base[:, batch:].abs().sum().item()  # nothing written outside the columns
+        if not inverse:
+            assert oracle.rel_l2(want, oracle.fft64(x, -1)) <= rel_bar(n)
+
+
 def test_scratch_shared_across_streams(gpu, torch_cuda):
     """Calls of one handle on two streams that both use the handle's scratch
     (global passes at 2^20, the strided gather/scatter buffer, the VJP's

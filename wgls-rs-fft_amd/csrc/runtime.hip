@@ -489,6 +489,55 @@ __global__ void __launch_bounds__(256) k_scatter(const float2* __restrict__ src,
         dst[(e >> log2n) * dist + (e & mask) * stride] = src[e];
 }
 
+// Adjacent signals (distance 1, any element stride): column layouts.  The
+// element-by-element kernels above then read (gather) or write (scatter) one
+// 8-byte value per row touched.  Here a 32 x 32 tile of (signal, element)
+// pairs moves along the signals on the strided side (256-byte row segments)
+// and along the elements on the contiguous side, transposed through LDS.
+// Tiles are numbered in one dimension (n / 32 exceeds the grid's y limit at
+// large n) and visited grid-stride.
+constexpr uint32_t kTile = 32;
+
+__global__ void __launch_bounds__(256) k_gather_adjacent(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                         uint64_t n, uint64_t batch, uint64_t stride) {
+    __shared__ float2 tile[kTile][kTile + 1];
+    const uint64_t tb = (batch + kTile - 1) / kTile, tiles = tb * ((n + kTile - 1) / kTile);
+    const uint32_t lx = threadIdx.x % kTile, ly = threadIdx.x / kTile;
+    for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const uint64_t b0 = (t % tb) * kTile, j0 = (t / tb) * kTile;
+        for (uint32_t r = ly; r < kTile; r += 256 / kTile) {  // src[b + j stride], b fastest
+            const uint64_t b = b0 + lx, j = j0 + r;
+            if (b < batch && j < n) tile[r][lx] = src[b + j * stride];
+        }
+        __syncthreads();
+        for (uint32_t r = ly; r < kTile; r += 256 / kTile) {  // dst[b n + j], j fastest
+            const uint64_t b = b0 + r, j = j0 + lx;
+            if (b < batch && j < n) dst[b * n + j] = tile[lx][r];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scatter_adjacent(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                          uint64_t n, uint64_t batch, uint64_t stride) {
+    __shared__ float2 tile[kTile][kTile + 1];
+    const uint64_t tb = (batch + kTile - 1) / kTile, tiles = tb * ((n + kTile - 1) / kTile);
+    const uint32_t lx = threadIdx.x % kTile, ly = threadIdx.x / kTile;
+    for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const uint64_t b0 = (t % tb) * kTile, j0 = (t / tb) * kTile;
+        for (uint32_t r = ly; r < kTile; r += 256 / kTile) {  // src[b n + j], j fastest
+            const uint64_t b = b0 + r, j = j0 + lx;
+            if (b < batch && j < n) tile[r][lx] = src[b * n + j];
+        }
+        __syncthreads();
+        for (uint32_t r = ly; r < kTile; r += 256 / kTile) {  // dst[b + j stride], b fastest
+            const uint64_t b = b0 + lx, j = j0 + r;
+            if (b < batch && j < n) dst[b + j * stride] = tile[lx][r];
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace wgfft
 
 using namespace wgfft;
@@ -743,14 +792,26 @@ int wgfft_exec_strided_device(wgfft_t* h, const float* d_in, float* d_out, uint6
     ScratchUse su(h, s);
     if (su.e != hipSuccess) return hip_fail(su.e, "stream ordering of the scratch buffers");
     const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 8192);
+    // the adjacent-signal kernels visit 32 x 32 tiles grid-stride
+    const uint64_t tiles = ((batch + kTile - 1) / kTile) * ((n + kTile - 1) / kTile);
+    const unsigned tblocks = (unsigned)std::min<uint64_t>(tiles, 8192);
     float2* aux = (float2*)h->io_aux.p;
     const uint32_t log2n = (uint32_t)log2u(n);
-    hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), 0, s, (const float2*)d_in, aux, log2n, total, istride, idist);
+    if (idist == 1 && batch > 1)
+        hipLaunchKernelGGL(k_gather_adjacent, dim3(tblocks), dim3(256), 0, s, (const float2*)d_in, aux, n, batch,
+                           istride);
+    else
+        hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), 0, s, (const float2*)d_in, aux, log2n, total, istride,
+                           idist);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "gather launch");
     if ((rc = h->run(p, inv ? kInverse : kForward, aux, aux, nullptr, batch, inv ? 1.0f / (float)n : 1.0f, s)))
         return rc;
-    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, (const float2*)aux, (float2*)d_out, log2n, total,
-                       ostride, odist);
+    if (odist == 1 && batch > 1)
+        hipLaunchKernelGGL(k_scatter_adjacent, dim3(tblocks), dim3(256), 0, s, (const float2*)aux, (float2*)d_out, n,
+                           batch, ostride);
+    else
+        hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, (const float2*)aux, (float2*)d_out, log2n, total,
+                           ostride, odist);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "scatter launch");
     return WGFFT_OK;
 }

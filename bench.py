@@ -183,10 +183,11 @@ def timed(torch, fn, steps: int, warmup: int) -> float:
 
 
 def bench_rotating(torch, gpu, x, steps: int, pairs: int = 4) -> dict:
-    """The headline transform cycling through `pairs` distinct input/output
-    buffer pairs (a working set of 4 x 256 MiB, four times the 256 MB
-    memory-side cache), so no step re-reads what the previous one left in
-    that cache."""
+    """The forward transform of x's shape cycling through `pairs` distinct
+    input/output buffer pairs (at 16.8 M samples a working set of 4 x 256 MiB,
+    four times the 256 MB memory-side cache), so no step re-reads what the
+    previous one left in that cache; beside it, a device copy over the same
+    rotation."""
     ins = [x] + [torch.empty_like(x).copy_(x) for _ in range(pairs - 1)]
     outs = [torch.empty_like(x) for _ in range(pairs)]
     state = {"i": 0}
@@ -247,7 +248,8 @@ def _ragged_log2(count, kmin, kmax, seed):
 
 
 def bench_shapes(torch, gpu, steps: int, tune: bool) -> dict:
-    """The same 16.8 M samples as 256 x 65536 and 16384 x 1024."""
+    """The same 16.8 M samples as 256 x 65536 and 16384 x 1024, on one
+    buffer pair and rotating through four."""
     out = {}
     for n, batch in ((256, 65536), (16384, 1024)):
         x = (torch.rand(2 * n * batch, device="cuda") * 2 - 1).view(torch.complex64).view(batch, n)
@@ -259,7 +261,7 @@ def bench_shapes(torch, gpu, steps: int, tune: bool) -> dict:
         gbs = 16.0 * s / (ms * 1e-3) / 1e9
         out[f"{n}x{batch}"] = {"ms": round(ms, 4), "gsamples_s": round(s / (ms * 1e-3) / 1e9, 2),
                                "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
-                               "plan": gpu.describe(n)}
+                               "plan": gpu.describe(n), "rotating": bench_rotating(torch, gpu, x, steps)}
         del x, y
     return out
 

@@ -273,8 +273,16 @@ def bench_c1_and_host(torch, gpu, wgfft) -> dict:
     out = {}
     x = torch.from_numpy(wgfft.synth_uniform((8, 1024), 0x5EED0001)).cuda()
     y = torch.empty_like(x)
+    # back-to-back calls from Python (the wrapper's checks and ctypes call are
+    # ~8 us of host time per call, which bounds this figure) and from C++
+    # (wgfft_bench_device: calls in a loop between HIP events, the device's
+    # own rate for this launch)
     out["c1_device_us"] = {"fwd": round(1000 * timed(torch, lambda: gpu.exec_device(x, y), 2000, 100), 2),
-                           "inv": round(1000 * timed(torch, lambda: gpu.exec_device(y, x, inverse=True), 2000, 100), 2)}
+                           "inv": round(1000 * timed(torch, lambda: gpu.exec_device(y, x, inverse=True), 2000, 100), 2),
+                           "via": "python exec_device"}
+    out["c1_bench_device_us"] = {"fwd": round(1e6 * gpu.bench_device(x, y, warmup=100, iters=2000), 2),
+                                 "inv": round(1e6 * gpu.bench_device(y, x, inverse=True, warmup=100, iters=2000), 2),
+                                 "via": "wgfft_bench_device"}
     for (n, batch, reps) in ((1024, 8, 200), (4096, 4096, 5)):
         xh = wgfft.synth_uniform((batch, n), 0x5EED0002)
         gpu.fft_array(xh)  # warm (plan, staging)

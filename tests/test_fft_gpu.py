@@ -187,8 +187,22 @@ def test_default_plans():
             assert want in g.describe(1 << log2n), (log2n, g.describe(1 << log2n))
         x = oracle.synth_uniform((2, 1 << 16), 0x5EED0260)
         check_fwd_inv(g, x)
+        # 512 .. 2048: small batches (at most 4 per CU) run one transform per
+        # workgroup, until a variant is pinned or tuned
+        for log2n in (9, 10, 11):
+            d = g.describe(1 << log2n)
+            assert "; batches <= " in d and " G=1 " in d.split("; batches <= ")[1], d
+        assert "; batches <= " not in g.describe(1 << 12)
     finally:
         g.close()
+    p = wgfft.GpuFft.new()
+    try:
+        p.select_variant(1024, 0)
+        assert "; batches <= " not in p.describe(1024)
+        x = oracle.synth_uniform((8, 1024), 0x5EED0261)
+        check_fwd_inv(p, x)
+    finally:
+        p.close()
 
 
 @pytest.mark.parametrize("log2n", [17, 20, 23])

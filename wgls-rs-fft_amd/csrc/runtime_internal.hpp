@@ -37,6 +37,9 @@ struct Plan {
     int log2n = 0;
     Kind kind = kSmall;
     const BatchedVariant* var = nullptr;  // kBatched
+    // kBatched, untuned plans only: the one-transform-per-workgroup variant
+    // used for batches of at most kSmallBatchPerCu transforms per CU
+    const BatchedVariant* var_small = nullptr;
     float2* tw = nullptr;                 // kBatched: W_N^j, j < N
     std::vector<int> pass_log2r;          // kMultiPass
     std::vector<const GlobalPassVariant*> pass_var;
@@ -64,6 +67,8 @@ int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x
 const char* adjoint_describe(const AdjointVariant* v, char* buf, uint64_t len);
 void synth_uniform(float* out, uint64_t nfloats, uint64_t seed);
 const BatchedVariant* default_variant(int log2n);
+const BatchedVariant* small_batch_variant(int log2n);
+constexpr uint64_t kSmallBatchPerCu = 4;
 hipError_t launch_add_noise(const float2* src, float2* dst, uint64_t n, uint64_t batch, uint32_t seed,
                             float sigma, hipStream_t s);
 std::vector<int> multipass_radices(int log2n);

@@ -250,6 +250,13 @@ extern "C" int wgfft_plan_describe(wgfft_t* h, uint64_t n, char* desc, uint64_t 
         std::snprintf(desc, desc_len, "register N=%llu one thread per transform", (unsigned long long)n);
     } else if (p->kind == Plan::kBatched) {
         describe(p->var, desc, desc_len);
+        if (p->var_small) {
+            char small[160];
+            describe(p->var_small, small, sizeof small);
+            const std::string full = std::string(desc) + "; batches <= " +
+                                     std::to_string(kSmallBatchPerCu * (uint64_t)h->num_cus) + ": " + small;
+            std::snprintf(desc, desc_len, "%s", full.c_str());
+        }
     } else {
         std::snprintf(desc, desc_len, "%s", describe_multipass(p).c_str());
     }
@@ -379,6 +386,7 @@ static int autotune_locked(Handle* h, uint64_t n, uint64_t batch, char* desc, ui
     if (best) {
         h->tuned[p->log2n] = best;
         p->var = best;
+        p->var_small = nullptr;
         describe(best, desc, desc_len);
         if (desc && desc_len && !adj_desc.empty()) {
             std::string full = std::string(desc) + adj_desc;
@@ -435,6 +443,7 @@ extern "C" int wgfft_select_variant(wgfft_t* h, uint64_t n, int index, char* des
         if (i++ == index) {
             h->tuned[p->log2n] = &v;
             p->var = &v;
+            p->var_small = nullptr;
             describe(&v, desc, desc_len);
             return WGFFT_OK;
         }

@@ -20,3 +20,7 @@ WGFFT_REGISTER_BATCHED_F(2048, 16, 2, 9, 16x8x16, 16, 8, 16);
 // per 2^24 samples; no gain at 256 and 512).
 WGFFT_REGISTER_BATCHED_F(1024, 16, 4, 512, 16x4x16, 16, 4, 16);
 WGFFT_REGISTER_BATCHED_F(2048, 16, 2, 512, 16x8x16, 16, 8, 16);
+// One transform per workgroup (G = 1): a small batch spread over as many
+// CUs as it has transforms.
+WGFFT_REGISTER_BATCHED_F(1024, 16, 1, 512, 16x4x16, 16, 4, 16);
+WGFFT_REGISTER_BATCHED_F(2048, 16, 1, 512, 16x8x16, 16, 8, 16);

@@ -21,3 +21,5 @@ WGFFT_REGISTER_BATCHED_F(256, 16, 16, 9, 16x16, 16, 16);
 WGFFT_REGISTER_BATCHED_F(256, 32, 32, 9, 16x16, 16, 16);
 WGFFT_REGISTER_BATCHED_F(512, 32, 16, 9, 16x16x2, 16, 16, 2);
 WGFFT_REGISTER_BATCHED_F(512, 16, 8, 9, 8x8x8, 8, 8, 8);
+// One transform per workgroup (G = 1, E = 8: one wavefront).
+WGFFT_REGISTER_BATCHED(512, 8, 1, 8x8x8, 8, 8, 8);

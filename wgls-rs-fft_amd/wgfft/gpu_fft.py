@@ -152,12 +152,19 @@ class GpuFft(FftExecutor):
 
     # ---- device buffers (torch tensors on the GPU) ---------------------
     @staticmethod
-    def _stream(stream) -> int:
-        if stream is None:
-            import torch
+    def _stream(stream, t=None) -> int:
+        """The HIP stream to enqueue on: `stream` as given, else torch's
+        current stream of tensor t's device (through torch's raw-stream
+        query where it exists: 0.14 us against 2.8 us for
+        torch.cuda.current_stream() per call)."""
+        if stream is not None:
+            return int(stream)
+        import torch
 
-            return torch.cuda.current_stream().cuda_stream
-        return int(stream)
+        raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+        if raw is not None:
+            return raw(t.get_device() if t is not None else torch.cuda.current_device())
+        return torch.cuda.current_stream().cuda_stream
 
     def exec_device(self, x, out, inverse: bool = False, stream=None) -> None:
         """Transform a (batch, n) complex64 CUDA tensor into `out`."""
@@ -165,7 +172,7 @@ class GpuFft(FftExecutor):
         _check_tensor(out, x.shape)
         batch, n = x.shape
         check(self._L.wgfft_exec_device(self._h, x.data_ptr(), out.data_ptr(), n, batch,
-                                        WGFFT_INVERSE if inverse else WGFFT_FORWARD, self._stream(stream)))
+                                        WGFFT_INVERSE if inverse else WGFFT_FORWARD, self._stream(stream, x)))
 
     def exec_device_noalloc(self, x, out, inverse: bool = False, stream=None) -> None:
         """exec_device without any allocation (wgfft_exec_device_noalloc): the
@@ -174,7 +181,7 @@ class GpuFft(FftExecutor):
         _check_tensor(out, x.shape)
         batch, n = x.shape
         check(self._L.wgfft_exec_device_noalloc(self._h, x.data_ptr(), out.data_ptr(), n, batch,
-                                                WGFFT_INVERSE if inverse else WGFFT_FORWARD, self._stream(stream)))
+                                                WGFFT_INVERSE if inverse else WGFFT_FORWARD, self._stream(stream, x)))
 
     def exec_windowed_device(self, x, w, out, stream=None) -> None:
         """out = FFT(w * x) per row: x, out (batch, n) complex64, w n floats."""
@@ -183,7 +190,7 @@ class GpuFft(FftExecutor):
         batch, n = x.shape
         _check_real(w, n, "window")
         check(self._L.wgfft_exec_windowed_device(self._h, x.data_ptr(), w.data_ptr(), out.data_ptr(), n, batch,
-                                                 self._stream(stream)))
+                                                 self._stream(stream, x)))
 
     def vjp_windowed_device(self, x, w, gy, gx, gw, stream=None) -> None:
         """gx = w * F^H gy and gw = sum over rows of Re(conj(F^H gy) x)."""
@@ -194,7 +201,7 @@ class GpuFft(FftExecutor):
         _check_real(w, n, "window")
         _check_real(gw, n, "gw")
         check(self._L.wgfft_vjp_windowed_device(self._h, x.data_ptr(), w.data_ptr(), gy.data_ptr(),
-                                                gx.data_ptr(), gw.data_ptr(), n, batch, self._stream(stream)))
+                                                gx.data_ptr(), gw.data_ptr(), n, batch, self._stream(stream, x)))
 
     def fwd_adjoint_masked_device(self, x, w, mask, y, gx, gw, loss, stream=None) -> None:
         """Fused forward + adjoint of L = sum m |FFT(w x)|^2: writes gx, gw,
@@ -210,7 +217,7 @@ class GpuFft(FftExecutor):
         _check_real(loss, 1, "loss")
         check(self._L.wgfft_fwd_adjoint_masked_device(
             self._h, x.data_ptr(), w.data_ptr(), mask.data_ptr(), 0 if y is None else y.data_ptr(),
-            gx.data_ptr(), gw.data_ptr(), loss.data_ptr(), n, batch, self._stream(stream)))
+            gx.data_ptr(), gw.data_ptr(), loss.data_ptr(), n, batch, self._stream(stream, x)))
 
     def exec_strided_device(self, x, out, inverse: bool = False, stream=None) -> None:
         """Transform each row of a (batch, n) complex64 CUDA tensor view with
@@ -225,7 +232,7 @@ class GpuFft(FftExecutor):
         odist, ostride = out.stride()
         check(self._L.wgfft_exec_strided_device(self._h, x.data_ptr(), out.data_ptr(), n, batch, istride, idist,
                                                 ostride, odist, WGFFT_INVERSE if inverse else WGFFT_FORWARD,
-                                                self._stream(stream)))
+                                                self._stream(stream, x)))
 
     def add_noise_device(self, x, seed: int = 0, sigma: float = 0.1, stream=None) -> None:
         """The reference pipeline's Gaussian noise stage, in place on a
@@ -233,7 +240,7 @@ class GpuFft(FftExecutor):
         _check_tensor(x)
         batch, n = x.shape
         check(self._L.wgfft_add_noise_device(self._h, x.data_ptr(), n, batch, seed & 0xFFFFFFFF, sigma,
-                                             self._stream(stream)))
+                                             self._stream(stream, x)))
 
     def filter_device(self, x, s, out, noise_sigma: float = 0.0, seed: int = 0, stream=None) -> None:
         """out = ifft(fft(x + noise) * s), 1/N scaled: a (batch, n) complex64
@@ -245,7 +252,7 @@ class GpuFft(FftExecutor):
         batch, n = x.shape
         _check_real(s, n, "spectral window")
         check(self._L.wgfft_filter_device(self._h, x.data_ptr(), s.data_ptr(), out.data_ptr(), n, batch,
-                                          seed & 0xFFFFFFFF, noise_sigma, self._stream(stream)))
+                                          seed & 0xFFFFFFFF, noise_sigma, self._stream(stream, x)))
 
     def bench_device(self, x, out, inverse: bool = False, warmup: int = 3, iters: int = 50) -> float:
         """Seconds per exec_device call on resident buffers, timed with HIP
@@ -352,7 +359,7 @@ class RaggedPlan:
                                  f"expected contiguous complex64 tensors of at least {self.extent} elements on the GPU")
         check(self._gpu._L.wgfft_ragged_exec_device(self._p, x.data_ptr(), out.data_ptr(),
                                                     WGFFT_INVERSE if inverse else WGFFT_FORWARD, int(mode),
-                                                    GpuFft._stream(stream)))
+                                                    GpuFft._stream(stream, x)))
 
     def describe(self) -> str:
         """Kernel choice of the plan (wgfft_ragged_describe)."""

@@ -205,11 +205,15 @@ int wgfft_bench_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, 
                        int warmup, int iters, double* sec_per_iter);
 
 /* Time every compiled kernel variant for length n on a scratch batch of
- * `batch` transforms and make the fastest the handle's choice for n.
- * Writes the chosen variant's description into desc (if non-NULL). */
+ * `batch` transforms and make the fastest the handle's choice for n, for
+ * every batch size (an untuned plan of 512 .. 2048 runs batches of at most
+ * 4 transforms per CU one transform per workgroup; a tuned or pinned one
+ * does not).  Writes the chosen variant's description into desc (if
+ * non-NULL). */
 int wgfft_autotune(wgfft_t* h, uint64_t n, uint64_t batch, char* desc, uint64_t desc_len);
 
-/* Description of the variant the handle currently uses for length n. */
+/* Description of the variant the handle currently uses for length n (and,
+ * for an untuned plan of 512 .. 2048, the one its small batches use). */
 int wgfft_plan_describe(wgfft_t* h, uint64_t n, char* desc, uint64_t desc_len);
 
 /* Number of compiled single-kernel (LDS-resident) variants for length n;

@@ -126,7 +126,8 @@ int wgfft_exec_device_noalloc(wgfft_t* h, const float* d_in, float* d_out, uint6
  * buffers, input and output disjoint (or the same rows, in place) -- are
  * transformed in one pass for n <= 16384.  Other layouts are gathered into a
  * contiguous scratch buffer, transformed in place there and scattered, on the
- * caller's stream. */
+ * caller's stream (adjacent signals, distance 1 -- the columns of a matrix --
+ * through 32 x 32 tiles that keep both sides coalesced). */
 int wgfft_exec_strided_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, uint64_t batch,
                               uint64_t istride, uint64_t idist, uint64_t ostride, uint64_t odist, int dir,
                               void* stream);

@@ -294,8 +294,10 @@ static int autotune_locked(Handle* h, uint64_t n, uint64_t batch, char* desc, ui
     std::vector<const BatchedVariant*> cands;
     for (const auto& v : batched_registry())
         if (v.log2n == p->log2n) cands.push_back(&v);
-    // two sweeps in opposite orders, each candidate keeping its faster time
-    // (one sweep order kept a plan 9% slower than the default at 8192 x 1024)
+    // two sweeps in opposite orders, each candidate keeping its faster mean
+    // (clock drift along one sweep favours whatever it times last; with
+    // best-burst scores the sweep kept a plan 5-9% slower than the default
+    // at 8192 x 1024)
     std::vector<float> cand_ms(cands.size(), 1e30f);
     for (int sweep = 0; sweep < 2; ++sweep)
         for (size_t i = 0; i < cands.size(); ++i) {

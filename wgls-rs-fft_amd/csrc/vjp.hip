@@ -155,8 +155,8 @@ __global__ void __launch_bounds__(256) k_sum_all(const float* __restrict__ part,
 // eight loads in flight; lanes combine in a fixed order through LDS (no
 // cross-block ordering, so no device-scope fences).  Columns < n go to gw;
 // when loss != NULL, column n holds per-row loss partials summed into
-// loss[0].  Measured on partial matrices of 256-1024 rows: 32 columns (257
-// blocks at N = 8192) for N <= 4096 (768 x 4100: 3.1 against 3.8 us with
+// loss[0].  Measured on partial matrices of 256-1024 rows: 32 columns (129
+// blocks at N = 4096) for N <= 4096 (768 x 4100: 3.1 against 3.8 us with
 // 64), 64 columns above (512 x 8196: 4.1 against 5.1 us with 32).
 template <int COLS>
 __global__ void __launch_bounds__(1024)

@@ -565,6 +565,47 @@ def test_column_layouts(gpu, torch_cuda, n, batch, pitch):
             assert oracle.rel_l2(want, oracle.fft64(x, -1)) <= rel_bar(n)
 
 
+def _energy(torch, t, rows_per_chunk):
+    """sum |t|^2 in FP64, a chunk of rows at a time (no full-size temporaries)."""
+    total = 0.0
+    for r0 in range(0, t.shape[0], rows_per_chunk):
+        total += float(torch.view_as_real(t[r0:r0 + rows_per_chunk]).double().pow(2).sum())
+    return total
+
+
+@pytest.mark.parametrize("n,batch", [(4096, (1 << 20) + 37), (1 << 22, 1025)])
+def test_batches_beyond_2pow32_elements(gpu, torch_cuda, n, batch):
+    """More than 2^32 complex elements in one call (34-35 GB in and as much
+    out; the batched kernel at 4096, the global passes at 2^22 with the batch
+    in 2 GiB chunks): every transform keeps its energy (Parseval over the
+    whole batch), the transforms at the start, on both sides of the 2^31- and
+    2^32-element offsets and at the end match FP64, and the in-place inverse
+    returns them."""
+    torch = torch_cuda
+    assert n * batch > (1 << 32)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0x5EED0950 + n)
+    x = torch.rand((batch, 2 * n), device="cuda", generator=gen)
+    x.mul_(2.0).sub_(1.0)
+    x = x.view(torch.complex64)
+    y = torch.empty_like(x)
+    try:
+        gpu.exec_device(x, y)
+        torch.cuda.synchronize()
+        chunk = max(1, (1 << 27) // n)
+        e_x, e_y = _energy(torch, x, chunk), _energy(torch, y, chunk)
+        assert abs(e_y / (n * e_x) - 1.0) <= 1e-5, (e_x, e_y)
+        rows = sorted({0, (1 << 31) // n - 1, (1 << 31) // n, (1 << 32) // n - 1, (1 << 32) // n, batch - 1})
+        xr = x[rows].cpu().numpy()
+        assert oracle.rel_l2(y[rows].cpu().numpy(), oracle.fft64(xr, -1)) <= rel_bar(n)
+        gpu.exec_device(y, y, inverse=True)
+        torch.cuda.synchronize()
+        assert oracle.rel_l2(y[rows].cpu().numpy(), xr) <= 2e-5
+    finally:
+        del x, y
+        torch.cuda.empty_cache()
+
+
 def test_scratch_shared_across_streams(gpu, torch_cuda):
     """Calls of one handle on two streams that both use the handle's scratch
     (global passes at 2^20, the strided gather/scatter buffer, the VJP's

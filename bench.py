@@ -508,6 +508,13 @@ def main() -> int:
                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                        "traffic_source": traffic_src if traffic is not None else None,
                        "copy_gbs": round(2 * x.numel() * 8 / (copy_ms * 1e-3) / 1e9, 1)}
+    # the measurement protocol's repeat statistics: five more timed regions of
+    # the same workload (at least 200 calls each), median and spread
+    reps = [timed(torch, lambda: gpu.exec_device(x, y), max(SECONDARY_STEPS, args.steps // 4), 5) for _ in range(5)]
+    reps_sorted = sorted(reps)
+    out["headline_repeats"] = {"ms": [round(r, 5) for r in reps], "median_ms": round(reps_sorted[2], 5),
+                               "median_gsamples_s": round(samples / (reps_sorted[2] * 1e-3) / 1e9, 2),
+                               "spread": round((reps_sorted[-1] - reps_sorted[0]) / reps_sorted[2], 4)}
     out["headline_rotating"] = bench_rotating(torch, gpu, x, args.steps)
     # GpuFftTrait::benchmark_gpu_only semantics (src/lib.rs:109-143): 3 warm-up
     # calls with waits, then 50 back-to-back calls between HIP events on the

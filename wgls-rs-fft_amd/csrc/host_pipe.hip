@@ -5,13 +5,18 @@
 // pageable buffer straight to the device runs at the HIP runtime's staged rate
 // (measured on MI355X: 30 GB/s up, 18 GB/s down), and the transform then waits
 // for the whole upload.  Here the batch moves in chunks of whole transforms
-// through two pinned staging slots per direction: host threads copy chunk
-// i + 1 into pinned memory and chunk i - 1 out of it, while the DMA engines
-// move chunk i (57 GB/s per direction from pinned memory) on two copy
-// streams and the transform runs on the handle's stream.
+// through four pinned staging slots per direction: host threads copy chunk
+// i into pinned memory and chunk i - 3 out of it, while the DMA engines move
+// the chunks between (57 GB/s per direction from pinned memory) on two copy
+// streams and the transforms run on the handle's stream.  4096 x 4096 host
+// to host: 4.3-4.8 ms with 32 MiB chunks (two slots of 64 MiB: 5.4-7.0 ms;
+// 16 MiB chunks: a similar median with outliers to 11 ms, the copy threads
+// waking twice per chunk).
 #include <hip/hip_runtime.h>
+#include <sched.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <condition_variable>
 #include <cstring>
 #include <functional>
@@ -96,12 +101,13 @@ private:
 };
 
 struct HostPipe {
-    static constexpr int kSlots = 2;
+    static constexpr int kMaxSlots = 8;
+    int slots = 4;  // staging slots per direction (WGFFT_HOST_SLOTS, 2 .. kMaxSlots)
     uint64_t slot_bytes = 0;
-    void* pin_in[kSlots] = {};
-    void* pin_out[kSlots] = {};
+    void* pin_in[kMaxSlots] = {};
+    void* pin_out[kMaxSlots] = {};
     hipStream_t s_in = nullptr, s_out = nullptr;
-    hipEvent_t ev_in[kSlots] = {}, ev_run[kSlots] = {}, ev_out[kSlots] = {};
+    hipEvent_t ev_in[kMaxSlots] = {}, ev_run[kMaxSlots] = {}, ev_out[kMaxSlots] = {};
     CopyPool* pool = nullptr;
 };
 
@@ -113,7 +119,7 @@ void Handle::free_pipe() {
             hipStreamSynchronize(s);
             hipStreamDestroy(s);
         }
-    for (int i = 0; i < HostPipe::kSlots; ++i) {
+    for (int i = 0; i < HostPipe::kMaxSlots; ++i) {
         for (hipEvent_t ev : {hp->ev_in[i], hp->ev_run[i], hp->ev_out[i]})
             if (ev) hipEventDestroy(ev);
         if (hp->pin_in[i]) hipHostFree(hp->pin_in[i]);
@@ -131,19 +137,28 @@ static int pipe_ready(Handle* h, uint64_t bytes) {
         HostPipe* hp = new (std::nothrow) HostPipe();
         if (!hp) return fail(WGFFT_ERR_OUT_OF_MEMORY, "host pipeline allocation failed");
         h->pipe = hp;
+        if (const char* env = std::getenv("WGFFT_HOST_SLOTS"))
+            hp->slots = std::max(2, std::min(HostPipe::kMaxSlots, std::atoi(env)));
         if ((e = hipStreamCreateWithFlags(&hp->s_in, hipStreamNonBlocking)) == hipSuccess)
             e = hipStreamCreateWithFlags(&hp->s_out, hipStreamNonBlocking);
-        for (int i = 0; i < HostPipe::kSlots && e == hipSuccess; ++i)
+        for (int i = 0; i < hp->slots && e == hipSuccess; ++i)
             for (hipEvent_t* ev : {&hp->ev_in[i], &hp->ev_run[i], &hp->ev_out[i]})
                 if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
         if (e != hipSuccess) {
             h->free_pipe();  // no half-built pipeline is kept
             return hip_fail(e, "host pipeline streams and events");
         }
-        // 8 threads in all copy at ~89 GB/s on the MI355X host (1 thread: 27 GB/s)
-        const unsigned hw = std::thread::hardware_concurrency();
+        // copy threads (the caller and helpers): WGFFT_HOST_COPY_THREADS, else
+        // 8, at most the CPUs this process may run on (8 threads copy at
+        // ~89 GB/s on the MI355X host, 1 thread at 27 GB/s)
+        int cpus = (int)std::thread::hardware_concurrency();
+        cpu_set_t set;
+        if (sched_getaffinity(0, sizeof set, &set) == 0) cpus = CPU_COUNT(&set);
+        int want = 8;
+        if (const char* env = std::getenv("WGFFT_HOST_COPY_THREADS")) want = std::max(1, std::atoi(env));
+        const int threads = std::max(1, std::min(want, cpus));
         try {  // nothing may throw across the C-ABI
-            hp->pool = new CopyPool((int)std::min(7u, hw > 1 ? hw - 1 : 0u));
+            hp->pool = new CopyPool(threads - 1);
         } catch (...) {
             hp->pool = nullptr;
         }
@@ -154,13 +169,13 @@ static int pipe_ready(Handle* h, uint64_t bytes) {
     }
     HostPipe* hp = h->pipe;
     if (hp->slot_bytes >= bytes) return WGFFT_OK;
-    for (int i = 0; i < HostPipe::kSlots; ++i) {
+    for (int i = 0; i < hp->slots; ++i) {
         if (hp->pin_in[i]) hipHostFree(hp->pin_in[i]);
         if (hp->pin_out[i]) hipHostFree(hp->pin_out[i]);
         hp->pin_in[i] = hp->pin_out[i] = nullptr;
     }
     hp->slot_bytes = 0;
-    for (int i = 0; i < HostPipe::kSlots; ++i)
+    for (int i = 0; i < hp->slots; ++i)
         if ((e = hipHostMalloc(&hp->pin_in[i], bytes, hipHostMallocDefault)) != hipSuccess ||
             (e = hipHostMalloc(&hp->pin_out[i], bytes, hipHostMallocDefault)) != hipSuccess) {
             (void)hipGetLastError();
@@ -183,8 +198,9 @@ int Handle::run_host(const float* in, float* out, uint64_t n, uint64_t batch,
     const uint64_t slot = cb * tb;
     int rc;
     if ((rc = pipe_ready(this, slot))) return rc;
-    if ((rc = ensure(io_in, HostPipe::kSlots * slot)) || (rc = ensure(io_out, HostPipe::kSlots * slot))) return rc;
     HostPipe* hp = pipe;
+    const int K = hp->slots;
+    if ((rc = ensure(io_in, K * slot)) || (rc = ensure(io_out, K * slot))) return rc;
     const uint64_t chunks = (batch + cb - 1) / cb;
     auto bytes_of = [&](uint64_t i) { return std::min(cb, batch - i * cb) * tb; };
     auto dev_in = [&](int s) { return (float2*)((char*)io_in.p + s * slot); };
@@ -195,20 +211,28 @@ int Handle::run_host(const float* in, float* out, uint64_t n, uint64_t batch,
         hipStreamSynchronize(stream);
         hipStreamSynchronize(hp->s_out);
     };
-    for (uint64_t i = 0; i <= chunks; ++i) {
+    // Chunk i goes through slot i % K; the host copies chunk i in, enqueues
+    // its upload, transform and read-back, then copies chunk i - (K - 1)
+    // out, so K - 1 chunks' DMA and transforms are in flight while it copies
+    // (with one, each iteration waited ~0.57 ms for the previous chunk's
+    // upload + transform + read-back of 16 MB).  Slot s is reused by chunk i
+    // only after chunk i - K's upload, transform and read-back (events) and
+    // its copy out (iteration i - 1 at the latest).
+    const uint64_t lag = (uint64_t)K - 1;
+    for (uint64_t i = 0; i < chunks + lag; ++i) {
         if (i < chunks) {
-            const int s = (int)(i % HostPipe::kSlots);
+            const int s = (int)(i % K);
             const uint64_t bytes = bytes_of(i);
-            // pinned slot s was last read by chunk i - 2's upload
-            if (i >= HostPipe::kSlots && (e = hipEventSynchronize(hp->ev_in[s])) != hipSuccess) break;
+            // pinned slot s was last read by chunk i - K's upload
+            if (i >= (uint64_t)K && (e = hipEventSynchronize(hp->ev_in[s])) != hipSuccess) break;
             hp->pool->copy(hp->pin_in[s], (const char*)in + i * cb * tb, bytes);
-            // device slot s: chunk i - 2's transform has read it and its result has been read back
-            if (i >= HostPipe::kSlots && (e = hipStreamWaitEvent(hp->s_in, hp->ev_run[s], 0)) != hipSuccess) break;
+            // device slot s: chunk i - K's transform has read it and its result has been read back
+            if (i >= (uint64_t)K && (e = hipStreamWaitEvent(hp->s_in, hp->ev_run[s], 0)) != hipSuccess) break;
             if ((e = hipMemcpyAsync(dev_in(s), hp->pin_in[s], bytes, hipMemcpyHostToDevice, hp->s_in)) != hipSuccess ||
                 (e = hipEventRecord(hp->ev_in[s], hp->s_in)) != hipSuccess ||
                 (e = hipStreamWaitEvent(stream, hp->ev_in[s], 0)) != hipSuccess)
                 break;
-            if (i >= HostPipe::kSlots && (e = hipStreamWaitEvent(stream, hp->ev_out[s], 0)) != hipSuccess) break;
+            if (i >= (uint64_t)K && (e = hipStreamWaitEvent(stream, hp->ev_out[s], 0)) != hipSuccess) break;
             if ((rc = run_chunk(dev_in(s), dev_out(s), bytes / tb))) {
                 drain();
                 return rc;
@@ -219,9 +243,9 @@ int Handle::run_host(const float* in, float* out, uint64_t n, uint64_t batch,
                 (e = hipEventRecord(hp->ev_out[s], hp->s_out)) != hipSuccess)
                 break;
         }
-        if (i >= 1) {  // chunk i - 1 back to the caller's buffer
-            const uint64_t j = i - 1;
-            const int s = (int)(j % HostPipe::kSlots);
+        if (i >= lag) {  // chunk i - lag back to the caller's buffer
+            const uint64_t j = i - lag;
+            const int s = (int)(j % K);
             if ((e = hipEventSynchronize(hp->ev_out[s])) != hipSuccess) break;
             hp->pool->copy((char*)out + j * cb * tb, hp->pin_out[s], bytes_of(j));
         }

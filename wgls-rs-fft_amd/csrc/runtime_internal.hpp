@@ -97,7 +97,7 @@ struct Handle {
     // staging in chunks of whole transforms of about host_chunk_bytes
     // (transforms larger than that are copied directly).
     static constexpr uint64_t kHostPipeMinBytes = 8ull << 20;
-    uint64_t host_chunk_bytes = 64ull << 20;
+    uint64_t host_chunk_bytes = 32ull << 20;
     HostPipe* pipe = nullptr;
     // mixed-length plans created on this handle and not yet destroyed; they
     // use its device and lock, so wgfft_destroy frees them with it

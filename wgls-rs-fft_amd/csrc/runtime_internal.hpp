@@ -97,7 +97,8 @@ struct Handle {
     // staging in chunks of whole transforms of about host_chunk_bytes
     // (transforms larger than that are copied directly).
     static constexpr uint64_t kHostPipeMinBytes = 8ull << 20;
-    uint64_t host_chunk_bytes = 32ull << 20;
+    uint64_t host_chunk_bytes = 32ull << 20;  // the largest chunk
+    bool host_chunk_fixed = false;            // WGFFT_HOST_CHUNK_BYTES given: every chunk that size
     HostPipe* pipe = nullptr;
     // mixed-length plans created on this handle and not yet destroyed; they
     // use its device and lock, so wgfft_destroy frees them with it

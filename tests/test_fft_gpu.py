@@ -476,13 +476,14 @@ def test_device_api_inplace_and_stream(gpu, torch_cuda):
     assert oracle.rel_l2(z.cpu().numpy(), x) <= 5e-6
 
 
-@pytest.mark.parametrize("n,batch", [(4096, 4096 + 37), (16384, 700), (64, 300000), (1 << 21, 9), (1 << 22, 3)])
+@pytest.mark.parametrize("n,batch", [(1024, 600), (4096, 4096 + 37), (16384, 700), (64, 300000), (1 << 21, 9),
+                                     (1 << 22, 3)])
 def test_host_pipeline_matches_device(gpu, torch_cuda, n, batch):
-    """Host-buffer calls of 8 MB and more stream through pinned staging in
-    chunks of whole transforms, four slots deep (here with a partial last
-    chunk, at 2^21 two transforms per 32 MiB chunk, and at 2^22 one), bit-
-    identical to the device entry point, forward, inverse, in place and
-    windowed."""
+    """Host-buffer calls of 4 MB and more stream through pinned staging in
+    chunks of whole transforms (a quarter of the call, 1 to 32 MiB), four
+    slots deep (here with partial last chunks, at 2^21 two transforms per
+    chunk and at 2^22 one): bit-identical to the device entry point,
+    forward, inverse, in place and windowed."""
     torch = torch_cuda
     L = wgfft._lib
     x = wgfft.synth_uniform((batch, n), 0x5EED0700 + n)

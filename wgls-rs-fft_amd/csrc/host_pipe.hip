@@ -188,17 +188,17 @@ static int pipe_ready(Handle* h, uint64_t bytes) {
 
 bool Handle::host_pipelined(uint64_t n, uint64_t batch) const {
     const uint64_t tb = n * sizeof(float2);
-    return tb * batch >= kHostPipeMinBytes && tb <= host_chunk_bytes;
+    return tb * batch >= host_pipe_min_bytes && tb <= host_chunk_bytes;
 }
 
 int Handle::run_host(const float* in, float* out, uint64_t n, uint64_t batch,
                      const std::function<int(const float2*, float2*, uint64_t)>& run_chunk) {
     const uint64_t tb = n * sizeof(float2);
-    // a quarter of the call, 4 .. 32 MiB, so that a call of 8-128 MB has
+    // a quarter of the call, 1 .. 32 MiB, so that a call of 4-128 MB has
     // chunks to overlap (32 MB host to host: 1.37 ms in 8 MiB chunks against
     // 2.06 ms as one; 128 MB: 4.4 ms in 32 MiB chunks against 5.8 in 8 MiB)
     uint64_t chunk_bytes = host_chunk_bytes;
-    if (!host_chunk_fixed) chunk_bytes = std::min(host_chunk_bytes, std::max<uint64_t>(4ull << 20, tb * batch / 4));
+    if (!host_chunk_fixed) chunk_bytes = std::min(host_chunk_bytes, std::max(host_chunk_min_bytes, tb * batch / 4));
     const uint64_t cb = std::max<uint64_t>(1, chunk_bytes / tb);  // transforms per chunk
     const uint64_t slot = cb * tb;
     int rc;

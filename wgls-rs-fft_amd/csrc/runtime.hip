@@ -688,6 +688,9 @@ int wgfft_create(wgfft_t** out, int device) {
     }
     noalloc_lookup(device);  // with `device` current
     if (const char* env = std::getenv("WGFFT_MP_CHUNK_BYTES")) h->mp_chunk_bytes = std::strtoull(env, nullptr, 10);
+    if (const char* env = std::getenv("WGFFT_HOST_PIPE_MIN_BYTES")) h->host_pipe_min_bytes = std::strtoull(env, nullptr, 10);
+    if (const char* env = std::getenv("WGFFT_HOST_CHUNK_MIN_BYTES"))
+        h->host_chunk_min_bytes = std::max<uint64_t>(1ull << 16, std::strtoull(env, nullptr, 10));
     if (const char* env = std::getenv("WGFFT_HOST_CHUNK_BYTES")) {
         h->host_chunk_bytes = std::max<uint64_t>(1ull << 20, std::strtoull(env, nullptr, 10));
         h->host_chunk_fixed = true;

@@ -268,8 +268,10 @@ def bench_shapes(torch, gpu, steps: int, tune: bool) -> dict:
 
 def bench_c1_and_host(torch, gpu, wgfft) -> dict:
     """C1 (N=1024 x 8, launch/latency-bound: device us per forward and
-    inverse call) and the host-buffer API (wgfft_exec_host: H2D + transform
-    + D2H, synchronous, PCIe-inclusive) at C1 and at 4096 x 4096."""
+    inverse call) and the host-buffer API (H2D + transform + D2H,
+    synchronous, PCIe-inclusive) at C1 and at 4096 x 4096: on one flat array
+    (wgfft_exec_host) and on a list of separate signals, as GpuFft::fft takes
+    them (wgfft_exec_host_rows), each call returning new arrays."""
     out = {}
     x = torch.from_numpy(wgfft.synth_uniform((8, 1024), 0x5EED0001)).cuda()
     y = torch.empty_like(x)
@@ -285,15 +287,17 @@ def bench_c1_and_host(torch, gpu, wgfft) -> dict:
                                  "via": "wgfft_bench_device"}
     for (n, batch, reps) in ((1024, 8, 201), (4096, 4096, 9)):
         xh = wgfft.synth_uniform((batch, n), 0x5EED0002)
-        gpu.fft_array(xh)  # warm (plan, staging)
-        ts = []
-        for _ in range(reps):  # the median call (host jitter spreads the tail)
-            t0 = time.perf_counter()
-            gpu.fft_array(xh)
-            ts.append(time.perf_counter() - t0)
-        dt = sorted(ts)[reps // 2]
-        out[f"host_api_{n}x{batch}"] = {"ms": round(dt * 1e3, 4),
-                                       "gsamples_s": round(n * batch / dt / 1e9, 3)}
+        rows = [xh[b].copy() for b in range(batch)]  # separate buffers, like Vec<Vec<Complex<f32>>>
+        for key, call in (("host_api", lambda: gpu.fft_array(xh)), ("host_list", lambda: gpu.fft(rows))):
+            call()  # warm (plan, staging)
+            call()
+            ts = []
+            for _ in range(reps):  # the median call (host jitter spreads the tail)
+                t0 = time.perf_counter()
+                call()
+                ts.append(time.perf_counter() - t0)
+            dt = sorted(ts)[reps // 2]
+            out[f"{key}_{n}x{batch}"] = {"ms": round(dt * 1e3, 4), "gsamples_s": round(n * batch / dt / 1e9, 3)}
     return out
 
 

@@ -95,6 +95,16 @@ int wgfft_validate_lengths(const uint64_t* lens, uint64_t count, uint64_t* n_out
  * 483-488) and transform_batch_internal (src/lib.rs:517-568). */
 int wgfft_exec_host(wgfft_t* h, const float* in, float* out, uint64_t n, uint64_t batch, int dir);
 
+/* The same with every signal in a buffer of its own, as GpuFft::fft takes
+ * and returns them (&[Vec<Complex<f32>>] in, Vec<Vec<Complex<f32>>> out):
+ * in_rows[b] and out_rows[b] each point at n complex64 values.  The staging
+ * copies gather and scatter the rows, so the batch is never flattened into
+ * one buffer.  Rows must not overlap one another; out_rows[b] may equal
+ * in_rows[b].  Replaces transform_batch_internal's flatten and re-boxing
+ * (src/lib.rs:539-544, 563-565) together with the transform. */
+int wgfft_exec_host_rows(wgfft_t* h, const float* const* in_rows, float* const* out_rows, uint64_t n, uint64_t batch,
+                         int dir);
+
 /* Same on device buffers, enqueued on `stream`.  in == out is allowed.
  * Replaces execute_compute_pass (src/lib.rs:582-628) with data resident. */
 int wgfft_exec_device(wgfft_t* h, const float* d_in, float* d_out, uint64_t n, uint64_t batch, int dir,

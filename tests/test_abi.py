@@ -47,6 +47,7 @@ def test_no_device_here_is_a_clean_error():
 def test_null_handle_paths():
     L = _lib.load()
     assert L.wgfft_exec_host(None, None, None, 8, 1, -1) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
+    assert L.wgfft_exec_host_rows(None, None, None, 8, 1, -1) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
     assert L.wgfft_exec_device(None, None, None, 8, 1, -1, None) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
     assert L.wgfft_filter_device(None, None, None, None, 8, 1, 0, 0.0, None) == wgfft.WGFFT_ERR_INVALID_ARGUMENT
     assert L.wgfft_exec_strided_device(None, None, None, 8, 1, 1, 8, 1, 8, -1, None) == wgfft.WGFFT_ERR_INVALID_ARGUMENT

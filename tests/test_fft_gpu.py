@@ -328,8 +328,36 @@ def test_error_contract_on_device(gpu):
     with pytest.raises(wgfft.WgfftError) as e:
         gpu.fft([np.zeros(12, np.complex64)])
     assert str(e.value) == "Transform length must be a non-zero power of two"
+    # the same errors for rows that are converted before the call, and for a
+    # ragged batch whose first row is the longer one
+    with pytest.raises(wgfft.WgfftError) as e:
+        gpu.ifft([[0j] * 12, [0j] * 12])
+    assert str(e.value) == "Transform length must be a non-zero power of two"
+    with pytest.raises(wgfft.WgfftError) as e:
+        gpu.fft([np.zeros(16, np.complex64), [0j] * 8])
+    assert str(e.value) == "All input vectors in a batch must have the same length"
+    with pytest.raises(wgfft.WgfftError) as e:
+        gpu.fft([np.zeros(0, np.complex64)] * 2)
+    assert e.value.code == wgfft.WGFFT_ERR_INVALID_LENGTH
     assert gpu.name()
     assert wgfft.is_gpu_available()
+
+
+def test_list_inputs_of_every_form(gpu):
+    """fft() takes any sequence of signals: contiguous complex64 rows go to
+    the library where they lie, other rows (complex128, strided views,
+    Python lists) are converted first; every form gives the same result."""
+    x = wgfft.synth_uniform((5, 256), 0x5EED0900)
+    want = gpu.fft(list(x))
+    assert oracle.rel_l2(np.stack(want), oracle.fft64(x, -1)) <= rel_bar(256)
+    wide = np.zeros((5, 512), np.complex64)
+    wide[:, ::2] = x
+    for form in (x, tuple(x), [r.astype(np.complex128) for r in x], list(wide[:, ::2]), [list(r) for r in x],
+                 [x[0], x[1].astype(np.complex128), x[2], list(x[3]), x[4]]):
+        got = gpu.fft(form)
+        assert len(got) == 5 and all(g.dtype == np.complex64 and np.array_equal(g, w) for g, w in zip(got, want))
+    back = gpu.ifft(want)
+    assert oracle.rel_l2(np.stack(back), x) <= rel_bar(256)
 
 
 def test_alloc_cap_out_of_memory(gpu):
@@ -506,6 +534,37 @@ def test_host_pipeline_matches_device(gpu, torch_cuda, n, batch):
     yw = np.empty_like(x)
     L.check(gpu._L.wgfft_exec_windowed_host(gpu._h, x.ctypes.data, w.ctypes.data, yw.ctypes.data, n, batch))
     assert np.array_equal(yw, d.cpu().numpy())
+
+
+@pytest.mark.parametrize("n,batch", [(1024, 8), (256, 700), (4096, 4096 + 37), (1 << 22, 3), (1 << 23, 2)])
+def test_host_rows_match_flat(gpu, n, batch):
+    """wgfft_exec_host_rows, the entry behind fft()/ifft() on lists of long
+    signals: the staging copies gather each signal from its own buffer and
+    scatter each result into its own (on the calling thread below 256 KB,
+    split over the copy threads above, with rows straddling the threads'
+    shares; below 4 MB as one chunk on the handle's stream, above through the
+    pipeline; at 2^23 each transform is larger than a staging chunk and moves
+    on its own).  Bit-identical to
+    wgfft_exec_host on the flattened batch, forward and inverse, also in
+    place (out_rows == in_rows) on buffers allocated last row first."""
+    L = wgfft._lib
+    x = wgfft.synth_uniform((batch, n), 0x5EED0800 + n)
+    rows = [x[b].copy() for b in range(batch)]  # separate buffers
+    for inverse in (False, True):
+        dirn = L.WGFFT_INVERSE if inverse else L.WGFFT_FORWARD
+        want = gpu.fft_array(x, inverse=inverse)
+        got = gpu.ifft(rows) if inverse else gpu.fft(rows)
+        assert len(got) == batch and all(np.array_equal(g, w) for g, w in zip(got, want)), inverse
+        bufs = [None] * batch
+        for b in reversed(range(batch)):  # allocated last row first
+            bufs[b] = rows[b].copy()
+        ptrs = np.array([b.ctypes.data for b in bufs], dtype=np.uint64)
+        L.check(gpu._L.wgfft_exec_host_rows(gpu._h, ptrs.ctypes.data, ptrs.ctypes.data, n, batch, dirn))
+        assert all(np.array_equal(b, w) for b, w in zip(bufs, want)), inverse
+    assert np.array_equal(rows[batch // 2], x[batch // 2])  # inputs untouched
+    null_rows = np.zeros(batch, dtype=np.uint64)
+    with pytest.raises(wgfft.WgfftError):
+        L.check(gpu._L.wgfft_exec_host_rows(gpu._h, null_rows.ctypes.data, null_rows.ctypes.data, n, batch, 1))
 
 
 @pytest.mark.parametrize("n,batch", [(256, 33), (4096, 17), (1 << 17, 3)])

@@ -19,6 +19,7 @@ C_TO_RUST = {
     "const int*": "*const c_int", "wgfft_t*": "*mut wgfft_t", "wgfft_t**": "*mut *mut wgfft_t",
     "wgfft_ragged_t*": "*mut wgfft_ragged_t", "const wgfft_ragged_t*": "*const wgfft_ragged_t",
     "wgfft_ragged_t**": "*mut *mut wgfft_ragged_t",
+    "const float* const*": "*const *const f32", "float* const*": "*const *mut f32",
 }
 
 

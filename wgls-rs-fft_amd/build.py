@@ -52,6 +52,15 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     shutil.move(tmp, LIB)
+    # the list entry points' bridge (a CPython extension, no HIP code)
+    import sysconfig
+
+    rows = os.path.join(OUT_DIR, "_rows" + sysconfig.get_config_var("EXT_SUFFIX"))
+    cmd = ["gcc", "-O2", "-shared", "-fPIC", "-Wall", "-I", sysconfig.get_paths()["include"], "-o", rows,
+           os.path.join(HERE, "wgfft", "_rows.c")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"_rows failed to build:\n{r.stdout}\n{r.stderr}")
     # C++ mirror tests, linked against the library with an $ORIGIN-relative rpath
     cxx_tests = os.path.join(OUT_DIR, "gpu_fft_tests")
     cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", cxx_tests, os.path.join(HERE, "cpp", "gpu_fft_tests.cpp"),

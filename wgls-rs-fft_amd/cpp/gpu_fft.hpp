@@ -107,13 +107,22 @@ private:
         return out;
     }
     Batch transform(const Batch& inputs, int dir) {
+        std::vector<uint64_t> lens(inputs.size());
+        for (size_t i = 0; i < inputs.size(); ++i) lens[i] = inputs[i].size();
         uint64_t n = 0;
-        std::vector<Complex> flat = flatten(inputs, &n);
+        check(wgfft_validate_lengths(lens.data(), lens.size(), &n));
         if (inputs.empty()) return {};
-        std::vector<Complex> out(flat.size());
-        check(wgfft_exec_host(h_, reinterpret_cast<const float*>(flat.data()), reinterpret_cast<float*>(out.data()),
-                              n, inputs.size(), dir));
-        return split(out, n, inputs.size());
+        // every signal where it lies: the library's staging copies gather and
+        // scatter the rows (wgfft_exec_host_rows), nothing is flattened
+        Batch out(inputs.size(), std::vector<Complex>(n));
+        std::vector<const float*> in_rows(inputs.size());
+        std::vector<float*> out_rows(inputs.size());
+        for (size_t b = 0; b < inputs.size(); ++b) {
+            in_rows[b] = reinterpret_cast<const float*>(inputs[b].data());
+            out_rows[b] = reinterpret_cast<float*>(out[b].data());
+        }
+        check(wgfft_exec_host_rows(h_, in_rows.data(), out_rows.data(), n, inputs.size(), dir));
+        return out;
     }
 
     wgfft_t* h_ = nullptr;

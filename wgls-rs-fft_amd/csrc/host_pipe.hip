@@ -30,8 +30,11 @@
 
 namespace wgfft {
 
-// A fixed set of threads that split one memcpy between them; the calling
-// thread takes a share too and returns when every share is done.
+// A fixed set of threads that split one copy between them; the calling
+// thread takes a share too and returns when every share is done.  A copy is
+// `bytes` of one contiguous region, or of rows of row_bytes each gathered
+// from (src_rows) or scattered to (dst_rows) separate buffers: the byte range
+// is split evenly whatever the row size.
 class CopyPool {
 public:
     explicit CopyPool(int helpers) {
@@ -43,23 +46,39 @@ public:
         }
     }
     ~CopyPool() { stop_all(); }
-    void copy(void* dst, const void* src, uint64_t bytes) {
-        const int parts = (int)threads_.size() + 1;
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            dst_ = (char*)dst;
-            src_ = (const char*)src;
-            bytes_ = bytes;
-            pending_ = parts - 1;
-            ++gen_;
-        }
-        cv_.notify_all();
-        share(0, parts);
-        std::unique_lock<std::mutex> lk(mu_);
-        done_cv_.wait(lk, [this] { return pending_ == 0; });
+    void copy(void* dst, const void* src, uint64_t bytes) { run((char*)dst, (const char*)src, nullptr, nullptr, bytes, 0); }
+    void gather(void* dst, const float* const* src_rows, uint64_t row_bytes, uint64_t rows) {
+        run((char*)dst, nullptr, (const char* const*)src_rows, nullptr, row_bytes * rows, row_bytes);
+    }
+    void scatter(float* const* dst_rows, const void* src, uint64_t row_bytes, uint64_t rows) {
+        run(nullptr, (const char*)src, nullptr, (char* const*)dst_rows, row_bytes * rows, row_bytes);
     }
 
 private:
+    void run(char* dst, const char* src, const char* const* src_rows, char* const* dst_rows, uint64_t bytes,
+             uint64_t row_bytes) {
+        // a small copy stays on the calling thread: waking the helpers costs
+        // more than they save below ~256 KB
+        const int parts = bytes < kInlineBytes ? 1 : (int)threads_.size() + 1;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            dst_ = dst;
+            src_ = src;
+            src_rows_ = src_rows;
+            dst_rows_ = dst_rows;
+            row_bytes_ = row_bytes;
+            bytes_ = bytes;
+            if (parts > 1) {
+                pending_ = parts - 1;
+                ++gen_;
+            }
+        }
+        if (parts > 1) cv_.notify_all();
+        share(0, parts);
+        if (parts == 1) return;
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [this] { return pending_ == 0; });
+    }
     void stop_all() {
         {
             std::lock_guard<std::mutex> lk(mu_);
@@ -70,10 +89,24 @@ private:
         threads_.clear();
     }
     void share(int k, int parts) {
-        // 4 KB-aligned shares
+        // 4 KB-aligned shares of the byte range
         const uint64_t per = ((bytes_ / parts) + 4095) & ~4095ull;
         const uint64_t b = std::min<uint64_t>(bytes_, per * k), e = std::min<uint64_t>(bytes_, b + per);
-        if (e > b) std::memcpy(dst_ + b, src_ + b, e - b);
+        if (e <= b) return;
+        if (!src_rows_ && !dst_rows_) {
+            std::memcpy(dst_ + b, src_ + b, e - b);
+            return;
+        }
+        // the rows overlapping [b, e), each from its own buffer
+        for (uint64_t r = b / row_bytes_, pos = b; pos < e; ++r) {
+            const uint64_t off = pos - r * row_bytes_;
+            const uint64_t len = std::min(e, (r + 1) * row_bytes_) - pos;
+            if (src_rows_)
+                std::memcpy(dst_ + pos, src_rows_[r] + off, len);
+            else
+                std::memcpy(dst_rows_[r] + off, src_ + pos, len);
+            pos += len;
+        }
     }
     void work(int k) {
         uint64_t seen = 0;
@@ -89,6 +122,7 @@ private:
             if (--pending_ == 0) done_cv_.notify_one();
         }
     }
+    static constexpr uint64_t kInlineBytes = 256ull << 10;
     std::vector<std::thread> threads_;
     std::mutex mu_;
     std::condition_variable cv_, done_cv_;
@@ -97,6 +131,9 @@ private:
     int pending_ = 0;
     char* dst_ = nullptr;
     const char* src_ = nullptr;
+    const char* const* src_rows_ = nullptr;
+    char* const* dst_rows_ = nullptr;
+    uint64_t row_bytes_ = 0;
     uint64_t bytes_ = 0;
 };
 
@@ -186,14 +223,40 @@ static int pipe_ready(Handle* h, uint64_t bytes) {
     return WGFFT_OK;
 }
 
-bool Handle::host_pipelined(uint64_t n, uint64_t batch) const {
-    const uint64_t tb = n * sizeof(float2);
-    return tb * batch >= host_pipe_min_bytes && tb <= host_chunk_bytes;
-}
-
 int Handle::run_host(const float* in, float* out, uint64_t n, uint64_t batch,
-                     const std::function<int(const float2*, float2*, uint64_t)>& run_chunk) {
+                     const std::function<int(const float2*, float2*, uint64_t)>& run_chunk,
+                     const float* const* in_rows, float* const* out_rows) {
     const uint64_t tb = n * sizeof(float2);
+    int rc;
+    if (tb * batch < host_pipe_min_bytes) {
+        // one chunk on the handle's stream alone: below the pipelining
+        // threshold the copy streams' event hand-offs cost more than the
+        // overlap saves
+        const uint64_t total = tb * batch;
+        if ((rc = pipe_ready(this, total))) return rc;
+        HostPipe* hp = pipe;
+        if ((rc = ensure(io_in, total)) || (rc = ensure(io_out, total))) return rc;
+        if (in_rows)
+            hp->pool->gather(hp->pin_in[0], in_rows, tb, batch);
+        else
+            hp->pool->copy(hp->pin_in[0], in, total);
+        hipError_t e = hipMemcpyAsync(io_in.p, hp->pin_in[0], total, hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(H2D)");
+        if ((rc = run_chunk((const float2*)io_in.p, (float2*)io_out.p, batch))) {
+            hipStreamSynchronize(stream);
+            return rc;
+        }
+        if ((e = hipMemcpyAsync(hp->pin_out[0], io_out.p, total, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(stream)) != hipSuccess) {
+            hipStreamSynchronize(stream);
+            return hip_fail(e, "host staging");
+        }
+        if (out_rows)
+            hp->pool->scatter(out_rows, hp->pin_out[0], tb, batch);
+        else
+            hp->pool->copy(out, hp->pin_out[0], total);
+        return WGFFT_OK;
+    }
     // a quarter of the call, 1 .. 32 MiB, so that a call of 4-128 MB has
     // chunks to overlap (32 MB host to host: 1.37 ms in 8 MiB chunks against
     // 2.06 ms as one; 128 MB: 4.4 ms in 32 MiB chunks against 5.8 in 8 MiB)
@@ -201,7 +264,6 @@ int Handle::run_host(const float* in, float* out, uint64_t n, uint64_t batch,
     if (!host_chunk_fixed) chunk_bytes = std::min(host_chunk_bytes, std::max(host_chunk_min_bytes, tb * batch / 4));
     const uint64_t cb = std::max<uint64_t>(1, chunk_bytes / tb);  // transforms per chunk
     const uint64_t slot = cb * tb;
-    int rc;
     if ((rc = pipe_ready(this, slot))) return rc;
     HostPipe* hp = pipe;
     const int K = hp->slots;
@@ -230,7 +292,10 @@ int Handle::run_host(const float* in, float* out, uint64_t n, uint64_t batch,
             const uint64_t bytes = bytes_of(i);
             // pinned slot s was last read by chunk i - K's upload
             if (i >= (uint64_t)K && (e = hipEventSynchronize(hp->ev_in[s])) != hipSuccess) break;
-            hp->pool->copy(hp->pin_in[s], (const char*)in + i * cb * tb, bytes);
+            if (in_rows)
+                hp->pool->gather(hp->pin_in[s], in_rows + i * cb, tb, bytes / tb);
+            else
+                hp->pool->copy(hp->pin_in[s], (const char*)in + i * cb * tb, bytes);
             // device slot s: chunk i - K's transform has read it and its result has been read back
             if (i >= (uint64_t)K && (e = hipStreamWaitEvent(hp->s_in, hp->ev_run[s], 0)) != hipSuccess) break;
             if ((e = hipMemcpyAsync(dev_in(s), hp->pin_in[s], bytes, hipMemcpyHostToDevice, hp->s_in)) != hipSuccess ||
@@ -252,7 +317,10 @@ int Handle::run_host(const float* in, float* out, uint64_t n, uint64_t batch,
             const uint64_t j = i - lag;
             const int s = (int)(j % K);
             if ((e = hipEventSynchronize(hp->ev_out[s])) != hipSuccess) break;
-            hp->pool->copy((char*)out + j * cb * tb, hp->pin_out[s], bytes_of(j));
+            if (out_rows)
+                hp->pool->scatter(out_rows + j * cb, hp->pin_out[s], tb, bytes_of(j) / tb);
+            else
+                hp->pool->copy((char*)out + j * cb * tb, hp->pin_out[s], bytes_of(j));
         }
     }
     if (e != hipSuccess) {

@@ -852,7 +852,7 @@ int wgfft_exec_host(wgfft_t* h, const float* in, float* out, uint64_t n, uint64_
     Plan* p;
     if ((rc = h->get_plan(n, &p))) return rc;
     bool inv = dir == WGFFT_INVERSE;
-    if (h->host_pipelined(n, batch)) {
+    if (h->host_staged(n)) {
         rc = h->run_host(in, out, n, batch, [&](const float2* d_in, float2* d_out, uint64_t b) {
             return h->run(p, inv ? kInverse : kForward, d_in, d_out, nullptr, b, inv ? 1.0f / (float)n : 1.0f,
                           h->stream);
@@ -868,6 +868,49 @@ int wgfft_exec_host(wgfft_t* h, const float* in, float* out, uint64_t n, uint64_
         return rc;
     if ((e = hipMemcpyAsync(out, h->io_out.p, bytes, hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
         return hip_fail(e, "hipMemcpyAsync(D2H)");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    h->scratch_settled(h->stream);
+    return WGFFT_OK;
+}
+
+int wgfft_exec_host_rows(wgfft_t* h, const float* const* in_rows, float* const* out_rows, uint64_t n, uint64_t batch,
+                         int dir) {
+    if (!h) return fail(WGFFT_ERR_INVALID_ARGUMENT, "handle is NULL");
+    int rc;
+    if ((rc = check_len(n)) || (rc = check_dir(dir)) || (rc = check_size(n, batch))) return rc;
+    if (batch == 0) return WGFFT_OK;
+    if (!in_rows || !out_rows) return fail(WGFFT_ERR_INVALID_ARGUMENT, "row pointer array is NULL");
+    for (uint64_t b = 0; b < batch; ++b)
+        if (!in_rows[b] || !out_rows[b]) return fail(WGFFT_ERR_INVALID_ARGUMENT, "host buffer is NULL");
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    if ((e = h->scratch_quiesce()) != hipSuccess) return hip_fail(e, "stream ordering of the scratch buffers");
+    Plan* p;
+    if ((rc = h->get_plan(n, &p))) return rc;
+    const bool inv = dir == WGFFT_INVERSE;
+    const float scale = inv ? 1.0f / (float)n : 1.0f;
+    const uint64_t tb = n * sizeof(float2);
+    if (h->host_staged(n)) {  // the staging copies gather and scatter the rows
+        rc = h->run_host(nullptr, nullptr, n, batch,
+                         [&](const float2* d_in, float2* d_out, uint64_t b) {
+                             return h->run(p, inv ? kInverse : kForward, d_in, d_out, nullptr, b, scale, h->stream);
+                         },
+                         in_rows, out_rows);
+        if (rc == WGFFT_OK) h->scratch_settled(h->stream);
+        return rc;
+    }
+    // transforms larger than a staging chunk: one at a time, each row copied directly
+    if ((rc = h->ensure(h->io_in, tb)) || (rc = h->ensure(h->io_out, tb))) return rc;
+    for (uint64_t b = 0; b < batch; ++b) {
+        if ((e = hipMemcpyAsync(h->io_in.p, in_rows[b], tb, hipMemcpyHostToDevice, h->stream)) != hipSuccess)
+            return hip_fail(e, "hipMemcpyAsync(H2D)");
+        if ((rc = h->run(p, inv ? kInverse : kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, 1,
+                         scale, h->stream)))
+            return rc;
+        if ((e = hipMemcpyAsync(out_rows[b], h->io_out.p, tb, hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
+            return hip_fail(e, "hipMemcpyAsync(D2H)");
+    }
     if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     h->scratch_settled(h->stream);
     return WGFFT_OK;
@@ -906,7 +949,7 @@ int wgfft_exec_windowed_host(wgfft_t* h, const float* x, const float* w, float* 
     if ((rc = h->get_plan(n, &p))) return rc;
     if ((e = hipMemcpyAsync(h->io_w.p, w, n * sizeof(float), hipMemcpyHostToDevice, h->stream)) != hipSuccess)
         return hip_fail(e, "hipMemcpyAsync(H2D)");
-    if (h->host_pipelined(n, batch)) {
+    if (h->host_staged(n)) {
         rc = h->run_host(x, y, n, batch, [&](const float2* d_in, float2* d_out, uint64_t b) {
             return h->run(p, kWindowed, d_in, d_out, (const float*)h->io_w.p, b, 1.0f, h->stream);
         });

@@ -95,9 +95,12 @@ struct Handle {
     uint64_t mp_chunk_bytes = 2ull << 30;  // batch chunk of the multipass path (bounds its scratch)
     // Host-buffer calls of at least host_pipe_min_bytes move through pinned
     // staging in chunks of whole transforms of a quarter of the call, between
-    // host_chunk_min_bytes and host_chunk_bytes (transforms larger than that
-    // are copied directly).  Measured: 1-2 MB calls are faster copied
-    // directly, 4 MB calls through 1 MiB chunks (0.31 against 0.39 ms).
+    // host_chunk_min_bytes and host_chunk_bytes, pipelined over two copy
+    // streams; smaller calls stage as one chunk on the handle's stream
+    // (transforms larger than a chunk are copied directly).  Measured: 4 MB
+    // calls through 1 MiB chunks 0.31 against 0.39 ms as one; 2 MB calls as
+    // one staged chunk 0.16 against 0.21 ms copied directly from pageable
+    // memory.
     uint64_t host_pipe_min_bytes = 4ull << 20;   // WGFFT_HOST_PIPE_MIN_BYTES
     uint64_t host_chunk_min_bytes = 1ull << 20;  // WGFFT_HOST_CHUNK_MIN_BYTES
     uint64_t host_chunk_bytes = 32ull << 20;  // the largest chunk
@@ -142,11 +145,17 @@ struct Handle {
             uint64_t idist = 0, uint64_t odist = 0);
     int run_passes(Plan* p, Mode mode, const float2* in, float2* out, const float* win, uint64_t batch, float scale,
                    hipStream_t s, float2* buf_a, float2* buf_b, uint64_t chunk);
-    bool host_pipelined(uint64_t n, uint64_t batch) const;
+    // host calls of transforms up to a staging chunk go through pinned
+    // staging (run_host); larger transforms are copied directly
+    bool host_staged(uint64_t n) const { return n * sizeof(float2) <= host_chunk_bytes; }
     // Host in -> host out through the staging pipeline; run_chunk(d_in, d_out,
-    // transforms) enqueues one chunk's transform on `stream`.
+    // transforms) enqueues one chunk's transform on `stream`.  Calls below
+    // host_pipe_min_bytes stage as one chunk on `stream`.  With in_rows /
+    // out_rows the signals are separate buffers, gathered and scattered by the
+    // staging copies (in / out unused).
     int run_host(const float* in, float* out, uint64_t n, uint64_t batch,
-                 const std::function<int(const float2*, float2*, uint64_t)>& run_chunk);
+                 const std::function<int(const float2*, float2*, uint64_t)>& run_chunk,
+                 const float* const* in_rows = nullptr, float* const* out_rows = nullptr);
     void free_pipe();
 };
 

@@ -154,21 +154,16 @@ impl GpuFft {
         if inputs.is_empty() {
             return Ok(Vec::new());
         }
-        // Complex<f32> is #[repr(C)] { re, im }: the flattened batch is the
-        // interleaved f32 buffer the library takes
-        let flat: Vec<Complex<f32>> = inputs.concat();
-        let mut out = vec![Complex::new(0.0f32, 0.0f32); flat.len()];
+        // Complex<f32> is #[repr(C)] { re, im }: each Vec is the interleaved
+        // f32 row the library takes; the staging copies gather the input rows
+        // and scatter into the output Vecs, so nothing is flattened or re-boxed
+        let mut out: Vec<Vec<Complex<f32>>> = (0..inputs.len()).map(|_| vec![Complex::new(0.0f32, 0.0f32); n]).collect();
+        let in_rows: Vec<*const f32> = inputs.iter().map(|v| v.as_ptr() as *const f32).collect();
+        let out_rows: Vec<*mut f32> = out.iter_mut().map(|v| v.as_mut_ptr() as *mut f32).collect();
         check(unsafe {
-            sys::wgfft_exec_host(
-                self.h,
-                flat.as_ptr() as *const f32,
-                out.as_mut_ptr() as *mut f32,
-                n as u64,
-                inputs.len() as u64,
-                dir,
-            )
+            sys::wgfft_exec_host_rows(self.h, in_rows.as_ptr(), out_rows.as_ptr(), n as u64, inputs.len() as u64, dir)
         })?;
-        Ok(out.chunks_exact(n).map(|c| c.to_vec()).collect())
+        Ok(out)
     }
 
     /// Transform `batch` signals of length `n` already in device memory,

@@ -37,6 +37,14 @@ extern "C" {
     pub fn wgfft_name() -> *const c_char;
     pub fn wgfft_validate_lengths(lens: *const u64, count: u64, n_out: *mut u64) -> c_int;
     pub fn wgfft_exec_host(h: *mut wgfft_t, input: *const f32, out: *mut f32, n: u64, batch: u64, dir: c_int) -> c_int;
+    pub fn wgfft_exec_host_rows(
+        h: *mut wgfft_t,
+        in_rows: *const *const f32,
+        out_rows: *const *mut f32,
+        n: u64,
+        batch: u64,
+        dir: c_int,
+    ) -> c_int;
     pub fn wgfft_exec_device(
         h: *mut wgfft_t,
         d_in: *const f32,

@@ -24,7 +24,7 @@ WGFFT_INVERSE = 1
 # Every symbol include/wgfft.h declares.
 EXPORTS = (
     "wgfft_device_available", "wgfft_create", "wgfft_destroy", "wgfft_last_error", "wgfft_name",
-    "wgfft_validate_lengths", "wgfft_exec_host", "wgfft_exec_device", "wgfft_exec_device_noalloc",
+    "wgfft_validate_lengths", "wgfft_exec_host", "wgfft_exec_host_rows", "wgfft_exec_device", "wgfft_exec_device_noalloc",
     "wgfft_exec_windowed_host",
     "wgfft_exec_windowed_device", "wgfft_vjp_windowed_device", "wgfft_fwd_adjoint_masked_device",
     "wgfft_filter_device", "wgfft_exec_strided_device", "wgfft_add_noise_device",
@@ -78,6 +78,7 @@ def load() -> ctypes.CDLL:
     L.wgfft_name.restype = ctypes.c_char_p
     L.wgfft_validate_lengths.argtypes = [ctypes.POINTER(_u64), _u64, ctypes.POINTER(_u64)]
     L.wgfft_exec_host.argtypes = [_vp, _vp, _vp, _u64, _u64, _i]
+    L.wgfft_exec_host_rows.argtypes = [_vp, _vp, _vp, _u64, _u64, _i]
     L.wgfft_exec_device.argtypes = [_vp, _vp, _vp, _u64, _u64, _i, _vp]
     L.wgfft_exec_device_noalloc.argtypes = [_vp, _vp, _vp, _u64, _u64, _i, _vp]
     L.wgfft_exec_windowed_host.argtypes = [_vp, _vp, _vp, _vp, _u64, _u64]
@@ -111,6 +112,29 @@ def load() -> ctypes.CDLL:
     L.wgfft_set_alloc_cap.restype = None
     _lib = L
     return L
+
+
+_rows = None
+
+
+def rows_bridge():
+    """The _rows extension (wgfft/_lib/_rows*.so, built with the library:
+    calls wgfft_exec_host_rows on a list of numpy rows, reading their
+    addresses through the buffer protocol) and that entry point's address."""
+    global _rows
+    if _rows is None:
+        import importlib.machinery
+        import importlib.util
+        import sysconfig
+
+        path = os.path.join(_HERE, "_lib", "_rows" + sysconfig.get_config_var("EXT_SUFFIX"))
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is missing: build it with `python wgls-rs-fft_amd/build.py`")
+        loader = importlib.machinery.ExtensionFileLoader("wgfft._rows", path)
+        mod = importlib.util.module_from_spec(importlib.util.spec_from_file_location("wgfft._rows", path, loader=loader))
+        loader.exec_module(mod)
+        _rows = (mod, ctypes.cast(load().wgfft_exec_host_rows, ctypes.c_void_p).value)
+    return _rows
 
 
 def last_error() -> str:

@@ -31,9 +31,10 @@ from ._lib import WGFFT_FORWARD, WGFFT_INVERSE, WgfftError, check
 def validate_lengths(lengths: Sequence[int]) -> int:
     """Return the common length n (0 for an empty batch) or raise WgfftError."""
     L = _lib.load()
-    arr = (ctypes.c_uint64 * max(1, len(lengths)))(*[int(v) for v in lengths])
+    arr = np.zeros(1, dtype=np.uint64) if not len(lengths) else np.asarray(lengths, dtype=np.uint64)
     n = ctypes.c_uint64(0)
-    check(L.wgfft_validate_lengths(arr, len(lengths), ctypes.byref(n)))
+    check(L.wgfft_validate_lengths(arr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(lengths),
+                                   ctypes.byref(n)))
     return int(n.value)
 
 
@@ -42,8 +43,9 @@ def is_gpu_available() -> bool:
     return bool(_lib.load().wgfft_device_available())
 
 
-def _flatten(inputs: Sequence) -> tuple[np.ndarray, int]:
-    n = validate_lengths([len(x) for x in inputs])
+def _flatten(inputs: Sequence, n: int = None) -> tuple[np.ndarray, int]:
+    if n is None:
+        n = validate_lengths([len(x) for x in inputs])
     if not inputs:
         return np.empty((0, 0), dtype=np.complex64), 0
     flat = np.empty((len(inputs), n), dtype=np.complex64)
@@ -133,11 +135,28 @@ class GpuFft(FftExecutor):
         return list(out)
 
     def _transform(self, inputs: Sequence, direction: int) -> List[np.ndarray]:
-        flat, n = _flatten(inputs)
-        if flat.shape[0] == 0:
+        """One wgfft_exec_host_rows call over the signals where they lie (the
+        staging copies gather them: no flattened copy of the batch), results
+        as views of one output array.  Signals that are not contiguous
+        complex64 arrays are converted first.  The lengths are checked in
+        full (validate_lengths, the reference's errors in its order) only when
+        a row does not have the first row's length or the call fails: the
+        bridge rejects rows of another length and the library a length that
+        is not a power of two."""
+        if len(inputs) == 0:
             return []
-        out = np.empty_like(flat)
-        check(self._L.wgfft_exec_host(self._h, flat.ctypes.data, out.ctypes.data, n, flat.shape[0], direction))
+        n = len(inputs[0])
+        bridge, fn = _lib.rows_bridge()
+        out = np.empty((len(inputs), n), dtype=np.complex64)
+        h = self._h.value if self._h else 0
+        rc, bad = bridge.exec_rows(fn, h, inputs, out.ctypes.data, n, direction)
+        if bad >= 0 or rc != _lib.WGFFT_OK:
+            validate_lengths(list(map(len, inputs)))  # raises for a ragged batch or a bad length
+        if bad >= 0:
+            rows = [np.ascontiguousarray(x, dtype=np.complex64) for x in inputs]
+            rc, bad = bridge.exec_rows(fn, h, rows, out.ctypes.data, n, direction)
+            assert bad < 0
+        check(rc)
         return list(out)
 
     # ---- batched arrays (one contiguous buffer, TODO.md:7 of the reference)

@@ -536,6 +536,44 @@ def test_host_pipeline_matches_device(gpu, torch_cuda, n, batch):
     assert np.array_equal(yw, d.cpu().numpy())
 
 
+def test_host_calls_from_several_threads(gpu):
+    """fft()/ifft() release the GIL for the library call, so Python threads
+    can call one handle at once: the handle's lock serializes them and every
+    result is bit-identical to the same call made alone, on a shared handle
+    and on a handle per thread, with calls small (one staged chunk) and
+    large (pipelined)."""
+    import threading
+
+    cases = [(wgfft.synth_uniform((b, n), 0x5EED0A00 + n), inv) for n, b, inv in
+             ((1024, 64, False), (4096, 600, True), (16384, 128, False), (256, 3000, True))]
+    want = [(gpu.ifft if inv else gpu.fft)(list(x)) for x, inv in cases]
+    own = [wgfft.GpuFft.new() for _ in range(2)]
+    errors = []
+
+    def worker(k, h):
+        try:
+            for rep in range(6):
+                i = (k + rep) % len(cases)
+                x, inv = cases[i]
+                got = (h.ifft if inv else h.fft)(list(x))
+                if not all(np.array_equal(g, w) for g, w in zip(got, want[i])):
+                    errors.append((k, rep, i))
+        except Exception as e:  # reported below with the worker's index
+            errors.append((k, repr(e)))
+
+    try:
+        threads = [threading.Thread(target=worker, args=(k, gpu if k < 4 else own[k - 4])) for k in range(6)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(timeout=120)
+        assert not any(t.is_alive() for t in threads)
+        assert not errors, errors
+    finally:
+        for h in own:
+            h.close()
+
+
 @pytest.mark.parametrize("n,batch", [(1024, 8), (256, 700), (4096, 4096 + 37), (1 << 22, 3), (1 << 23, 2)])
 def test_host_rows_match_flat(gpu, n, batch):
     """wgfft_exec_host_rows, the entry behind fft()/ifft() on lists of long

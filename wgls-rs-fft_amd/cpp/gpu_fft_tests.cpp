@@ -4,6 +4,7 @@
 //   gpu_fft_tests cpu   error contract without a device
 //   gpu_fft_tests gpu   transforms on the GPU
 // Prints one line per check and exits non-zero on the first failure.
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -206,6 +207,34 @@ static int run_gpu() {
         CHECK(max_diff(fft.ifft(fft.fft(s3))[0], s3[0]) < 1e-3f, "three-pass split round trip");
         CHECK(wgfft_autotune(fft.handle(), 1024, 512, desc, sizeof desc) == WGFFT_OK, "autotune 1024");
         CHECK(wgfft_autotune(fft.handle(), 1u << 17, 4, desc, sizeof desc) == WGFFT_OK, "autotune 2^17");
+    }
+    // the rows entry behind fft()/ifft() (wgfft_exec_host_rows): bit-identical
+    // to the flat entry on the same signals, in place too, and for transforms
+    // larger than a staging chunk
+    {
+        const size_t n = 256, batch = 700;  // 1.4 MB: one staged chunk, copies split over the copy threads
+        Batch in(batch);
+        for (size_t b = 0; b < batch; ++b) {
+            in[b] = make_input(n);
+            in[b][b % n] = Complex((float)b, -1.0f);
+        }
+        Batch out = fft.fft(in);
+        std::vector<Complex> flat, fo(n * batch);
+        for (const auto& v : in) flat.insert(flat.end(), v.begin(), v.end());
+        check(wgfft_exec_host(fft.handle(), reinterpret_cast<const float*>(flat.data()),
+                              reinterpret_cast<float*>(fo.data()), n, batch, WGFFT_FORWARD));
+        bool same = out.size() == batch;
+        for (size_t b = 0; b < batch && same; ++b) same = std::equal(out[b].begin(), out[b].end(), fo.begin() + b * n);
+        CHECK(same, "rows entry equals the flat entry (256 x 700)");
+        std::vector<float*> rows(batch);
+        for (size_t b = 0; b < batch; ++b) rows[b] = reinterpret_cast<float*>(in[b].data());
+        check(wgfft_exec_host_rows(fft.handle(), rows.data(), rows.data(), n, batch, WGFFT_FORWARD));
+        same = true;
+        for (size_t b = 0; b < batch && same; ++b) same = in[b] == out[b];
+        CHECK(same, "rows entry in place");
+        Batch huge(2, make_input(1u << 23));  // each transform larger than a staging chunk: row by row
+        Batch hb = fft.ifft(fft.fft(huge));
+        CHECK(max_diff(hb[1], huge[1]) < 1e-3f, "rows entry 2^23 round trip");
     }
     {
         wgfft_t* h = nullptr;

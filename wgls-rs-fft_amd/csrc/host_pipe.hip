@@ -11,7 +11,11 @@
 // streams and the transforms run on the handle's stream.  4096 x 4096 host
 // to host: 4.3-4.8 ms with 32 MiB chunks (two slots of 64 MiB: 5.4-7.0 ms;
 // 16 MiB chunks: a similar median with outliers to 11 ms, the copy threads
-// waking twice per chunk).
+// waking twice per chunk).  Calls below 4 MB stage as one chunk on the
+// handle's stream.  The signals may also be separate buffers, one pointer per
+// signal (wgfft_exec_host_rows, the reference's Vec<Vec<Complex<f32>>>): the
+// copy threads then gather them into the staging slots and scatter the
+// results, so the batch is never flattened.
 #include <hip/hip_runtime.h>
 #include <sched.h>
 

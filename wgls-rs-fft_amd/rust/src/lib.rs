@@ -4,9 +4,10 @@
 //!
 //! The transforms run in the hand-written gfx950 HIP kernels of
 //! `libwgfft.so`; this crate is the host side: it validates a batch with the
-//! reference's rules and messages, flattens `Vec<Vec<Complex<f32>>>` (already
-//! interleaved `re, im` f32) into one buffer, and calls the C-ABI of
-//! `include/wgfft.h`.  Forward is unnormalised; the inverse is 1/N scaled,
+//! reference's rules and messages and passes each `Vec<Complex<f32>>` of the
+//! batch (already interleaved `re, im` f32) where it lies to the C-ABI of
+//! `include/wgfft.h` (`wgfft_exec_host_rows`: the library's staging copies
+//! gather the rows and scatter the results into new `Vec`s).  Forward is unnormalised; the inverse is 1/N scaled,
 //! as in the reference.  There is no batch cap (the reference panics past
 //! `min(128 MiB / 8N, 1024)` transforms, `src/lib.rs:736-739`).
 //!

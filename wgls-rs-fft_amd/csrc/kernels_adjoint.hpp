@@ -245,11 +245,25 @@ k_filter(FilterArgs a) {
 // Thread t holds the single-layout slots of both halves: slot i RF + m is
 // index t + T i + m H/RF (RF the first radix of the H-point plan).
 // WAVES: the minimum waves per SIMD the register allocation must allow.
-template <int N, int E, bool FWD, int WAVES, int... Rs>
+// CH: elements per step of the closing gw / gx loop.  The loop re-reads x
+// and w for CH elements at a time, one step ahead of the arithmetic (CH = E:
+// every load up front), so fewer values are in flight next to the live
+// transform and the gw partials.
+// STOREY: Y is stored (the fused kernel with a Y output; compiled apart, as
+// holding Y for the store next to gY costs registers).
+// FL (kS2* bits): kS2Prefetch loads the next transform's input (x, or gY
+// for the VJP) into registers at the top of each transform, so its HBM
+// latency overlaps the two transforms instead of stalling the workgroup
+// between them; kS2CachedX reads x with ordinary loads rather than
+// non-temporal ones, so the closing re-read of x finds it in the L2.
+constexpr int kS2Prefetch = 1, kS2CachedX = 2;
+template <int N, int E, bool FWD, int WAVES, int CH, bool STOREY, int FL, int... Rs>
 __global__ void __launch_bounds__((N / 2) / E) __attribute__((amdgpu_waves_per_eu(WAVES)))
 k_adjoint_split2(AdjointArgs a) {
     constexpr int H = N / 2;
     constexpr int T = H / E;
+    constexpr bool PF = (FL & kS2Prefetch) != 0;
+    constexpr bool XNT = (FL & kS2CachedX) == 0;
     using MP = GdMap<H, E, 1, Rs...>;
     __shared__ __align__(16) float2 lds[LdsLayoutSingle<H, E, Rs...>::len];
     __shared__ __align__(16) float2 lds_tw[twlds_len(ilog2(N))];
@@ -262,21 +276,54 @@ k_adjoint_split2(AdjointArgs a) {
 #pragma unroll
     for (int e = 0; e < E; ++e) acc0[e] = acc1[e] = 0.f;
     float lacc = 0.f;
-    for (uint64_t tr = blockIdx.x; tr < a.batch; tr += gridDim.x) {
-        const uint32_t t = launder_lane(threadIdx.x);
-        float2 y0[E], y1[E];
+    // the input of transform tr: x at n and n + H (forward), or gY as
+    // 16-byte pairs (gY[2k], gY[2k + 1]) at k (VJP)
+    auto load_input = [&](uint64_t tr, uint32_t t, float2 (&i0)[E], float2 (&i1)[E]) {
+        __builtin_amdgcn_s_setprio(2);
         if constexpr (FWD) {
             const float2* x = a.x + tr * N;
-            __builtin_amdgcn_s_setprio(2);
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const uint32_t n = MP::row(e, t);
-                const f2v va = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(x + n));
-                const f2v vb = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(x + n + H));
-                y0[e] = make_float2(va.x, va.y);
-                y1[e] = make_float2(vb.x, vb.y);
+                if constexpr (XNT) {
+                    const f2v va = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(x + n));
+                    const f2v vb = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(x + n + H));
+                    i0[e] = make_float2(va.x, va.y);
+                    i1[e] = make_float2(vb.x, vb.y);
+                } else {
+                    i0[e] = x[n];
+                    i1[e] = x[n + H];
+                }
             }
-            __builtin_amdgcn_s_setprio(0);
+        } else {
+            const float4* g4 = reinterpret_cast<const float4*>(a.gy + tr * N);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const float4 v = ld16<true>(g4 + MP::row(e, t));
+                i0[e] = make_float2(v.x, v.y);
+                i1[e] = make_float2(v.z, v.w);
+            }
+        }
+        __builtin_amdgcn_s_setprio(0);
+    };
+    float2 p0[PF ? E : 1], p1[PF ? E : 1];  // the prefetched input (PF)
+    if constexpr (PF) {
+        if (blockIdx.x < a.batch) load_input(blockIdx.x, threadIdx.x, p0, p1);
+    }
+    for (uint64_t tr = blockIdx.x; tr < a.batch; tr += gridDim.x) {
+        const uint32_t t = launder_lane(threadIdx.x);
+        float2 y0[E], y1[E];
+        if constexpr (PF) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                y0[e] = p0[e];
+                y1[e] = p1[e];
+            }
+            if (tr + gridDim.x < a.batch) load_input(tr + gridDim.x, t, p0, p1);
+        } else {
+            load_input(tr, t, y0, y1);
+        }
+        if constexpr (FWD) {
             // W_N^n with n = t + offset(e): W_N^t times a compile-time root
             const float2 wt = cmul(twn.hi[t >> twn.h], twn.lo[t & ((1u << twn.h) - 1)]);
             static_for<E>([&](auto ec) {
@@ -297,7 +344,7 @@ k_adjoint_split2(AdjointArgs a) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const uint32_t k = MP::row(e, t1);
-                if (a.y)
+                if constexpr (STOREY)
                     st16<false>(reinterpret_cast<float4*>(a.y + tr * N + 2 * k),
                                 make_float4(y0[e].x, y0[e].y, y1[e].x, y1[e].y));
                 const float2 mv = m2[k];
@@ -307,16 +354,6 @@ k_adjoint_split2(AdjointArgs a) {
             }
             WGFFT_LDS_SYNC();  // the second half's last exchange read before the adjoint writes
             __builtin_amdgcn_sched_barrier(0);
-        } else {
-            const float4* g4 = reinterpret_cast<const float4*>(a.gy + tr * N);
-            __builtin_amdgcn_s_setprio(2);
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const float4 v = ld16<true>(g4 + MP::row(e, t));
-                y0[e] = make_float2(v.x, v.y);
-                y1[e] = make_float2(v.z, v.w);
-            }
-            __builtin_amdgcn_s_setprio(0);
         }
         const uint32_t ta = launder_lane(t);
         Engine<H, E, T, true>::run_single(y0, lds, twh, ta, Radices<Rs...>{});
@@ -330,27 +367,74 @@ k_adjoint_split2(AdjointArgs a) {
         const float* wv = launder(a.w);
         float2* gx = a.gx + tr * N;
         const float2 wtb = cmul(twn.hi[tb >> twn.h], twn.lo[tb & ((1u << twn.h) - 1)]);
-        static_for<E>([&](auto ec) {
-            constexpr int e = decltype(ec)::value;
-            constexpr CplxF c = cx_root(N, MP::offset(e));
-            const uint32_t n = MP::row(e, tb);
-            const float2 g1 = twmul<true>(y1[e], cmul(wtb, make_float2(c.x, c.y)));
-            const float2 ga = cadd(y0[e], g1), gb = csub(y0[e], g1);
-            const float2 xa = x[n], xb = x[n + H];
-            acc0[e] += ga.x * xa.x + ga.y * xa.y;
-            acc1[e] += gb.x * xb.x + gb.y * xb.y;
-            y0[e] = cscale(ga, wv[n]);
-            y1[e] = cscale(gb, wv[n + H]);
-        });
-        __builtin_amdgcn_s_setprio(2);
+        static_assert(E % CH == 0, "the closing loop steps through whole chunks");
+        if constexpr (CH == E) {
+            static_for<E>([&](auto ec) {
+                constexpr int e = decltype(ec)::value;
+                constexpr CplxF c = cx_root(N, MP::offset(e));
+                const uint32_t n = MP::row(e, tb);
+                const float2 g1 = twmul<true>(y1[e], cmul(wtb, make_float2(c.x, c.y)));
+                const float2 ga = cadd(y0[e], g1), gb = csub(y0[e], g1);
+                const float2 xa = x[n], xb = x[n + H];
+                acc0[e] += ga.x * xa.x + ga.y * xa.y;
+                acc1[e] += gb.x * xb.x + gb.y * xb.y;
+                y0[e] = cscale(ga, wv[n]);
+                y1[e] = cscale(gb, wv[n + H]);
+            });
+            __builtin_amdgcn_s_setprio(2);
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const uint32_t n = MP::row(e, tb);
-            const f2v va = {y0[e].x, y0[e].y}, vb = {y1[e].x, y1[e].y};
-            __builtin_nontemporal_store(va, reinterpret_cast<f2v*>(gx + n));
-            __builtin_nontemporal_store(vb, reinterpret_cast<f2v*>(gx + n + H));
+            for (int e = 0; e < E; ++e) {
+                const uint32_t n = MP::row(e, tb);
+                const f2v va = {y0[e].x, y0[e].y}, vb = {y1[e].x, y1[e].y};
+                __builtin_nontemporal_store(va, reinterpret_cast<f2v*>(gx + n));
+                __builtin_nontemporal_store(vb, reinterpret_cast<f2v*>(gx + n + H));
+            }
+            __builtin_amdgcn_s_setprio(0);
+            WGFFT_LDS_SYNC();  // the next transform's first half overwrites the exchange buffer
+            continue;
         }
-        __builtin_amdgcn_s_setprio(0);
+        constexpr int NC = E / CH;
+        struct Chunk {
+            float2 xa[CH], xb[CH];
+            float wa[CH], wb[CH];
+        };
+        auto fetch = [&](Chunk& k, auto cc) {
+            constexpr int c0 = decltype(cc)::value * CH;
+            static_for<CH>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const uint32_t n = MP::row(c0 + i, tb);
+                k.xa[i] = x[n];
+                k.xb[i] = x[n + H];
+                k.wa[i] = wv[n];
+                k.wb[i] = wv[n + H];
+            });
+        };
+        Chunk cur;
+        fetch(cur, std::integral_constant<int, 0>{});
+        static_for<NC>([&](auto cc) {
+            constexpr int c = decltype(cc)::value;
+            Chunk nxt;
+            if constexpr (c + 1 < NC) fetch(nxt, std::integral_constant<int, c + 1>{});
+            if constexpr (CH < E) __builtin_amdgcn_sched_barrier(0);
+            static_for<CH>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                constexpr int e = c * CH + i;
+                constexpr CplxF r = cx_root(N, MP::offset(e));
+                const uint32_t n = MP::row(e, tb);
+                const float2 g1 = twmul<true>(y1[e], cmul(wtb, make_float2(r.x, r.y)));
+                const float2 ga = cadd(y0[e], g1), gb = csub(y0[e], g1);
+                acc0[e] += ga.x * cur.xa[i].x + ga.y * cur.xa[i].y;
+                acc1[e] += gb.x * cur.xb[i].x + gb.y * cur.xb[i].y;
+                const float2 oa = cscale(ga, cur.wa[i]), ob = cscale(gb, cur.wb[i]);
+                const f2v va = {oa.x, oa.y}, vb = {ob.x, ob.y};
+                __builtin_nontemporal_store(va, reinterpret_cast<f2v*>(gx + n));
+                __builtin_nontemporal_store(vb, reinterpret_cast<f2v*>(gx + n + H));
+            });
+            if constexpr (c + 1 < NC) {
+                cur = nxt;
+                if constexpr (CH < E) __builtin_amdgcn_sched_barrier(0);
+            }
+        });
         WGFFT_LDS_SYNC();  // the next transform's first half overwrites the exchange buffer
     }
     const uint32_t t = threadIdx.x;
@@ -455,25 +539,30 @@ k_filter_split2(FilterArgs a) {
     }
 }
 
-template <int N, int E, int WAVES, int... Rs>
+template <int N, int E, int WAVES, int CH, int FL, int... Rs>
 struct AdjointSplit2Kernel {
     static constexpr int THREADS = (N / 2) / E;
-    template <bool FWD>
+    template <bool FWD, bool STOREY>
     static hipError_t go(const AdjointArgs& a) {
-        hipLaunchKernelGGL((k_adjoint_split2<N, E, FWD, WAVES, Rs...>), dim3((unsigned)a.grid), dim3(THREADS), 0,
-                           a.stream, a);
+        hipLaunchKernelGGL((k_adjoint_split2<N, E, FWD, WAVES, CH, STOREY, FL, Rs...>), dim3((unsigned)a.grid),
+                           dim3(THREADS), 0, a.stream, a);
         return hipGetLastError();
     }
-    static hipError_t launch(bool fwd, const AdjointArgs& a) { return fwd ? go<true>(a) : go<false>(a); }
-    template <bool FWD>
+    static hipError_t launch(bool fwd, const AdjointArgs& a) {
+        return fwd ? (a.y ? go<true, true>(a) : go<true, false>(a)) : go<false, false>(a);
+    }
+    template <bool FWD, bool STOREY>
     static int occ() {
         int blocks = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &blocks, (const void*)k_adjoint_split2<N, E, FWD, WAVES, Rs...>, THREADS, 0) != hipSuccess)
+                &blocks, (const void*)k_adjoint_split2<N, E, FWD, WAVES, CH, STOREY, FL, Rs...>, THREADS, 0) !=
+            hipSuccess)
             return 1;
         return blocks > 0 ? blocks : 1;
     }
-    static int blocks_per_cu(bool fwd) { return fwd ? occ<true>() : occ<false>(); }
+    static int blocks_per_cu(bool fwd, bool y) {
+        return fwd ? (y ? occ<true, true>() : occ<true, false>()) : occ<false, false>();
+    }
     static hipError_t filter(const FilterArgs& a) {
         if (a.sigma != 0.f)
             hipLaunchKernelGGL((k_filter_split2<N, E, true, Rs...>), dim3((unsigned)a.grid), dim3(THREADS), 0,
@@ -509,7 +598,7 @@ struct AdjointKernel {
             return 1;
         return blocks > 0 ? blocks : 1;
     }
-    static int blocks_per_cu(bool fwd) { return fwd ? occ<true>() : occ<false>(); }
+    static int blocks_per_cu(bool fwd, bool) { return fwd ? occ<true>() : occ<false>(); }
     static hipError_t filter(const FilterArgs& a) {
         if (a.sigma != 0.f)
             hipLaunchKernelGGL((k_filter<N, E, OPT, true, Rs...>), dim3((unsigned)a.grid), dim3(THREADS), 0, a.stream,
@@ -533,7 +622,7 @@ struct AdjointVariant {
     int opt;  // kAdj* bits
     const char* plan;
     hipError_t (*launch)(bool, const AdjointArgs&);
-    int (*blocks_per_cu)(bool);
+    int (*blocks_per_cu)(bool fwd, bool y);  // y: the fused kernel stores Y
     hipError_t (*filter)(const FilterArgs&);  // the spectral filter with the same plan
     int (*filter_blocks_per_cu)(bool noise);
     int roles;  // kRole* bits: untuned default for these calls
@@ -553,13 +642,16 @@ struct AdjointRegistrar {
         &::wgfft::AdjointKernel<N, E, OPT, __VA_ARGS__>::filter_blocks_per_cu, 0})
 
 // Split-form variant of length N; PLAN is the radix split of N/2 (first
-// radix = last radix, the single layout).
-#define WGFFT_REGISTER_ADJOINT_SPLIT2(N, E, WAVES, ROLES, PLAN, ...)                                        \
-    static ::wgfft::AdjointRegistrar _wgfft_adjs2_##N##_##E##_##WAVES##_##PLAN(::wgfft::AdjointVariant{       \
-        ::wgfft::ilog2(N), E, ::wgfft::kAdjSplit2 | ((WAVES) << 8), "2x" #PLAN,                               \
-        &::wgfft::AdjointSplit2Kernel<N, E, WAVES, __VA_ARGS__>::launch,                                     \
-        &::wgfft::AdjointSplit2Kernel<N, E, WAVES, __VA_ARGS__>::blocks_per_cu,                              \
-        &::wgfft::AdjointSplit2Kernel<N, E, WAVES, __VA_ARGS__>::filter,                                     \
-        &::wgfft::AdjointSplit2Kernel<N, E, WAVES, __VA_ARGS__>::filter_blocks_per_cu, ROLES})
+// radix = last radix, the single layout).  opt carries WAVES in bits 8-11,
+// CH in bits 12-17 and the kS2* flags FL in bits 18-19.
+#define WGFFT_ADJ_SPLIT2_T(N, E, WAVES, CH, FL, ...) ::wgfft::AdjointSplit2Kernel<N, E, WAVES, CH, FL, __VA_ARGS__>
+#define WGFFT_REGISTER_ADJOINT_SPLIT2(N, E, WAVES, CH, FL, ROLES, PLAN, ...)                                  \
+    static ::wgfft::AdjointRegistrar _wgfft_adjs2_##N##_##E##_##WAVES##_##CH##_##FL##_##PLAN(                 \
+        ::wgfft::AdjointVariant{                                                                              \
+            ::wgfft::ilog2(N), E, ::wgfft::kAdjSplit2 | ((WAVES) << 8) | ((CH) << 12) | ((FL) << 18),         \
+            "2x" #PLAN, &WGFFT_ADJ_SPLIT2_T(N, E, WAVES, CH, FL, __VA_ARGS__)::launch,                        \
+            &WGFFT_ADJ_SPLIT2_T(N, E, WAVES, CH, FL, __VA_ARGS__)::blocks_per_cu,                             \
+            &WGFFT_ADJ_SPLIT2_T(N, E, WAVES, CH, FL, __VA_ARGS__)::filter,                                    \
+            &WGFFT_ADJ_SPLIT2_T(N, E, WAVES, CH, FL, __VA_ARGS__)::filter_blocks_per_cu, ROLES})
 
 }  // namespace wgfft

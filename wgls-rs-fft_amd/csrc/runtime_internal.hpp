@@ -138,7 +138,8 @@ struct Handle {
     int blocks_per_cu(const BatchedVariant* v, Mode m, bool cache = true);
     // workgroups per CU of a fused kernel, kept in adj_occ_cache (the
     // occupancy query costs microseconds per call): kind 0 the VJP, 1 the
-    // fused forward + adjoint, 2 the filter, 3 the filter with noise
+    // fused forward + adjoint, 2 the filter, 3 the filter with noise, 4 the
+    // fused forward + adjoint that also stores Y
     int adjoint_blocks_per_cu(const AdjointVariant* v, int kind);
     int run(Plan* p, Mode mode, const float2* in, float2* out, const float* win, uint64_t batch, float scale,
             hipStream_t s, const BatchedVariant* force = nullptr, float2* fixed = nullptr, uint64_t fixed_bytes = 0,

@@ -1,7 +1,9 @@
 // variants_adjoint.hip -- fused forward+adjoint and VJP kernels per length.
-// For each length the first registration is the default (at 4096 and 8192
-// the fastest measured on MI355X); the third argument holds the kAdj*
-// options (1 keep x in registers, 2 LDS twiddles).
+// For each length and role (fused, VJP, filter) the untuned default is the
+// first registration claiming the role, else the first registration (at
+// 4096 and 8192 the fastest measured on MI355X); the third argument of
+// WGFFT_REGISTER_ADJOINT holds the kAdj* options (1 keep x in registers,
+// 2 LDS twiddles).
 #include "kernels_adjoint.hpp"
 
 WGFFT_REGISTER_ADJOINT(512, 32, 0, 16x16x2, 16, 16, 2);
@@ -28,12 +30,22 @@ WGFFT_REGISTER_ADJOINT(8192, 16, 3, 8x16x8x8, 8, 16, 8, 8);
 WGFFT_REGISTER_ADJOINT(16384, 32, 0, 16x16x16x4, 16, 16, 16, 4);
 
 // Split form (a radix-2 stage and two half-length transforms through a
-// half-length LDS footprint), the untuned default of the spectral filter at
-// both lengths (8192 x 1024: 31.9 against 47.5 us; 16384 x 512: 39.6
-// against 48.7 us) and of the VJP at 8192 (kernel 35.6 us).  As the fused
-// forward + adjoint it holds 188 VGPRs for the gw partials (two workgroups
-// per CU) and measured no faster than the first registration; three or four
-// workgroups per CU (capped registers, or per-transform partial rows
-// instead of the gw registers) measured slower still (77-125 us).
-WGFFT_REGISTER_ADJOINT_SPLIT2(8192, 16, 2, ::wgfft::kRoleVjp | ::wgfft::kRoleFilter, 16x16x16, 16, 16, 16);
-WGFFT_REGISTER_ADJOINT_SPLIT2(16384, 16, 2, ::wgfft::kRoleFilter, 16x8x4x16, 16, 8, 4, 16);
+// half-length LDS footprint).  Arguments after E: minimum waves per SIMD,
+// elements per step of the closing gw / gx loop, kS2* flags (1 prefetch the
+// next transform's input, 2 read x through the caches for its re-read),
+// default roles.  Measured at 8192 x 1024 on MI355X (gw reduction
+// included): the spectral filter 28-31 us against 40-47 us for the
+// one-footprint kernels; the fused forward + adjoint 52.5-52.9 us with x
+// read through the caches against 61-63 us with non-temporal x reads (the
+// closing re-read of x then misses the L2); the VJP 37.7-38.9 us with the
+// input prefetch against 39.9-41 us.  Measured and not kept for the fused
+// kernel: capping registers for three or four workgroups per CU (spills,
+// 57-65 us), the input prefetch (58-63 us) and the gw partials in an LDS
+// array instead of registers (53.0 us, no gain).
+WGFFT_REGISTER_ADJOINT_SPLIT2(8192, 16, 2, 16, 0, ::wgfft::kRoleFilter, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_ADJOINT_SPLIT2(8192, 16, 2, 16, 2, ::wgfft::kRoleFused, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_ADJOINT_SPLIT2(8192, 16, 2, 16, 1, ::wgfft::kRoleVjp, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_ADJOINT_SPLIT2(8192, 16, 2, 4, 2, 0, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_ADJOINT_SPLIT2(16384, 16, 2, 16, 0, ::wgfft::kRoleFilter, 16x8x4x16, 16, 8, 4, 16);
+WGFFT_REGISTER_ADJOINT_SPLIT2(16384, 16, 2, 16, 2, 0, 16x8x4x16, 16, 8, 4, 16);
+WGFFT_REGISTER_ADJOINT_SPLIT2(16384, 16, 2, 16, 1, 0, 16x8x4x16, 16, 8, 4, 16);

@@ -44,7 +44,7 @@ int Handle::adjoint_blocks_per_cu(const AdjointVariant* v, int kind) {
     auto key = std::make_pair(v, kind);
     auto it = adj_occ_cache.find(key);
     if (it != adj_occ_cache.end()) return it->second;
-    const int b = kind < 2 ? v->blocks_per_cu(kind == 1) : v->filter_blocks_per_cu(kind == 3);
+    const int b = (kind < 2 || kind == 4) ? v->blocks_per_cu(kind != 0, kind == 4) : v->filter_blocks_per_cu(kind == 3);
     adj_occ_cache[key] = b;
     return b;
 }
@@ -58,10 +58,12 @@ const std::vector<const AdjointVariant*> adjoint_variants(int log2n) {
 
 const char* adjoint_describe(const AdjointVariant* v, char* buf, uint64_t len) {
     if (buf && len)
-        std::snprintf(buf, len, "fused N=%d plan=%s E=%d keep_x=%d%s%s%s", 1 << v->log2n, v->plan, v->E,
+        std::snprintf(buf, len, "fused N=%d plan=%s E=%d keep_x=%d%s%s%s%s%s%s", 1 << v->log2n, v->plan, v->E,
                       (v->opt & kAdjKeepX) ? 1 : 0, (v->opt & kAdjTwLds) ? " tw-lds" : "",
                       (v->opt & kAdjSplit2) ? " split2" : "",
-                      (v->opt >> 8) ? (" waves>=" + std::to_string(v->opt >> 8)).c_str() : "");
+                      ((v->opt >> 8) & 15) ? (" waves>=" + std::to_string((v->opt >> 8) & 15)).c_str() : "",
+                      ((v->opt >> 12) & 63) ? (" chunk=" + std::to_string((v->opt >> 12) & 63)).c_str() : "",
+                      ((v->opt >> 18) & 1) ? " prefetch" : "", ((v->opt >> 18) & 2) ? " cached-x" : "");
     return buf;
 }
 
@@ -229,7 +231,7 @@ int fused_adjoint(Handle* h, const AdjointVariant* v, bool fwd, const float* d_x
     // partial matrix: one row of n + 4 floats per workgroup (gw partials,
     // then the loss partial in column n; rows kept 16-byte aligned)
     const uint64_t ld = n + 4;
-    int grid = h->adjoint_blocks_per_cu(v, fwd ? 1 : 0) * h->num_cus;
+    int grid = h->adjoint_blocks_per_cu(v, fwd ? (d_y ? 4 : 1) : 0) * h->num_cus;
     if ((uint64_t)grid > batch) grid = (int)batch;
     const uint64_t rows = (uint64_t)grid;
     int rc;

@@ -255,8 +255,12 @@ k_filter(FilterArgs a) {
 // for the VJP) into registers at the top of each transform, so its HBM
 // latency overlaps the two transforms instead of stalling the workgroup
 // between them; kS2CachedX reads x with ordinary loads rather than
-// non-temporal ones, so the closing re-read of x finds it in the L2.
-constexpr int kS2Prefetch = 1, kS2CachedX = 2;
+// non-temporal ones, so the closing re-read of x finds it in the L2;
+// kS2KeepX0 / kS2KeepX1 hold the thread's x at n / at n + H in registers
+// from the load to the gw accumulation instead of re-reading them (the
+// re-read misses the L2 about half the time: 64 workgroups of an XCD keep
+// 64 KB of x each in flight, the size of that L2).
+constexpr int kS2Prefetch = 1, kS2CachedX = 2, kS2KeepX0 = 4, kS2KeepX1 = 8;
 template <int N, int E, bool FWD, int WAVES, int CH, bool STOREY, int FL, int... Rs>
 __global__ void __launch_bounds__((N / 2) / E) __attribute__((amdgpu_waves_per_eu(WAVES)))
 k_adjoint_split2(AdjointArgs a) {
@@ -264,6 +268,8 @@ k_adjoint_split2(AdjointArgs a) {
     constexpr int T = H / E;
     constexpr bool PF = (FL & kS2Prefetch) != 0;
     constexpr bool XNT = (FL & kS2CachedX) == 0;
+    constexpr bool KX0 = FWD && (FL & kS2KeepX0) != 0, KX1 = FWD && (FL & kS2KeepX1) != 0;
+    static_assert(!(PF && (KX0 || KX1)), "the prefetch and the kept x share the registers");
     using MP = GdMap<H, E, 1, Rs...>;
     __shared__ __align__(16) float2 lds[LdsLayoutSingle<H, E, Rs...>::len];
     __shared__ __align__(16) float2 lds_tw[twlds_len(ilog2(N))];
@@ -323,6 +329,12 @@ k_adjoint_split2(AdjointArgs a) {
         } else {
             load_input(tr, t, y0, y1);
         }
+        float2 xk0[KX0 ? E : 1], xk1[KX1 ? E : 1];  // x kept for the gw accumulation
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if constexpr (KX0) xk0[e] = y0[e];
+            if constexpr (KX1) xk1[e] = y1[e];
+        }
         if constexpr (FWD) {
             // W_N^n with n = t + offset(e): W_N^t times a compile-time root
             const float2 wt = cmul(twn.hi[t >> twn.h], twn.lo[t & ((1u << twn.h) - 1)]);
@@ -375,7 +387,7 @@ k_adjoint_split2(AdjointArgs a) {
                 const uint32_t n = MP::row(e, tb);
                 const float2 g1 = twmul<true>(y1[e], cmul(wtb, make_float2(c.x, c.y)));
                 const float2 ga = cadd(y0[e], g1), gb = csub(y0[e], g1);
-                const float2 xa = x[n], xb = x[n + H];
+                const float2 xa = KX0 ? xk0[KX0 ? e : 0] : x[n], xb = KX1 ? xk1[KX1 ? e : 0] : x[n + H];
                 acc0[e] += ga.x * xa.x + ga.y * xa.y;
                 acc1[e] += gb.x * xb.x + gb.y * xb.y;
                 y0[e] = cscale(ga, wv[n]);
@@ -403,8 +415,8 @@ k_adjoint_split2(AdjointArgs a) {
             static_for<CH>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
                 const uint32_t n = MP::row(c0 + i, tb);
-                k.xa[i] = x[n];
-                k.xb[i] = x[n + H];
+                k.xa[i] = KX0 ? xk0[KX0 ? c0 + i : 0] : x[n];
+                k.xb[i] = KX1 ? xk1[KX1 ? c0 + i : 0] : x[n + H];
                 k.wa[i] = wv[n];
                 k.wb[i] = wv[n + H];
             });

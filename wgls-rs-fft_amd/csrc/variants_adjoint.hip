@@ -32,20 +32,26 @@ WGFFT_REGISTER_ADJOINT(16384, 32, 0, 16x16x16x4, 16, 16, 16, 4);
 // Split form (a radix-2 stage and two half-length transforms through a
 // half-length LDS footprint).  Arguments after E: minimum waves per SIMD,
 // elements per step of the closing gw / gx loop, kS2* flags (1 prefetch the
-// next transform's input, 2 read x through the caches for its re-read),
+// next transform's input, 2 read x through the caches, 4 and 8 keep the
+// thread's x at n and at n + H in registers for the gw accumulation),
 // default roles.  Measured at 8192 x 1024 on MI355X (gw reduction
-// included): the spectral filter 28-31 us against 40-47 us for the
-// one-footprint kernels; the fused forward + adjoint 52.5-52.9 us with x
-// read through the caches against 61-63 us with non-temporal x reads (the
-// closing re-read of x then misses the L2); the VJP 37.7-38.9 us with the
-// input prefetch against 39.9-41 us.  Measured and not kept for the fused
-// kernel: capping registers for three or four workgroups per CU (spills,
-// 57-65 us), the input prefetch (58-63 us) and the gw partials in an LDS
-// array instead of registers (53.0 us, no gain).
+// included): the spectral filter 28-33 us against 40-47 us for the
+// one-footprint kernels; the fused forward + adjoint 47.2-47.4 us with x
+// kept in registers and read through the caches, 50.0-52.0 us with x kept
+// and read non-temporally, 52.7-52.9 us re-reading x through the caches
+// and 61-63 us re-reading it after non-temporal reads (the re-read then
+// misses the L2); the VJP 37.8-38.3 us with the input prefetch against
+// 39.9-41 us.  Measured and not kept for the fused kernel: capping
+// registers for three or four workgroups per CU (spills, 57-65 us), the
+// input prefetch (58-63 us) and the gw partials in an LDS array instead of
+// registers (53.0 us).  At 16384 x 512: fused 66.1 us (x kept) against
+// 67.5 us for the one-footprint kernel, VJP 50.0 (prefetch) against 51.0.
 WGFFT_REGISTER_ADJOINT_SPLIT2(8192, 16, 2, 16, 0, ::wgfft::kRoleFilter, 16x16x16, 16, 16, 16);
-WGFFT_REGISTER_ADJOINT_SPLIT2(8192, 16, 2, 16, 2, ::wgfft::kRoleFused, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_ADJOINT_SPLIT2(8192, 16, 2, 4, 14, ::wgfft::kRoleFused, 16x16x16, 16, 16, 16);
 WGFFT_REGISTER_ADJOINT_SPLIT2(8192, 16, 2, 16, 1, ::wgfft::kRoleVjp, 16x16x16, 16, 16, 16);
-WGFFT_REGISTER_ADJOINT_SPLIT2(8192, 16, 2, 4, 2, 0, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_ADJOINT_SPLIT2(8192, 16, 2, 2, 14, 0, 16x16x16, 16, 16, 16);
+WGFFT_REGISTER_ADJOINT_SPLIT2(8192, 16, 2, 16, 2, 0, 16x16x16, 16, 16, 16);
 WGFFT_REGISTER_ADJOINT_SPLIT2(16384, 16, 2, 16, 0, ::wgfft::kRoleFilter, 16x8x4x16, 16, 8, 4, 16);
 WGFFT_REGISTER_ADJOINT_SPLIT2(16384, 16, 2, 16, 2, 0, 16x8x4x16, 16, 8, 4, 16);
-WGFFT_REGISTER_ADJOINT_SPLIT2(16384, 16, 2, 16, 1, 0, 16x8x4x16, 16, 8, 4, 16);
+WGFFT_REGISTER_ADJOINT_SPLIT2(16384, 16, 2, 16, 1, ::wgfft::kRoleVjp, 16x8x4x16, 16, 8, 4, 16);
+WGFFT_REGISTER_ADJOINT_SPLIT2(16384, 16, 2, 4, 14, ::wgfft::kRoleFused, 16x8x4x16, 16, 8, 4, 16);

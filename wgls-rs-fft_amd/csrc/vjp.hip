@@ -58,12 +58,14 @@ const std::vector<const AdjointVariant*> adjoint_variants(int log2n) {
 
 const char* adjoint_describe(const AdjointVariant* v, char* buf, uint64_t len) {
     if (buf && len)
-        std::snprintf(buf, len, "fused N=%d plan=%s E=%d keep_x=%d%s%s%s%s%s%s", 1 << v->log2n, v->plan, v->E,
+        std::snprintf(buf, len, "fused N=%d plan=%s E=%d keep_x=%d%s%s%s%s%s%s%s", 1 << v->log2n, v->plan, v->E,
                       (v->opt & kAdjKeepX) ? 1 : 0, (v->opt & kAdjTwLds) ? " tw-lds" : "",
                       (v->opt & kAdjSplit2) ? " split2" : "",
                       ((v->opt >> 8) & 15) ? (" waves>=" + std::to_string((v->opt >> 8) & 15)).c_str() : "",
                       ((v->opt >> 12) & 63) ? (" chunk=" + std::to_string((v->opt >> 12) & 63)).c_str() : "",
-                      ((v->opt >> 18) & 1) ? " prefetch" : "", ((v->opt >> 18) & 2) ? " cached-x" : "");
+                      ((v->opt >> 18) & 1) ? " prefetch" : "", ((v->opt >> 18) & 2) ? " cached-x" : "", ((v->opt >> 18) & 12) == 12 ? " keep-x"
+                                                    : ((v->opt >> 18) & 4) ? " keep-x0"
+                                                    : ((v->opt >> 18) & 8) ? " keep-x1" : "");
     return buf;
 }
 

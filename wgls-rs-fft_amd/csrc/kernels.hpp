@@ -45,6 +45,7 @@ constexpr int kFlagTwLds = 4;    // twiddle base powers from two short LDS table
 constexpr int kFlagNoPrefetch = 8;  // no register prefetch of the next transform (fewer registers)
 constexpr int kFlagSplit2 = 256;    // k_split2: a radix-2 stage, then the two halves in turn through LDS
 constexpr int kFlagPrio = 512;      // raised wave priority while issuing the prefetch loads and the stores
+constexpr int kFlagHalfPrefetch = 2048;  // prefetch only the first E/2 slots of the next transform
 // bits 4..7: minimum waves per SIMD the register allocation must allow (0: compiler's choice)
 constexpr int flag_min_waves(int fl) { return (fl >> 4) & 15; }
 
@@ -181,6 +182,23 @@ struct Io {
             for (int s = 0; s < E; ++s) d[s] = src[elem(s, t)];
         }
     }
+    // slots S0 <= s < S1 (S0, S1 even) into d[s - OFF]
+    template <int S0, int S1, int OFF, int M>
+    __device__ __forceinline__ static void load_slots(float2 (&d)[M], const float2* __restrict__ src, uint32_t t) {
+        static_assert(S0 % 2 == 0 && S1 % 2 == 0 && S1 - OFF <= M && S0 >= OFF, "slot range");
+        if constexpr (PAIR) {
+            const float4* s4 = reinterpret_cast<const float4*>(src);
+#pragma unroll
+            for (int q = S0 / 2; q < S1 / 2; ++q) {
+                const float4 v = s4[t + T * q];
+                d[2 * q - OFF] = make_float2(v.x, v.y);
+                d[2 * q + 1 - OFF] = make_float2(v.z, v.w);
+            }
+        } else {
+#pragma unroll
+            for (int s = S0; s < S1; ++s) d[s - OFF] = src[elem(s, t)];
+        }
+    }
     template <bool SCALE, bool NT = false>
     __device__ __forceinline__ static void store(float2* __restrict__ dst, const float2 (&d)[E], uint32_t t,
                                                  float scale) {
@@ -257,10 +275,17 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
         __syncthreads();
     }
     constexpr bool PREFETCH = (FL & kFlagNoPrefetch) == 0;
+    // half prefetch: slots [0, E/2) of the next transform are held in
+    // registers, slots [E/2, E) are loaded when its turn comes
+    constexpr bool HALF = PREFETCH && (FL & kFlagHalfPrefetch) != 0;
+    constexpr int NPF = PREFETCH ? (HALF ? E / 2 : E) : 1;
     uint64_t grp = blockIdx.x;
-    float2 nxt[PREFETCH ? E : 1];
+    float2 nxt[NPF];
     if constexpr (PREFETCH) {
-        if (grp < ngroups && grp * G + g < batch) IO::load(nxt, in + (grp * G + g) * idist, t);
+        if (grp < ngroups && grp * G + g < batch) {
+            if constexpr (HALF) IO::template load_slots<0, E / 2, 0>(nxt, in + (grp * G + g) * idist, t);
+            else IO::load(nxt, in + (grp * G + g) * idist, t);
+        }
     }
 
     for (; grp < ngroups; grp += gridDim.x) {
@@ -271,7 +296,16 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
         // (vmcnt retires in order).
         float w[E];
         if constexpr (MODE == kWindowed) IO::load_real(w, win, t);
-        if constexpr (PREFETCH) {
+        if constexpr (HALF) {
+#pragma unroll
+            for (int i = 0; i < E / 2; ++i) d[i] = nxt[i];
+            const uint64_t ngrp = grp + gridDim.x;
+            if constexpr ((FL & kFlagPrio) != 0) __builtin_amdgcn_s_setprio(2);
+            if (active) IO::template load_slots<E / 2, E, 0>(d, in + tr * idist, t);
+            if (ngrp < ngroups && ngrp * G + g < batch)
+                IO::template load_slots<0, E / 2, 0>(nxt, in + (ngrp * G + g) * idist, t);
+            if constexpr ((FL & kFlagPrio) != 0) __builtin_amdgcn_s_setprio(0);
+        } else if constexpr (PREFETCH) {
 #pragma unroll
             for (int i = 0; i < E; ++i) d[i] = nxt[i];
             const uint64_t ngrp = grp + gridDim.x;

@@ -154,7 +154,11 @@ static void describe(const BatchedVariant* v, char* desc, uint64_t len) {
     std::snprintf(desc, len, "lds N=%d plan=%s E=%d G=%d%s%s%s%s%s threads=%d lds=%dB", 1 << v->log2n, v->plan, v->E,
                   v->G, (v->flags & kFlagNtStore) ? " nt-store" : "", (v->flags & kFlagPrio) ? " prio" : "",
                   (v->flags & kFlagTwRegs) ? " tw-regs" : (v->flags & kFlagTwLds) ? " tw-lds" : "",
-                  (v->flags & kFlagNoPrefetch) ? " no-prefetch" : (v->flags & kFlagSplit2) ? " split2" : "", waves,
+                  (v->flags & kFlagNoPrefetch)     ? " no-prefetch"
+                  : (v->flags & kFlagHalfPrefetch) ? " half-prefetch"
+                  : (v->flags & kFlagSplit2)       ? " split2"
+                                                   : "",
+                  waves,
                   v->threads, v->lds_bytes);
 }
 

@@ -2,8 +2,9 @@
 // 16x16x16 with E = 16 uses the single I/O layout (8-byte lanes, two LDS
 // exchanges, ~half the registers of E = 32).  Flags (kFlag*): 1 stores
 // non-temporally; 2 holds the twiddle base powers in registers across
-// transforms; 4 reads them from two short LDS tables; 512 raises the wave
-// priority around the global loads and stores.
+// transforms; 4 reads them from two short LDS tables; 8 drops the register
+// prefetch of the next transform; 512 raises the wave priority around the
+// global loads and stores; 2048 prefetches half of the next transform.
 #include "kernels.hpp"
 
 WGFFT_REGISTER_BATCHED(4096, 32, 1, 16x16x16, 16, 16, 16);
@@ -26,3 +27,12 @@ WGFFT_REGISTER_BATCHED_F(4096, 32, 2, 9, 16x16x16, 16, 16, 16);
 // 1 + 512: non-temporal stores, and the wave's priority raised while it issues
 // the next transform's loads and this one's stores (41.1 against 41.5 us).
 WGFFT_REGISTER_BATCHED_F(4096, 16, 1, 513, 16x16x16, 16, 16, 16);
+
+// 1 + 512 + 2048: as above with only the first half of the next transform
+// prefetched into registers (113 against 151 VGPRs), so four workgroups
+// share a CU and 4096 transforms are exactly four rounds of 1024 workgroups
+// (three per CU left a last round of 256 transforms at one workgroup per
+// CU): 40.77 against 41.22 us, median of six alternating runs.  The same
+// flag measured 1-3% slower at 256 ... 2048 (G = 2 ... 16 transforms per
+// workgroup), and with LDS twiddles or without the priority at 4096.
+WGFFT_REGISTER_BATCHED_F(4096, 16, 1, 2561, 16x16x16, 16, 16, 16);

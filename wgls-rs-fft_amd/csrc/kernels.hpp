@@ -46,6 +46,7 @@ constexpr int kFlagNoPrefetch = 8;  // no register prefetch of the next transfor
 constexpr int kFlagSplit2 = 256;    // k_split2: a radix-2 stage, then the two halves in turn through LDS
 constexpr int kFlagPrio = 512;      // raised wave priority while issuing the prefetch loads and the stores
 constexpr int kFlagHalfPrefetch = 2048;  // prefetch only the first E/2 slots of the next transform
+constexpr int kFlagQuarterLoad = 4096;   // with kFlagHalfPrefetch: prefetch 3E/4 slots, load the last E/4 in turn
 // bits 4..7: minimum waves per SIMD the register allocation must allow (0: compiler's choice)
 constexpr int flag_min_waves(int fl) { return (fl >> 4) & 15; }
 
@@ -275,15 +276,16 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
         __syncthreads();
     }
     constexpr bool PREFETCH = (FL & kFlagNoPrefetch) == 0;
-    // half prefetch: slots [0, E/2) of the next transform are held in
-    // registers, slots [E/2, E) are loaded when its turn comes
+    // partial prefetch: slots [0, PS) of the next transform are held in
+    // registers, slots [PS, E) are loaded when its turn comes
     constexpr bool HALF = PREFETCH && (FL & kFlagHalfPrefetch) != 0;
-    constexpr int NPF = PREFETCH ? (HALF ? E / 2 : E) : 1;
+    constexpr int PS = (FL & kFlagQuarterLoad) ? 3 * E / 4 : E / 2;  // slots prefetched when HALF
+    constexpr int NPF = PREFETCH ? (HALF ? PS : E) : 1;
     uint64_t grp = blockIdx.x;
     float2 nxt[NPF];
     if constexpr (PREFETCH) {
         if (grp < ngroups && grp * G + g < batch) {
-            if constexpr (HALF) IO::template load_slots<0, E / 2, 0>(nxt, in + (grp * G + g) * idist, t);
+            if constexpr (HALF) IO::template load_slots<0, PS, 0>(nxt, in + (grp * G + g) * idist, t);
             else IO::load(nxt, in + (grp * G + g) * idist, t);
         }
     }
@@ -298,12 +300,12 @@ k_batched(const float2* __restrict__ in, float2* __restrict__ out, const float2*
         if constexpr (MODE == kWindowed) IO::load_real(w, win, t);
         if constexpr (HALF) {
 #pragma unroll
-            for (int i = 0; i < E / 2; ++i) d[i] = nxt[i];
+            for (int i = 0; i < PS; ++i) d[i] = nxt[i];
             const uint64_t ngrp = grp + gridDim.x;
             if constexpr ((FL & kFlagPrio) != 0) __builtin_amdgcn_s_setprio(2);
-            if (active) IO::template load_slots<E / 2, E, 0>(d, in + tr * idist, t);
+            if (active) IO::template load_slots<PS, E, 0>(d, in + tr * idist, t);
             if (ngrp < ngroups && ngrp * G + g < batch)
-                IO::template load_slots<0, E / 2, 0>(nxt, in + (ngrp * G + g) * idist, t);
+                IO::template load_slots<0, PS, 0>(nxt, in + (ngrp * G + g) * idist, t);
             if constexpr ((FL & kFlagPrio) != 0) __builtin_amdgcn_s_setprio(0);
         } else if constexpr (PREFETCH) {
 #pragma unroll

@@ -155,7 +155,7 @@ static void describe(const BatchedVariant* v, char* desc, uint64_t len) {
                   v->G, (v->flags & kFlagNtStore) ? " nt-store" : "", (v->flags & kFlagPrio) ? " prio" : "",
                   (v->flags & kFlagTwRegs) ? " tw-regs" : (v->flags & kFlagTwLds) ? " tw-lds" : "",
                   (v->flags & kFlagNoPrefetch)     ? " no-prefetch"
-                  : (v->flags & kFlagHalfPrefetch) ? " half-prefetch"
+                  : (v->flags & kFlagHalfPrefetch) ? ((v->flags & kFlagQuarterLoad) ? " 3/4-prefetch" : " half-prefetch")
                   : (v->flags & kFlagSplit2)       ? " split2"
                                                    : "",
                   waves,

@@ -17,8 +17,11 @@ The same JSON line also carries, measured in the same run:
     pipeline's noise stage fused at the load;
   * other_shapes: the same 16.8 M samples as 256 x 65536 and 16384 x 1024;
   * mixed_batch: 20,000 transforms of lengths 2^k, k ~ U{6..16}, as one
-    persistent launch, its hipGraph replay, and one launch per length;
-  * large_n: N = 2^22 x 16 and N = 2^26 x 1 forward transforms;
+    persistent launch, its hipGraph replay, and one launch per length; its
+    traffic floor (the scratch of the two-pass lengths included) as a rate
+    beside a device copy over the same buffers;
+  * large_n: N = 2^22 x 16 and N = 2^26 x 1 forward transforms; the passes'
+    combined traffic as a rate beside a device copy of the same buffers;
   * c5_windowed: N = 8192 x 1024 with a real window: the windowed forward,
     the VJP (gx, gw from gY) and the fused masked forward + adjoint, each
     against the plain forward of the same shape;
@@ -41,6 +44,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -229,6 +233,17 @@ def bench_mixed(torch, gpu, wgfft, steps: int) -> dict:
                      "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     out["kernel"] = plan.describe()
     plan.close()
+    # The traffic floor of this design: 16 B/sample, plus 16 B/sample of
+    # scratch written and re-read for the two-pass lengths (2^15, 2^16), set
+    # beside a device copy over the same two buffers.
+    two_pass = int(sum(1 << int(v) for v in k if int(v) > 14))
+    floor = 16 * int(total) + 16 * two_pass
+    copy_ms = timed(torch, lambda: y.copy_(x), 20, 3)
+    copy_gbs = 16.0 * total / (copy_ms * 1e-3) / 1e9
+    floor_gbs = floor / (out["persistent"]["ms"] * 1e-3) / 1e9
+    out["traffic_floor"] = {"bytes": floor, "bytes_per_sample": round(floor / total, 2),
+                            "gbs_at_persistent": round(floor_gbs, 1), "copy_gbs": round(copy_gbs, 1),
+                            "frac_of_copy": round(floor_gbs / copy_gbs, 4)}
     del x, y
     return out
 
@@ -423,10 +438,18 @@ def bench_large(torch, gpu, steps: int, tune: bool) -> dict:
         ms_inv = timed(torch, lambda: gpu.exec_device(y, x, inverse=True), max(20, steps // 20), 2)
         s = n * batch
         gbs = 16.0 * s / (ms * 1e-3) / 1e9
+        plan = gpu.describe(n)
+        # each global pass reads and writes every sample once: the passes
+        # together move 16 B/sample per pass, set beside a device copy of x
+        passes = len(re.findall(r"R\d+\(", plan)) or 1
+        copy_ms = timed(torch, lambda: y.copy_(x), 20, 2)
+        copy_gbs = 16.0 * s / (copy_ms * 1e-3) / 1e9
+        pass_gbs = passes * gbs
         out[f"2^{log2n}x{batch}"] = {"fwd_ms": round(ms, 4), "inv_ms": round(ms_inv, 4),
                                      "gsamples_s": round(s / (ms * 1e-3) / 1e9, 2),
                                      "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
-                                     "plan": gpu.describe(n)}
+                                     "plan": plan, "passes": passes, "pass_traffic_gbs": round(pass_gbs, 1),
+                                     "copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(pass_gbs / copy_gbs, 4)}
         del x, y
     return out
 

@@ -244,6 +244,9 @@ def bench_mixed(torch, gpu, wgfft, steps: int) -> dict:
     out["traffic_floor"] = {"bytes": floor, "bytes_per_sample": round(floor / total, 2),
                             "gbs_at_persistent": round(floor_gbs, 1), "copy_gbs": round(copy_gbs, 1),
                             "frac_of_copy": round(floor_gbs / copy_gbs, 4)}
+    # the copy alone, as a fraction of the same 16 B/sample roofline: the
+    # bound of any kernel that reads and writes these two buffers once
+    out["copy_same_buffers"] = {"ms": round(copy_ms, 4), "frac": round(copy_gbs / HBM_PEAK_GBS, 4)}
     del x, y
     return out
 

@@ -58,5 +58,7 @@ WGFFT_REGISTER_ADJOINT_SPLIT2(16384, 16, 2, 4, 14, ::wgfft::kRoleFused, 16x8x4x1
 // 4096 x 4096: the VJP in split form with the input prefetch takes 69.3 us
 // against 81.6-84.8 us for the one-footprint kernels (384 MiB moved: 5.8
 // TB/s); as the fused kernel or the filter the split form measured slower
-// (fused 80.4-124.6 against 75.7 us, filter 62.3 against 56.9 us).
+// (fused 80.4-124.6 against 75.7 us, filter 62.3 against 56.9 us; with x
+// kept in registers and read through the caches, as at 8192: 77.3-77.7 us
+// against 70.5 us for the default, two-wave register budget, 213-248 VGPRs).
 WGFFT_REGISTER_ADJOINT_SPLIT2(4096, 16, 2, 16, 1, ::wgfft::kRoleVjp, 16x8x16, 16, 8, 16);

@@ -49,6 +49,7 @@ WGFFT_REGISTER_GLOBAL_PASS_DIRECT(4096, 16, 2, 0, 16x16x16, 16, 16, 16);
 
 // 16-column direct tiles (128-byte runs) with the LDS of 8 columns: the
 // sub-FFTs run in two rounds.  (Four rounds of 4 columns at E = 16 were
-// measured slower at radix 2048.)
+// measured slower at radix 2048, and so were two rounds at E = 16: 1024
+// threads capped at 128 VGPRs spill 18, 573 against 532 us at 2^22 x 16.)
 WGFFT_REGISTER_GLOBAL_PASS_WIDE(1024, 16, 2, 16x4x16, 16, 4, 16);
 WGFFT_REGISTER_GLOBAL_PASS_WIDE(2048, 32, 2, 16x8x16, 16, 8, 16);

@@ -39,6 +39,8 @@ WGFFT_REGISTER_BATCHED_F(4096, 16, 1, 513, 16x16x16, 16, 16, 16);
 // default); 14 of 16 slots need 132 VGPRs (three workgroups per CU).  The
 // half prefetch measured 1-3% slower at 256 ... 2048 (G = 2 ... 16
 // transforms per workgroup), and with LDS twiddles or without the priority
-// at 4096.
+// at 4096; the three-quarter prefetch with ordinary stores took 41.7 against
+// 40.6 us on one buffer pair and 52.0 against 50.1 us over four rotating
+// pairs.
 WGFFT_REGISTER_BATCHED_F(4096, 16, 1, 2561, 16x16x16, 16, 16, 16);
 WGFFT_REGISTER_BATCHED_F(4096, 16, 1, 6657, 16x16x16, 16, 16, 16);

@@ -233,11 +233,9 @@ def bench_mixed(torch, gpu, wgfft, steps: int) -> dict:
                      "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     out["kernel"] = plan.describe()
     plan.close()
-    # The traffic floor of this design: 16 B/sample, plus 16 B/sample of
-    # scratch written and re-read for the two-pass lengths (2^15, 2^16), set
-    # beside a device copy over the same two buffers.
-    two_pass = int(sum(1 << int(v) for v in k if int(v) > 14))
-    floor = 16 * int(total) + 16 * two_pass
+    # the traffic floor of this design set beside a device copy over the
+    # same two buffers
+    floor = traffic_floor_bytes(k)
     copy_ms = timed(torch, lambda: y.copy_(x), 20, 3)
     copy_gbs = 16.0 * total / (copy_ms * 1e-3) / 1e9
     floor_gbs = floor / (out["persistent"]["ms"] * 1e-3) / 1e9
@@ -249,6 +247,20 @@ def bench_mixed(torch, gpu, wgfft, steps: int) -> dict:
     out["copy_same_buffers"] = {"ms": round(copy_ms, 4), "frac": round(copy_gbs / HBM_PEAK_GBS, 4)}
     del x, y
     return out
+
+
+def traffic_floor_bytes(log2n) -> int:
+    """HBM bytes one launch of the mixed-length kernel moves at least: 16 B
+    per sample, plus 16 B per sample of scratch written and re-read for the
+    lengths above 2^14, which run as two passes through a per-workgroup slot
+    (the lengths up to 2^14 stay in the LDS)."""
+    return int(sum((32 if int(k) > 14 else 16) << int(k) for k in log2n))
+
+
+def pass_count(plan: str) -> int:
+    """Global passes of a multipass plan description ("... passes=R2048(...),R2048(...)"): each pass
+    reads and writes every sample once."""
+    return len(re.findall(r"R\d+\(", plan)) or 1
 
 
 def _ragged_log2(count, kmin, kmax, seed):
@@ -442,9 +454,9 @@ def bench_large(torch, gpu, steps: int, tune: bool) -> dict:
         s = n * batch
         gbs = 16.0 * s / (ms * 1e-3) / 1e9
         plan = gpu.describe(n)
-        # each global pass reads and writes every sample once: the passes
-        # together move 16 B/sample per pass, set beside a device copy of x
-        passes = len(re.findall(r"R\d+\(", plan)) or 1
+        # the passes together move 16 B/sample per pass, set beside a device
+        # copy of x
+        passes = pass_count(plan)
         copy_ms = timed(torch, lambda: y.copy_(x), 20, 2)
         copy_gbs = 16.0 * s / (copy_ms * 1e-3) / 1e9
         pass_gbs = passes * gbs

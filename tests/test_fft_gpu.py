@@ -150,6 +150,38 @@ def test_every_first_pass_variant_windowed(log2n, batch):
             g.close()
 
 
+@pytest.mark.parametrize("n,batch", [(8192, 256), (4096, 512), (16384, 64)])
+def test_autotuned_windowed_entry(n, batch):
+    """The autotuner keeps the windowed entry's fastest variant apart when it
+    is not the plain transform's (reported after "; windowed: "): the windowed
+    call then runs that variant (bit-identical to pinning it), matches the
+    FP64 transform of w * x, and pinning a variant drops the separate choice."""
+    x = oracle.synth_uniform((batch, n), 0x5EED0270 + n)
+    w, _ = oracle.synth_window_mask(n, 0x5EED0271 + n)
+    want = oracle.windowed_fwd64(x[:4], w)
+    g = wgfft.GpuFft.new()
+    try:
+        desc = g.autotune(n, batch)
+        y = np.stack(g.fft_windowed(list(x), w))
+        assert oracle.rel_l2(y[:4], want) <= rel_bar(n), desc
+        plain = desc.split(";")[0]
+        chosen = desc.split("; windowed: ")[1].split(";")[0] if "; windowed: " in desc else plain
+        pinned = wgfft.GpuFft.new()
+        try:
+            names = [pinned.select_variant(n, i) for i in range(wgfft.GpuFft.variant_count(n))]
+            assert chosen in names, (chosen, names)
+            pinned.select_variant(n, names.index(chosen))
+            assert np.array_equal(np.stack(pinned.fft_windowed(list(x), w)), y)
+        finally:
+            pinned.close()
+        assert g.describe(n).startswith(plain)
+        g.select_variant(n, 0)
+        assert "; windowed: " not in g.describe(n)
+        assert oracle.rel_l2(np.stack(g.fft_windowed(list(x[:4]), w)), want) <= rel_bar(n)
+    finally:
+        g.close()
+
+
 # The autotuner's picks on MI355X at about 2^24 samples per call, which a
 # handle uses without tuning (a typo in the runtime's table would silently
 # fall back to the first registered variant).

@@ -380,6 +380,7 @@ int Handle::get_plan(uint64_t n, Plan** out) {
         p.var = pinned ? tuned[p.log2n] : default_variant(p.log2n);
         if (!p.var) return fail(WGFFT_ERR_INVALID_LENGTH, "no kernel variant for this length");
         if (!pinned) p.var_small = small_batch_variant(p.log2n);
+        if (pinned && tuned_win.count(p.log2n)) p.var_win = tuned_win[p.log2n];
         if ((rc = upload_table(twiddle_table(n, n, 1), &p.tw))) return rc;
     } else {
         if (p.log2n > 39) return fail(WGFFT_ERR_INVALID_LENGTH, "transform length above 2^39 is not supported");
@@ -426,6 +427,7 @@ int Handle::run(Plan* p, Mode mode, const float2* in, float2* out, const float* 
     if (p->kind == Plan::kBatched) {
         const BatchedVariant* v = force                                                      ? force
                                   : p->var_small && batch <= kSmallBatchPerCu * (uint64_t)num_cus ? p->var_small
+                                  : mode == kWindowed && p->var_win                              ? p->var_win
                                                                                                  : p->var;
         int grid = blocks_per_cu(v, mode, fixed == nullptr) * num_cus;
         BatchedArgs a{in, out, p->tw, win, batch, scale, grid, s, idist, odist};

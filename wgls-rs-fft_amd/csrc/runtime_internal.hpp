@@ -40,6 +40,9 @@ struct Plan {
     // kBatched, untuned plans only: the one-transform-per-workgroup variant
     // used for batches of at most kSmallBatchPerCu transforms per CU
     const BatchedVariant* var_small = nullptr;
+    // kBatched, tuned plans only: the windowed entry's variant when the
+    // autotuner found another one faster for it than var (nullptr: var)
+    const BatchedVariant* var_win = nullptr;
     float2* tw = nullptr;                 // kBatched: W_N^j, j < N
     std::vector<int> pass_log2r;          // kMultiPass
     std::vector<const GlobalPassVariant*> pass_var;
@@ -84,6 +87,7 @@ struct Handle {
     std::mutex mu;
     std::map<uint64_t, Plan> plans;
     std::map<int, const BatchedVariant*> tuned;  // log2n -> autotuned variant
+    std::map<int, const BatchedVariant*> tuned_win;  // log2n -> windowed-mode variant, where it differs
     std::map<int, const AdjointVariant*> tuned_adj;     // fused forward + adjoint
     std::map<int, const AdjointVariant*> tuned_vjp;     // VJP
     std::map<int, const AdjointVariant*> tuned_filter;  // spectral filter

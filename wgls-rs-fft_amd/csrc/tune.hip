@@ -261,6 +261,12 @@ extern "C" int wgfft_plan_describe(wgfft_t* h, uint64_t n, char* desc, uint64_t 
                                      std::to_string(kSmallBatchPerCu * (uint64_t)h->num_cus) + ": " + small;
             std::snprintf(desc, desc_len, "%s", full.c_str());
         }
+        if (p->var_win) {
+            char win[160];
+            describe(p->var_win, win, sizeof win);
+            const std::string full = std::string(desc) + "; windowed: " + win;
+            std::snprintf(desc, desc_len, "%s", full.c_str());
+        }
     } else {
         std::snprintf(desc, desc_len, "%s", describe_multipass(p).c_str());
     }
@@ -284,6 +290,9 @@ static int autotune_locked(Handle* h, uint64_t n, uint64_t batch, char* desc, ui
     }
     if ((rc = h->ensure(h->io_in, bytes)) || (rc = h->ensure(h->io_out, bytes))) return rc;
     if ((rc = fill_noise(h, h->io_in.p, bytes))) return rc;
+    // a window for the windowed entry's sweep (also the fused kernels' w and m below)
+    if ((rc = h->ensure(h->io_w, 2 * n * sizeof(float) + 64))) return rc;
+    if ((rc = fill_noise(h, h->io_w.p, 2 * n * sizeof(float) + 64))) return rc;
     Timer tm(h->stream);
     const BatchedVariant* best = nullptr;
     float best_ms = 1e30f;
@@ -291,6 +300,10 @@ static int autotune_locked(Handle* h, uint64_t n, uint64_t batch, char* desc, ui
     auto launch = [&] {
         return h->run(p, kForward, (const float2*)h->io_in.p, (float2*)h->io_out.p, nullptr, batch, 1.0f, h->stream,
                       cur);
+    };
+    auto launch_win = [&] {
+        return h->run(p, kWindowed, (const float2*)h->io_in.p, (float2*)h->io_out.p, (const float*)h->io_w.p, batch,
+                      1.0f, h->stream, cur);
     };
     if ((rc = tm.settle(launch))) return rc;
     int first_err = WGFFT_OK;
@@ -301,8 +314,12 @@ static int autotune_locked(Handle* h, uint64_t n, uint64_t batch, char* desc, ui
     // two sweeps in opposite orders, each candidate keeping its faster mean
     // (clock drift along one sweep favours whatever it times last; with
     // best-burst scores the sweep kept a plan 5-9% slower than the default
-    // at 8192 x 1024)
-    std::vector<float> cand_ms(cands.size(), 1e30f);
+    // at 8192 x 1024).  The windowed entry is timed beside the plain one and
+    // keeps its own fastest: its window loads change the register budget, and
+    // the plan fastest for one mode is not always so for the other (8192 x
+    // 1024: 2x8x8x8x8 forward 22.6 us, windowed 25.2 us; 2x16x16x16 23.7 and
+    // 22.9 us).
+    std::vector<float> cand_ms(cands.size(), 1e30f), cand_win(cands.size(), 1e30f);
     for (int sweep = 0; sweep < 2; ++sweep)
         for (size_t i = 0; i < cands.size(); ++i) {
             const size_t j = sweep ? cands.size() - 1 - i : i;
@@ -316,20 +333,27 @@ static int autotune_locked(Handle* h, uint64_t n, uint64_t batch, char* desc, ui
                 continue;
             }
             cand_ms[j] = std::min(cand_ms[j], ms);
+            if (tm.time(launch_win, &ms) == WGFFT_OK) cand_win[j] = std::min(cand_win[j], ms);
         }
-    for (size_t j = 0; j < cands.size(); ++j)
+    const BatchedVariant* best_win = nullptr;
+    float best_win_ms = 1e30f;
+    for (size_t j = 0; j < cands.size(); ++j) {
         if (cand_ms[j] < best_ms) {
             best_ms = cand_ms[j];
             best = cands[j];
         }
+        if (cand_win[j] < best_win_ms) {
+            best_win_ms = cand_win[j];
+            best_win = cands[j];
+        }
+    }
     if (!best) return fail(first_err ? first_err : WGFFT_ERR_INVALID_LENGTH, "autotune: no variant ran (" + first_msg + ")");
     // fused kernels for this length, timed per role: the fused forward +
     // adjoint, the VJP and the spectral filter each keep their own fastest
     std::string adj_desc;
     const auto adjs = adjoint_variants(p->log2n);
     if (adjs.size() > 1) {
-        if ((rc = h->ensure(h->io_aux, bytes)) || (rc = h->ensure(h->io_w, 2 * n * sizeof(float) + 64))) return rc;
-        if ((rc = fill_noise(h, h->io_w.p, 2 * n * sizeof(float) + 64))) return rc;
+        if ((rc = h->ensure(h->io_aux, bytes))) return rc;
         float* w = (float*)h->io_w.p;
         float* m = w + n;
         float* loss = m + n;
@@ -393,9 +417,20 @@ static int autotune_locked(Handle* h, uint64_t n, uint64_t batch, char* desc, ui
         h->tuned[p->log2n] = best;
         p->var = best;
         p->var_small = nullptr;
+        p->var_win = best_win && best_win != best ? best_win : nullptr;
+        if (p->var_win)
+            h->tuned_win[p->log2n] = p->var_win;
+        else
+            h->tuned_win.erase(p->log2n);
         describe(best, desc, desc_len);
-        if (desc && desc_len && !adj_desc.empty()) {
-            std::string full = std::string(desc) + adj_desc;
+        if (desc && desc_len) {
+            std::string full = desc;
+            if (p->var_win) {
+                char wbuf[160];
+                describe(p->var_win, wbuf, sizeof wbuf);
+                full += std::string("; windowed: ") + wbuf;
+            }
+            full += adj_desc;
             std::snprintf(desc, desc_len, "%s", full.c_str());
         }
         if (const char* path = std::getenv("WGFFT_TUNE_FILE")) {
@@ -448,8 +483,10 @@ extern "C" int wgfft_select_variant(wgfft_t* h, uint64_t n, int index, char* des
         if (v.log2n != p->log2n) continue;
         if (i++ == index) {
             h->tuned[p->log2n] = &v;
+            h->tuned_win.erase(p->log2n);
             p->var = &v;
             p->var_small = nullptr;
+            p->var_win = nullptr;
             describe(&v, desc, desc_len);
             return WGFFT_OK;
         }

@@ -195,7 +195,7 @@ DEFAULT_PLANS = {
     11: 'plan=16x8x16 E=16 G=2 prio',
     12: 'plan=16x16x16 E=16 G=1 nt-store prio 3/4-prefetch',
     13: 'plan=2x16x16x16 E=16 G=1 split2',
-    14: 'plan=2x16x8x4x16 E=16 G=1 split2',
+    14: 'plan=2x16x8x4x16 E=16 G=1 split2 cached-loads',
     15: 'passes=R256(16x16,G32,direct),R128(4x8x4,G32,direct)',
     16: 'passes=R256(16x16,G32,direct),R256(16x16,G32,direct)',
     17: 'passes=R512(8x8x8,G16,direct),R256(16x16,G32,direct)',

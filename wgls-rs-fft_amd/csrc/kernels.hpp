@@ -47,6 +47,7 @@ constexpr int kFlagSplit2 = 256;    // k_split2: a radix-2 stage, then the two h
 constexpr int kFlagPrio = 512;      // raised wave priority while issuing the prefetch loads and the stores
 constexpr int kFlagHalfPrefetch = 2048;  // prefetch only the first E/2 slots of the next transform
 constexpr int kFlagQuarterLoad = 4096;   // with kFlagHalfPrefetch: prefetch 3E/4 slots, load the last E/4 in turn
+constexpr int kFlagCachedLoad = 8192;    // k_split2: input read with ordinary loads (else non-temporally)
 // bits 4..7: minimum waves per SIMD the register allocation must allow (0: compiler's choice)
 constexpr int flag_min_waves(int fl) { return (fl >> 4) & 15; }
 
@@ -1054,18 +1055,22 @@ struct GlobalPassWideKernel {
 // i RF + m is index t + T i + m H/RF of both halves (RF the first radix).
 // The wave priority is raised while it issues the global loads and stores
 // (at 8192 x 1024, four workgroups per CU: 23-24 against 24-25 us).  The
-// input is read with ordinary loads, as in k_batched, so that a working set
-// that fits the 256 MB memory-side cache is served from it when read again:
-// with one 256 MiB buffer pair 48.7 against 52.1 us at 16384 x 1024 and
-// 44.6 against 50.8 us at 8192 x 2048 for non-temporal loads; over four
-// rotating pairs (1 GiB) the non-temporal loads are ~1.5% faster.
-template <int N, int E, int MODE, int... Rs>
+// input is read non-temporally, or with ordinary loads under
+// kFlagCachedLoad; which is faster depends on the working set and the
+// 256 MB memory-side cache (same box, one buffer pair re-read each call):
+// 8192 x 1024 (128 MiB) 21.2-22.3 us non-temporal against 23.5-23.7 us
+// ordinary, 8192 x 2048 (256 MiB) 50.8 against 44.6 us, 16384 x 1024
+// 52.1 against 48.7 us; over four rotating 256 MiB pairs non-temporal loads
+// are ~1.5% faster.  Both forms are registered and the autotuner picks per
+// shape.
+template <int N, int E, int FL, int MODE, int... Rs>
 __global__ void __launch_bounds__((N / 2) / E) __attribute__((amdgpu_waves_per_eu(4)))
 k_split2(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
          const float* __restrict__ win, uint64_t batch, float scale, uint64_t idist, uint64_t odist) {
     constexpr int H = N / 2;
     constexpr int T = H / E;
     constexpr bool INV = MODE == kInverse;
+    constexpr bool CACHED = (FL & kFlagCachedLoad) != 0;
     using MP = GdMap<H, E, 1, Rs...>;
     using EN = Engine<H, E, T, INV>;
     __shared__ __align__(16) float2 lds[LdsLayoutSingle<H, E, Rs...>::len];
@@ -1082,8 +1087,14 @@ k_split2(const float2* __restrict__ in, float2* __restrict__ out, const float2* 
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const uint32_t n = MP::row(e, t);
-            const f2v va = *reinterpret_cast<const f2v*>(x + n);
-            const f2v vb = *reinterpret_cast<const f2v*>(x + n + H);
+            f2v va, vb;
+            if constexpr (CACHED) {
+                va = *reinterpret_cast<const f2v*>(x + n);
+                vb = *reinterpret_cast<const f2v*>(x + n + H);
+            } else {
+                va = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(x + n));
+                vb = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(x + n + H));
+            }
             y0[e] = make_float2(va.x, va.y);
             y1[e] = make_float2(vb.x, vb.y);
         }
@@ -1122,7 +1133,7 @@ k_split2(const float2* __restrict__ in, float2* __restrict__ out, const float2* 
     }
 }
 
-template <int N, int E, int... Rs>
+template <int N, int E, int FL, int... Rs>
 struct Split2Kernel {
     static constexpr int T = (N / 2) / E;
     static constexpr int THREADS = T;
@@ -1133,7 +1144,7 @@ struct Split2Kernel {
         uint64_t grid = a.grid > 0 ? (uint64_t)a.grid : a.batch;
         if (grid > a.batch) grid = a.batch;
         if (grid == 0) return hipSuccess;
-        hipLaunchKernelGGL((k_split2<N, E, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0, a.stream, a.in,
+        hipLaunchKernelGGL((k_split2<N, E, FL, MODE, Rs...>), dim3((unsigned)grid), dim3(THREADS), 0, a.stream, a.in,
                            a.out, a.tw, a.win, a.batch, a.scale, a.idist ? a.idist : (uint64_t)N,
                            a.odist ? a.odist : (uint64_t)N);
         return hipGetLastError();
@@ -1148,7 +1159,7 @@ struct Split2Kernel {
     template <int MODE>
     static int occ() {
         int blocks = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_split2<N, E, MODE, Rs...>, THREADS,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)k_split2<N, E, FL, MODE, Rs...>, THREADS,
                                                          0) != hipSuccess)
             return 1;
         return blocks > 0 ? blocks : 1;
@@ -1163,11 +1174,14 @@ struct Split2Kernel {
 };
 
 // A k_split2 variant of length N; PLAN is the radix split of N/2.
-#define WGFFT_REGISTER_SPLIT2(N, E, PLAN, ...)                                                            \
-    static ::wgfft::Registrar _wgfft_split2_##N##_##E##_##PLAN(::wgfft::BatchedVariant{                 \
-        ::wgfft::ilog2(N), E, 1, ::wgfft::kFlagSplit2, ::wgfft::Split2Kernel<N, E, __VA_ARGS__>::THREADS, \
-        ::wgfft::Split2Kernel<N, E, __VA_ARGS__>::LDS_BYTES, "2x" #PLAN,                                 \
-        &::wgfft::Split2Kernel<N, E, __VA_ARGS__>::launch, &::wgfft::Split2Kernel<N, E, __VA_ARGS__>::blocks_per_cu})
+// FL: 0 or kFlagCachedLoad (8192).
+#define WGFFT_REGISTER_SPLIT2_F(N, E, FL, PLAN, ...)                                                          \
+    static ::wgfft::Registrar _wgfft_split2_##N##_##E##_##FL##_##PLAN(::wgfft::BatchedVariant{                \
+        ::wgfft::ilog2(N), E, 1, ::wgfft::kFlagSplit2 | (FL), ::wgfft::Split2Kernel<N, E, FL, __VA_ARGS__>::THREADS, \
+        ::wgfft::Split2Kernel<N, E, FL, __VA_ARGS__>::LDS_BYTES, "2x" #PLAN,                                  \
+        &::wgfft::Split2Kernel<N, E, FL, __VA_ARGS__>::launch,                                              \
+        &::wgfft::Split2Kernel<N, E, FL, __VA_ARGS__>::blocks_per_cu})
+#define WGFFT_REGISTER_SPLIT2(N, E, PLAN, ...) WGFFT_REGISTER_SPLIT2_F(N, E, 0, PLAN, __VA_ARGS__)
 
 // Persistent multi-pass kernel (tiles read and written non-temporally: each
 // element is touched once per pass): workgroups loop over the groups of all

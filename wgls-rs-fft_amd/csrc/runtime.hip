@@ -187,7 +187,7 @@ const DefaultBatched kDefaultBatched[] = {
     {5, "8x4", 16, 128, 0},       {6, "8x8", 16, 64, 0},          {7, "4x4x8", 16, 32, 0},
     {8, "8x4x8", 16, 16, 0},      {9, "8x8x8", 16, 8, 0},         {10, "16x4x16", 16, 4, kFlagPrio},
     {11, "16x8x16", 16, 2, kFlagPrio},    {12, "16x16x16", 16, 1, kFlagNtStore | kFlagPrio | kFlagHalfPrefetch | kFlagQuarterLoad},
-    {13, "2x16x16x16", 16, 1, kFlagSplit2},                        {14, "2x16x8x4x16", 16, 1, kFlagSplit2},
+    {13, "2x16x16x16", 16, 1, kFlagSplit2},                        {14, "2x16x8x4x16", 16, 1, kFlagSplit2 | kFlagCachedLoad},
 };
 
 // Default multipass plans of 2^15 .. 2^26, picked the same way (1.2x to 5.8x

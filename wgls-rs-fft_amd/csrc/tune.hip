@@ -151,14 +151,14 @@ static void describe(const BatchedVariant* v, char* desc, uint64_t len) {
     if (!desc || !len) return;
     char waves[24] = "";
     if (flag_min_waves(v->flags)) std::snprintf(waves, sizeof waves, " waves>=%d", flag_min_waves(v->flags));
-    std::snprintf(desc, len, "lds N=%d plan=%s E=%d G=%d%s%s%s%s%s threads=%d lds=%dB", 1 << v->log2n, v->plan, v->E,
+    std::snprintf(desc, len, "lds N=%d plan=%s E=%d G=%d%s%s%s%s%s%s threads=%d lds=%dB", 1 << v->log2n, v->plan, v->E,
                   v->G, (v->flags & kFlagNtStore) ? " nt-store" : "", (v->flags & kFlagPrio) ? " prio" : "",
                   (v->flags & kFlagTwRegs) ? " tw-regs" : (v->flags & kFlagTwLds) ? " tw-lds" : "",
                   (v->flags & kFlagNoPrefetch)     ? " no-prefetch"
                   : (v->flags & kFlagHalfPrefetch) ? ((v->flags & kFlagQuarterLoad) ? " 3/4-prefetch" : " half-prefetch")
                   : (v->flags & kFlagSplit2)       ? " split2"
                                                    : "",
-                  waves,
+                  (v->flags & kFlagCachedLoad) ? " cached-loads" : "", waves,
                   v->threads, v->lds_bytes);
 }
 

@@ -51,6 +51,11 @@ __device__ __forceinline__ void load_real_pairs(float (&r)[E], const float* __re
 //                transform waits on a global twiddle load.
 // (A register prefetch of the next transform's input on top of kAdjTwLds was
 // measured slower at N=4096: the extra registers cost a workgroup per CU.)
+// Y and gx are written once and not read back here, so they are stored
+// non-temporally, as in the forward kernels (4096 x 4096, default fused
+// variant on one box: 66.6-66.9 us against 71.5-72.5 us with ordinary
+// stores, with the Y output 77.6-77.9 against 95.2-96.0 us; the split form's
+// Y at 8192 x 1024: 53.7-54.0 against 55.6-56.7 us).
 constexpr int kAdjKeepX = 1, kAdjTwLds = 2, kAdjSplit2 = 4;
 // AdjointVariant::roles: the calls a variant is the untuned default for
 // (the first registered variant of a length serves any role none claims).
@@ -103,7 +108,7 @@ k_adjoint(AdjointArgs a) {
                 Engine<N, E, T, false>::run(d, lds, a.tw, t, Radices<Rs...>{});
             // keep the epilogue's loads out of the transform's register peak
             __builtin_amdgcn_sched_barrier(0);
-            if (a.y) IO::template store<false>(a.y + tr * N, d, t, 1.f);
+            if (a.y) IO::template store<false, true>(a.y + tr * N, d, t, 1.f);
             // gY = 2 m Y; loss += m |Y|^2
             float mv[E];
             IO::load_real(mv, a.mask, t);
@@ -148,7 +153,7 @@ k_adjoint(AdjointArgs a) {
             d[i] = cscale(d[i], wv[i]);
         }
         __builtin_amdgcn_s_setprio(2);  // and its stores
-        IO::template store<false>(a.gx + tr * N, d, ta, 1.f);
+        IO::template store<false, true>(a.gx + tr * N, d, ta, 1.f);
         __builtin_amdgcn_s_setprio(0);
         WGFFT_LDS_SYNC();
     }
@@ -357,7 +362,7 @@ k_adjoint_split2(AdjointArgs a) {
             for (int e = 0; e < E; ++e) {
                 const uint32_t k = MP::row(e, t1);
                 if constexpr (STOREY)
-                    st16<false>(reinterpret_cast<float4*>(a.y + tr * N + 2 * k),
+                    st16<true>(reinterpret_cast<float4*>(a.y + tr * N + 2 * k),
                                 make_float4(y0[e].x, y0[e].y, y1[e].x, y1[e].y));
                 const float2 mv = m2[k];
                 lacc += mv.x * (y0[e].x * y0[e].x + y0[e].y * y0[e].y) + mv.y * (y1[e].x * y1[e].x + y1[e].y * y1[e].y);
